@@ -1,0 +1,245 @@
+/* tatva_amd: C ABI of the MI355X (gfx950) element-assembly / colored-Hessian /
+ * matrix-free-HVP / Jacobi-PCG path.
+ *
+ * Conventions
+ *   - Every function returns an int status (TATVA_OK == 0); on failure the
+ *     message is available from tatva_last_error() (thread-local).
+ *   - Device buffers are plain pointers owned by the caller, sizes are element
+ *     counts.  Compute calls are asynchronous on the caller's hipStream_t
+ *     (passed as void*), except the PCG solvers, which return after the solve.
+ *   - Vectors are node-major interleaved: dof = node * ncomp + component, as in
+ *     tatva's flat state vectors.
+ *   - "dof_map" arguments map a full dof to its reduced (free) index, or -1 for a
+ *     constrained dof (Lifter/Fixed); NULL means the identity.
+ *   - CSR arrays are int32 (nnz < 2^31), sorted column indices per row.
+ *   - One mesh handle must not be used concurrently from two streams: it owns
+ *     the element scratch buffer and reduction counters its kernels share.
+ *
+ * Each declaration cites the reference function (tatva v0.11.1, path relative
+ * to the tatva repository) whose behaviour it replaces.
+ */
+#ifndef TATVA_AMD_H
+#define TATVA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TATVA_OK = 0,
+  TATVA_EINVAL = 1,   /* invalid argument (mirrors ValueError / TypeError)   */
+  TATVA_EHIP = 2,     /* HIP runtime error                                    */
+  TATVA_ENOMEM = 3,   /* allocation failed / workspace too small              */
+  TATVA_EBREAKDOWN = 4, /* CG breakdown: p.Ap <= 0 or non-finite              */
+  TATVA_ENOTCONV = 5  /* iteration limit reached before the tolerance         */
+};
+
+enum { TATVA_TRI3 = 0, TATVA_TET4 = 1, TATVA_HEX8 = 2 };
+
+enum {
+  TATVA_DENSITY_POISSON = 1,        /* psi = |grad u|^2/2 - f u        (p0 = f)           */
+  TATVA_DENSITY_LINEAR_ELASTIC = 2, /* psi = sigma:eps/2               (p0 = mu, p1 = lambda) */
+  TATVA_DENSITY_NEO_HOOKEAN = 3     /* psi = mu/2(I1-d-2lnJ)+lambda/2 ln^2 J (p0 = mu, p1 = lambda) */
+};
+
+/* Built-in energy density descriptor (replaces the user JAX density callables of
+ * tests/test_sparse.py:20-38 and tests/test_sparse_tracer.py:103-115). */
+typedef struct {
+  int32_t kind;
+  int32_t reserved;
+  double p0, p1, p2;
+} tatva_density;
+
+typedef struct tatva_mesh tatva_mesh;
+
+/* PCG result. status: TATVA_OK, TATVA_EBREAKDOWN or TATVA_ENOTCONV. */
+typedef struct {
+  int32_t iterations;
+  int32_t status;
+  double rel_residual;    /* ||r_k|| / ||b|| of the recursively updated residual */
+  double b_norm;
+  double op_ms;           /* with profiling on: summed duration of the sampled operator launches */
+  int32_t op_samples;     /* number of sampled operator launches (0 with profiling off) */
+  int32_t reserved;
+} tatva_pcg_stats;
+
+const char *tatva_last_error(void);
+int tatva_version(void);
+int tatva_device_count(int *out);
+
+/* Profiling switch (no counterpart in the reference): when on, each captured
+ * PCG chunk brackets its first operator application (SpMV or matrix-free HVP)
+ * with HIP events on the solver stream and reports the summed duration in
+ * tatva_pcg_stats.op_ms / op_samples. */
+int tatva_set_profiling(int enable);
+
+/* ---------------------------------------------------------------- mesh ---- */
+
+/* Uploads a mesh (host arrays) and builds its node->element incidence.
+ * Replaces Mesh(coords, elements) (tatva/mesh.py:53-85) + Operator.__post_init__
+ * / __check_init__ validation (tatva/operator.py:112-170).  quad_points (nq*dim)
+ * and quad_weights (nq) give the element quadrature (element/base.py:37-51). */
+int tatva_mesh_create(int device, int elem_kind, const double *coords, int64_t n_nodes,
+                      int dim, const int32_t *elements, int64_t n_elements, int nne, int nq,
+                      const double *quad_points, const double *quad_weights,
+                      tatva_mesh **out);
+int tatva_mesh_destroy(tatva_mesh *mesh);
+
+/* Integration weights detJ*w_q, out (E, Q).
+ * Replaces Operator.get_integration_weights (tatva/operator.py:172-192). */
+int tatva_weights(tatva_mesh *mesh, double *out, void *stream);
+
+/* Values at quadrature points, out (E, Q, ncomp) from nodal u (N, ncomp).
+ * Replaces Operator.eval (tatva/operator.py:358-377). */
+int tatva_eval(tatva_mesh *mesh, int ncomp, const double *u, double *out, void *stream);
+
+/* Gradients at quadrature points, out (E, Q, ncomp, dim), out[...,i,j] = du_i/dx_j.
+ * Replaces Operator.grad (tatva/operator.py:379-397; element/base.py:99-115). */
+int tatva_grad(tatva_mesh *mesh, int ncomp, const double *u, double *out, void *stream);
+
+/* Integral of values given at quadrature points (E, Q, ncomp), or of a nodal
+ * field (N, ncomp) when nodal != 0.  per_element (E, ncomp) and total (ncomp)
+ * may be NULL.  Replaces Operator.integrate / integrate_per_element /
+ * _integrate_quad_array (tatva/operator.py:307-356). */
+int tatva_integrate(tatva_mesh *mesh, int ncomp, int nodal, const double *values,
+                    double *per_element, double *total, void *stream);
+
+/* Density values psi at n_points quadrature points from gradients
+ * (n_points, ncomp, dim) and, for Poisson, values (n_points).  Replaces the
+ * autovmap'ed density callables evaluated on Operator.grad output
+ * (tests/test_sparse.py:33-38). */
+int tatva_density_eval(int dim, int64_t n_points, const tatva_density *density,
+                       const double *grad, const double *values, double *psi, void *stream);
+
+/* Total energy E(u) = sum_e sum_q psi(grad u) detJ w - f_ext.u (f_ext may be NULL),
+ * written to out[0].  Replaces op.integrate(psi(op.grad(u))) (operator.py:307-319). */
+int tatva_energy(tatva_mesh *mesh, const tatva_density *density, const double *u,
+                 const double *f_ext, double *out, void *stream);
+
+/* Residual r = dE/du (jax.grad of the energy; the reference's scatter-add is
+ * the transpose of the gather at tatva/operator.py:221).  Deterministic node
+ * gather, no float atomics.  With dof_map, writes only free entries to
+ * r[dof_map[dof]] (Lifter.reduce_adjoint for Fixed, lifter/base.py:235-251). */
+int tatva_residual(tatva_mesh *mesh, const tatva_density *density, const double *u,
+                   const double *f_ext, const int32_t *dof_map, double *r, void *stream);
+
+/* Hessian-vector product y = H(u) v, FP64 (jax.jvp(jax.grad(E), (u,), (v,))[1],
+ * README.md:93; sparse/base.py:213,264).  u is a full vector; v and y are
+ * reduced vectors when dof_map != NULL (v lifted with zeros on constrained dofs). */
+int tatva_hvp(tatva_mesh *mesh, const tatva_density *density, const double *u,
+              const double *v, const int32_t *dof_map, double *y, void *stream);
+
+/* Same in FP32 (FP32 state, tangent and accumulation; geometry from FP64 coords). */
+int tatva_hvp_f32(tatva_mesh *mesh, const tatva_density *density, const float *u,
+                  const float *v, const int32_t *dof_map, float *y, void *stream);
+
+/* Diagonal of H(u) (Jacobi preconditioner), FP64 / FP32 output (reduced with dof_map). */
+int tatva_hessian_diag(tatva_mesh *mesh, const tatva_density *density, const double *u,
+                       const int32_t *dof_map, double *diag, void *stream);
+int tatva_hessian_diag_f32(tatva_mesh *mesh, const tatva_density *density, const float *u,
+                           const int32_t *dof_map, float *diag, void *stream);
+
+/* ------------------------------------------------------ colored Hessian ---- */
+
+/* Workspace bytes for tatva_colored_hessian on this mesh with ncomp components. */
+int64_t tatva_colored_hessian_workspace(tatva_mesh *mesh, int ncomp);
+
+/* Colored sparse Hessian decompressed into CSR data.  Replaces sparse.jacfwd /
+ * colored_jacobian_batch (tatva/sparse/base.py:139-176,230-270): one JVP per
+ * color with seed (colors == c), J_compressed[row, color], then
+ * data[k] = J_compressed[rows[k], colors[indices[k]]] (base.py:167-174).
+ * The pattern (indptr, indices) and colors refer to the reduced (free) dofs
+ * when dof_map != NULL; red_to_full (n_rows) then lists the full dof of each row. */
+int tatva_colored_hessian(tatva_mesh *mesh, const tatva_density *density, const double *u,
+                          const int32_t *dof_map, const int32_t *red_to_full, int64_t n_rows,
+                          const int32_t *colors, int32_t n_colors, const int32_t *indptr,
+                          const int32_t *indices, int64_t nnz, double *data, void *workspace,
+                          int64_t workspace_bytes, void *stream);
+
+/* rows[k] = row of entry k and col_colors[k] = colors[indices[k]] for a CSR
+ * pattern (compute_rows_cols, tatva/sparse/base.py:108-136). */
+int tatva_rows_cols(int64_t n, const int32_t *indptr, const int32_t *indices,
+                    const int32_t *colors, int32_t *rows, int32_t *col_colors, void *stream);
+
+/* ------------------------------------------------------- CSR and solvers -- */
+
+/* y = A x for a CSR matrix (FP64).  Replaces ColoredMatrix.to_bcsr() @ x
+ * (tatva/sparse/base.py:95-100). */
+int tatva_spmv_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
+                   const double *data, const double *x, double *y, void *stream);
+
+/* Jacobi-PCG on a CSR matrix in FP64 (no counterpart in the reference; it
+ * replaces the user-side linear solve, cf. spsolve in tatva/utils.py:186-198).
+ * x holds x0 on entry.  Iterations run in hipGraph-captured chunks of `chunk`. */
+int64_t tatva_pcg_csr_workspace(int64_t n);
+int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
+                  const double *data, const double *b, double *x, double tol, int32_t maxiter,
+                  int32_t chunk, void *workspace, int64_t workspace_bytes,
+                  tatva_pcg_stats *stats, void *stream);
+
+/* Jacobi-PCG with the matrix-free FP32 Hessian-vector product of (mesh, density)
+ * at the FP32 state u (full vector); b, x, diag are reduced FP32 vectors. */
+int64_t tatva_pcg_matfree_workspace(tatva_mesh *mesh, int64_t n);
+int tatva_pcg_matfree_f32(tatva_mesh *mesh, const tatva_density *density, const float *u,
+                          const int32_t *dof_map, int64_t n, const float *diag, const float *b,
+                          float *x, double tol, int32_t maxiter, int32_t chunk, void *workspace,
+                          int64_t workspace_bytes, tatva_pcg_stats *stats, void *stream);
+
+/* ------------------------------------------------------- vector kernels ---- */
+
+/* out_full[free[i]] = in_red[i] for i < n_red (Lifter.lift, lifter/base.py:201-221;
+ * the constrained entries of out_full are left as they are). */
+int tatva_scatter_free(int64_t n_red, const int32_t *free_dofs, const double *in_red,
+                       double *out_full, void *stream);
+/* out_full[dofs[i]] = values[i] (or scalar value if values == NULL)
+ * (Fixed.apply_lift, lifter/constraints.py:312-314). */
+int tatva_set_dofs(int64_t n, const int32_t *dofs, const double *values, double value,
+                   double *out_full, void *stream);
+/* out_red[i] = in_full[free[i]] (Lifter.reduce, lifter/base.py:231-233). */
+int tatva_gather_free(int64_t n_red, const int32_t *free_dofs, const double *in_full,
+                      double *out_red, void *stream);
+/* u_full[free[i]] += alpha * du[i] with du FP32 or FP64. */
+int tatva_axpy_free_f32(int64_t n_red, const int32_t *free_dofs, double alpha, const float *du,
+                        double *u_full, void *stream);
+int tatva_axpy_free(int64_t n_red, const int32_t *free_dofs, double alpha, const double *du,
+                    double *u_full, void *stream);
+/* y = x cast to FP32 / FP64, scaled by alpha. */
+int tatva_cast_f64_f32(int64_t n, double alpha, const double *x, float *y, void *stream);
+int tatva_cast_f32_f64(int64_t n, double alpha, const float *x, double *y, void *stream);
+/* out[0] = x . y (deterministic reduction, FP64). */
+int tatva_dot(int64_t n, const double *x, const double *y, double *out, void *stream);
+
+/* ------------------------------------------------ host-side setup (no GPU) -- */
+
+typedef struct tatva_csr_host tatva_csr_host;
+
+/* CSR sparsity pattern of the element connectivity with ndpn dofs per node:
+ * every pair of element dofs, deduplicated, sorted (tatva/sparse/_extraction.py:
+ * 37-102).  With dof_map (full dof -> reduced index or -1) the pattern is built
+ * directly on the reduced dofs (= S[free][:, free], lifter/base.py:322-331). */
+int tatva_pattern_from_mesh(const int32_t *elements, int64_t n_elements, int nne,
+                            int64_t n_nodes, int ndpn, const int32_t *dof_map, int64_t n_rows,
+                            tatva_csr_host **out);
+/* Row/column submatrix S[keep][:, keep] of a CSR pattern; keep_map: old index ->
+ * new index or -1, monotone on kept indices (Lifter.reduce_sparsity,
+ * lifter/base.py:322-331). */
+int tatva_csr_submatrix(int64_t n, const int32_t *indptr, const int32_t *indices,
+                        const int32_t *keep_map, int64_t n_new, tatva_csr_host **out);
+int64_t tatva_csr_host_rows(const tatva_csr_host *csr);
+int64_t tatva_csr_host_nnz(const tatva_csr_host *csr);
+int tatva_csr_host_copy(const tatva_csr_host *csr, int32_t *indptr, int32_t *indices);
+void tatva_csr_host_free(tatva_csr_host *csr);
+
+/* Natural-order greedy distance-2 coloring of a CSR pattern (the in-repo
+ * tatva/sparse/_coloring.py:27-48,136-153 algorithm used by ColoredMatrix.from_csr,
+ * tatva/sparse/base.py:59-74).  Writes colors (n) and the number of colors. */
+int tatva_distance2_colors(int64_t n, const int32_t *indptr, const int32_t *indices,
+                           int32_t *colors, int32_t *n_colors);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TATVA_AMD_H */

@@ -1,0 +1,158 @@
+// Built-in energy densities with hand-derived first derivative P = dpsi/d(grad u)
+// and its directional derivative dP = d^2psi/d(grad u)^2 [dG].
+//   Poisson:  psi = |G|^2/2 - f u            (scalar field, NC = 1)
+//   Linear elasticity (tests/test_sparse.py:20-38):
+//             eps = sym(G), sigma = 2 mu eps + lambda tr(eps) I, psi = sigma:eps/2
+//   Compressible Neo-Hookean (tests/test_sparse_tracer.py:103-115):
+//             F = I + G, J = det F, psi = mu/2 (F:F - D - 2 ln J) + lambda/2 (ln J)^2
+//             P  = mu (F - F^-T) + lambda ln J F^-T
+//             dP = mu dG + (mu - lambda ln J) F^-T dG^T F^-T + lambda tr(F^-1 dG) F^-T
+// G[c][j] = d u_c / d x_j.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "element.hpp"
+
+namespace tatva {
+
+template <int D, class T>
+struct Poisson {
+  static constexpr int NC = 1;
+  static constexpr bool kSource = true;
+  T f;
+  struct Lin {};
+  __device__ __forceinline__ T psi(const T G[NC][D], T uq) const {
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) s += G[0][j] * G[0][j];
+    return T(0.5) * s - f * uq;
+  }
+  __device__ __forceinline__ void P(const T G[NC][D], T out[NC][D]) const {
+#pragma unroll
+    for (int j = 0; j < D; ++j) out[0][j] = G[0][j];
+  }
+  __device__ __forceinline__ Lin linearize(const T (&)[NC][D]) const { return Lin{}; }
+  __device__ __forceinline__ void dP(const Lin &, const T dG[NC][D], T out[NC][D]) const {
+#pragma unroll
+    for (int j = 0; j < D; ++j) out[0][j] = dG[0][j];
+  }
+};
+
+template <int D, class T>
+struct LinearElastic {
+  static constexpr int NC = D;
+  static constexpr bool kSource = false;
+  T mu, lam;
+  struct Lin {};
+  __device__ __forceinline__ void stress(const T G[NC][D], T s[NC][D]) const {
+    T tr = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) tr += G[i][i];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) s[i][j] = mu * (G[i][j] + G[j][i]) + (i == j ? lam * tr : T(0));
+  }
+  __device__ __forceinline__ T psi(const T G[NC][D], T) const {
+    T s[NC][D];
+    stress(G, s);
+    T e = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) e += s[i][j] * T(0.5) * (G[i][j] + G[j][i]);
+    return T(0.5) * e;
+  }
+  __device__ __forceinline__ void P(const T G[NC][D], T out[NC][D]) const { stress(G, out); }
+  __device__ __forceinline__ Lin linearize(const T (&)[NC][D]) const { return Lin{}; }
+  __device__ __forceinline__ void dP(const Lin &, const T dG[NC][D], T out[NC][D]) const {
+    stress(dG, out);
+  }
+};
+
+template <class T>
+__device__ __forceinline__ T tlog(T x);
+template <>
+__device__ __forceinline__ double tlog<double>(double x) { return log(x); }
+template <>
+__device__ __forceinline__ float tlog<float>(float x) { return logf(x); }
+
+template <int D, class T>
+struct NeoHookean {
+  static constexpr int NC = D;
+  static constexpr bool kSource = false;
+  T mu, lam;
+  struct Lin {
+    T Fi[D][D];  // F^-1
+    T c1;        // mu - lambda ln J
+    T lnJ;
+  };
+  __device__ __forceinline__ Lin linearize(const T (&G)[NC][D]) const {
+    T F[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) F[i][j] = G[i][j] + (i == j ? T(1) : T(0));
+    Lin L;
+    const T J = det_inv(F, L.Fi);
+    L.lnJ = tlog<T>(J);
+    L.c1 = mu - lam * L.lnJ;
+    return L;
+  }
+  __device__ __forceinline__ T psi(const T G[NC][D], T) const {
+    T F[D][D];
+    T I1 = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        F[i][j] = G[i][j] + (i == j ? T(1) : T(0));
+        I1 += F[i][j] * F[i][j];
+      }
+    const T lnJ = tlog<T>(det_only(F));
+    return T(0.5) * mu * (I1 - T(D) - T(2) * lnJ) + T(0.5) * lam * lnJ * lnJ;
+  }
+  __device__ __forceinline__ void P(const T G[NC][D], T out[NC][D]) const {
+    T Gc[NC][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) Gc[i][j] = G[i][j];
+    const Lin L = linearize(Gc);
+    const T b = lam * L.lnJ - mu;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j)
+        out[i][j] = mu * (G[i][j] + (i == j ? T(1) : T(0))) + b * L.Fi[j][i];
+  }
+  __device__ __forceinline__ void dP(const Lin &L, const T dG[NC][D], T out[NC][D]) const {
+    // W = F^-1 dG, Z = W F^-1; (F^-T dG^T F^-T)_ij = Z_ji; tr(F^-1 dG) = tr W.
+    T W[D][D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        T s = T(0);
+#pragma unroll
+        for (int l = 0; l < D; ++l) s += L.Fi[j][l] * dG[l][k];
+        W[j][k] = s;
+      }
+    T tr = T(0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) tr += W[j][j];
+    const T lt = lam * tr;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        T z = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) z += W[j][k] * L.Fi[k][i];
+        out[i][j] = mu * dG[i][j] + L.c1 * z + lt * L.Fi[j][i];
+      }
+  }
+};
+
+}  // namespace tatva

@@ -1,0 +1,785 @@
+// CSR SpMV, Jacobi-PCG (FP64 CSR and FP32 matrix-free) with hipGraph-captured
+// iteration chunks, and the small vector kernels of the Newton loop.
+//
+// PCG iteration (one chunk step = 3 kernels for CSR, 4 for matrix-free):
+//   Ap = A p, p.Ap            -> last block: alpha = rz / p.Ap
+//   x += alpha p, r -= alpha Ap, z = D^-1 r, r.z, r.r
+//                             -> last block: convergence test, beta = rz'/rz
+//   p = D^-1 r + beta p
+// Scalars live in a device-side state record; every kernel returns at once
+// when the state says "done", so a captured chunk may overrun convergence
+// without changing the result.  Reductions use fixed trees: deterministic.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <atomic>
+#include <mutex>
+
+#include "common.hpp"
+#include "mesh.hpp"
+
+namespace tatva {
+
+struct PcgState {
+  double rz;
+  double alpha;
+  double beta;
+  double rr;
+  double bb;
+  double tol2;
+  int iter;
+  int done;
+  int status;
+  int maxiter;
+};
+
+// ----------------------------------------------------------------- SpMV ----
+
+// LPR lanes per row; rows processed in a grid-stride loop over row groups.
+template <int LPR>
+__device__ __forceinline__ double csr_row_dot(int64_t row, int lane, const int32_t *__restrict__ indptr,
+                                              const int32_t *__restrict__ indices,
+                                              const double *__restrict__ data,
+                                              const double *__restrict__ x) {
+  double s = 0.0;
+  const int32_t k1 = indptr[row + 1];
+  for (int32_t k = indptr[row] + lane; k < k1; k += LPR) s += data[k] * x[indices[k]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+  return s;
+}
+
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t *__restrict__ indptr,
+                                                 const int32_t *__restrict__ indices,
+                                                 const double *__restrict__ data,
+                                                 const double *__restrict__ x, double *__restrict__ y) {
+  constexpr int G = kBlock / LPR;
+  const int lane = threadIdx.x % LPR;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t stride = (int64_t)gridDim.x * G;
+  for (int64_t row = lb * G + threadIdx.x / LPR; row < n; row += stride) {
+    const double s = csr_row_dot<LPR>(row, lane, indptr, indices, data, x);
+    if (lane == 0) y[row] = s;
+  }
+}
+
+// Ap = A p fused with the p.Ap reduction; the last block sets alpha.
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv_dot(int64_t n, const int32_t *__restrict__ indptr,
+                                                         const int32_t *__restrict__ indices,
+                                                         const double *__restrict__ data,
+                                                         const double *__restrict__ p,
+                                                         double *__restrict__ Ap,
+                                                         double *__restrict__ partials,
+                                                         unsigned int *__restrict__ counter,
+                                                         PcgState *__restrict__ st) {
+  if (st->done) return;
+  constexpr int G = kBlock / LPR;
+  const int lane = threadIdx.x % LPR;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t stride = (int64_t)gridDim.x * G;
+  double dot = 0.0;
+  for (int64_t row = lb * G + threadIdx.x / LPR; row < n; row += stride) {
+    const double s = csr_row_dot<LPR>(row, lane, indptr, indices, data, p);
+    if (lane == 0) {
+      Ap[row] = s;
+      dot += s * p[row];
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = dot;
+  if (last_block(counter)) {
+    const double pAp = block_sum_array(partials, gridDim.x);
+    if (threadIdx.x == 0) {
+      if (!(pAp > 0.0) || !isfinite(pAp)) {
+        st->done = 1;
+        st->status = TATVA_EBREAKDOWN;
+      } else {
+        st->alpha = st->rz / pAp;
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+// p.Ap for the matrix-free operator (FP32 vectors, FP64 accumulation).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pcg_dot_alpha(int64_t n, const T *__restrict__ p,
+                                                          const T *__restrict__ Ap,
+                                                          double *__restrict__ partials,
+                                                          unsigned int *__restrict__ counter,
+                                                          PcgState *__restrict__ st) {
+  if (st->done) return;
+  double dot = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dot += (double)p[i] * (double)Ap[i];
+  dot = block_sum(dot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = dot;
+  if (last_block(counter)) {
+    const double pAp = block_sum_array(partials, gridDim.x);
+    if (threadIdx.x == 0) {
+      if (!(pAp > 0.0) || !isfinite(pAp)) {
+        st->done = 1;
+        st->status = TATVA_EBREAKDOWN;
+      } else {
+        st->alpha = st->rz / pAp;
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, T *__restrict__ x, T *__restrict__ r,
+                                                       const T *__restrict__ p,
+                                                       const T *__restrict__ Ap,
+                                                       const T *__restrict__ dinv,
+                                                       double *__restrict__ partials,
+                                                       unsigned int *__restrict__ counter,
+                                                       PcgState *__restrict__ st) {
+  if (st->done) return;
+  const T alpha = T(st->alpha);
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const T ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    rz += (double)ri * (double)(dinv[i] * ri);
+    rr += (double)ri * (double)ri;
+  }
+  rz = block_sum(rz);
+  rr = block_sum(rr);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = rz;
+    partials[kMaxReduceBlocks + blockIdx.x] = rr;
+  }
+  if (last_block(counter)) {
+    const double rz_new = block_sum_array(partials, gridDim.x);
+    const double rr_new = block_sum_array(partials + kMaxReduceBlocks, gridDim.x);
+    if (threadIdx.x == 0) {
+      st->iter += 1;
+      st->rr = rr_new;
+      if (rr_new <= st->tol2) {
+        st->done = 1;
+        st->status = TATVA_OK;
+      } else if (!isfinite(rr_new)) {
+        st->done = 1;
+        st->status = TATVA_EBREAKDOWN;
+      } else if (st->iter >= st->maxiter) {
+        st->done = 1;
+        st->status = TATVA_ENOTCONV;
+      } else {
+        st->beta = rz_new / st->rz;
+        st->rz = rz_new;
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict__ p,
+                                                        const T *__restrict__ r,
+                                                        const T *__restrict__ dinv,
+                                                        const PcgState *__restrict__ st) {
+  if (st->done) return;
+  const T beta = T(st->beta);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = dinv[i] * r[i] + beta * p[i];
+}
+
+// r = b - A x (CSR)
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void k_csr_residual(int64_t n, const int32_t *__restrict__ indptr,
+                                                         const int32_t *__restrict__ indices,
+                                                         const double *__restrict__ data,
+                                                         const double *__restrict__ x,
+                                                         const double *__restrict__ b,
+                                                         double *__restrict__ r) {
+  constexpr int G = kBlock / LPR;
+  const int lane = threadIdx.x % LPR;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t stride = (int64_t)gridDim.x * G;
+  for (int64_t row = lb * G + threadIdx.x / LPR; row < n; row += stride) {
+    const double s = csr_row_dot<LPR>(row, lane, indptr, indices, data, x);
+    if (lane == 0) r[row] = b[row] - s;
+  }
+}
+
+// dinv[row] = 1 / A[row, row]; a missing or zero diagonal flags st->status.
+__global__ __launch_bounds__(kBlock) void k_csr_diag_inv(int64_t n, const int32_t *__restrict__ indptr,
+                                                         const int32_t *__restrict__ indices,
+                                                         const double *__restrict__ data,
+                                                         double *__restrict__ dinv,
+                                                         PcgState *__restrict__ st) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  double d = 0.0;
+  for (int32_t k = indptr[row]; k < indptr[row + 1]; ++k)
+    if (indices[k] == row) d = data[k];
+  if (d != 0.0 && isfinite(d)) {
+    dinv[row] = 1.0 / d;
+  } else {
+    dinv[row] = 1.0;
+    st->status = TATVA_EINVAL;
+  }
+}
+
+// diag -> 1/diag for FP32 Jacobi; a zero diagonal flags st->status.
+__global__ __launch_bounds__(kBlock) void k_inv_diag_f32(int64_t n, const float *__restrict__ diag,
+                                                         float *__restrict__ dinv,
+                                                         PcgState *__restrict__ st) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = diag[i];
+  if (d != 0.0f && isfinite(d)) {
+    dinv[i] = 1.0f / d;
+  } else {
+    dinv[i] = 1.0f;
+    st->status = TATVA_EINVAL;
+  }
+}
+
+// r given; z = D^-1 r, p = z, r.z, r.r, b.b -> state; done if already converged.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T *__restrict__ r,
+                                                     const T *__restrict__ dinv,
+                                                     const T *__restrict__ b, T *__restrict__ p,
+                                                     double tol, int maxiter,
+                                                     double *__restrict__ partials,
+                                                     unsigned int *__restrict__ counter,
+                                                     PcgState *__restrict__ st) {
+  double rz = 0.0, rr = 0.0, bb = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T ri = r[i];
+    const T z = dinv[i] * ri;
+    p[i] = z;
+    rz += (double)ri * (double)z;
+    rr += (double)ri * (double)ri;
+    bb += (double)b[i] * (double)b[i];
+  }
+  rz = block_sum(rz);
+  rr = block_sum(rr);
+  bb = block_sum(bb);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = rz;
+    partials[kMaxReduceBlocks + blockIdx.x] = rr;
+    partials[2 * kMaxReduceBlocks + blockIdx.x] = bb;
+  }
+  if (last_block(counter)) {
+    const double a = block_sum_array(partials, gridDim.x);
+    const double c = block_sum_array(partials + kMaxReduceBlocks, gridDim.x);
+    const double d = block_sum_array(partials + 2 * kMaxReduceBlocks, gridDim.x);
+    if (threadIdx.x == 0) {
+      const int bad = st->status;  // set by the diagonal kernel
+      st->rz = a;
+      st->rr = c;
+      st->bb = d;
+      st->tol2 = tol * tol * d;
+      st->alpha = 0.0;
+      st->beta = 0.0;
+      st->iter = 0;
+      st->maxiter = maxiter;
+      // b == 0 leaves x as given (as the oracle does); an initial residual
+      // already below tol needs no iteration.
+      int done = 1, status = TATVA_OK;
+      if (bad != TATVA_OK) {
+        status = bad;
+      } else if (d == 0.0 || c <= st->tol2) {
+        status = TATVA_OK;
+      } else if (maxiter <= 0) {
+        status = TATVA_ENOTCONV;
+      } else {
+        done = 0;
+      }
+      st->status = status;
+      st->done = done;
+      *counter = 0u;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_copy_residual_f32(int64_t n, const T *__restrict__ b,
+                                                              const T *__restrict__ Ax,
+                                                              T *__restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] - Ax[i];
+}
+
+__global__ void k_state_reset(PcgState *st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->status = TATVA_OK;
+    st->done = 0;
+    st->iter = 0;
+  }
+}
+
+// ----------------------------------------------------- device resources ----
+
+struct DeviceRes {
+  std::mutex mu;
+  bool init = false;
+  hipStream_t cap = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipEvent_t prof_a = nullptr, prof_b = nullptr;  // timing events around a sampled operator
+  PcgState *host_state = nullptr;
+};
+
+static DeviceRes g_res[64];
+static std::atomic<int> g_profiling{0};
+
+static int get_res(int device, DeviceRes **out) {
+  if (device < 0 || device >= 64) {
+    set_error("device id out of range");
+    return TATVA_EINVAL;
+  }
+  DeviceRes &r = g_res[device];
+  if (!r.init) {
+    TATVA_TRY_HIP(hipStreamCreateWithFlags(&r.cap, hipStreamNonBlocking));
+    TATVA_TRY_HIP(hipEventCreateWithFlags(&r.ev_in, hipEventDisableTiming));
+    TATVA_TRY_HIP(hipEventCreateWithFlags(&r.ev_out, hipEventDisableTiming));
+    TATVA_TRY_HIP(hipEventCreate(&r.prof_a));
+    TATVA_TRY_HIP(hipEventCreate(&r.prof_b));
+    TATVA_TRY_HIP(hipHostMalloc((void **)&r.host_state, sizeof(PcgState), hipHostMallocDefault));
+    r.init = true;
+  }
+  *out = &r;
+  return TATVA_OK;
+}
+
+static int choose_lpr(int64_t n, int64_t nnz) {
+  const double avg = n > 0 ? double(nnz) / double(n) : 1.0;
+  if (avg <= 6) return 4;
+  if (avg <= 14) return 8;
+  if (avg <= 40) return 16;
+  if (avg <= 96) return 32;
+  return 64;
+}
+
+template <class F>
+static int with_lpr(int lpr, F &&f) {
+  switch (lpr) {
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 8: return f(std::integral_constant<int, 8>{});
+    case 16: return f(std::integral_constant<int, 16>{});
+    case 32: return f(std::integral_constant<int, 32>{});
+    default: return f(std::integral_constant<int, 64>{});
+  }
+}
+
+static int spmv_grid(int64_t n, int lpr) {
+  const int64_t rows_per_block = kBlock / lpr;
+  int64_t b = (n + rows_per_block - 1) / rows_per_block;
+  if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
+  return (int)(b < 1 ? 1 : b);
+}
+
+struct WsLayout {
+  size_t off[8];
+  size_t total;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// vectors: nvec vectors of n elements of size esz, then partials, state, counters.
+static WsLayout ws_layout(int64_t n, int nvec, size_t esz) {
+  WsLayout L{};
+  size_t o = 0;
+  for (int i = 0; i < nvec; ++i) {
+    L.off[i] = o;
+    o = align256(o + (size_t)n * esz);
+  }
+  L.off[5] = o;  // partials: 3 * kMaxReduceBlocks doubles
+  o = align256(o + 3 * kMaxReduceBlocks * sizeof(double));
+  L.off[6] = o;  // state
+  o = align256(o + sizeof(PcgState));
+  L.off[7] = o;  // counters
+  o = align256(o + 8 * sizeof(unsigned int));
+  L.total = o;
+  return L;
+}
+
+// Runs captured chunks until the device state reports done; fills stats.
+// Runs captured chunks until the device state reports done; fills stats.  With
+// profiling, the chunk's first operator application is bracketed by
+// prof_a/prof_b and its duration is read after each chunk.
+static int run_chunks(DeviceRes *res, hipGraphExec_t exec, PcgState *st, int maxiter, int chunk,
+                      bool profiling, tatva_pcg_stats *stats) {
+  const int max_launch = (maxiter + chunk - 1) / chunk + 1;
+  double op_ms = 0.0;
+  int samples = 0;
+  bool launched = false;
+  for (int l = 0; l <= max_launch + 1; ++l) {
+    TATVA_TRY_HIP(hipMemcpyAsync(res->host_state, st, sizeof(PcgState), hipMemcpyDeviceToHost,
+                                 res->cap));
+    TATVA_TRY_HIP(hipStreamSynchronize(res->cap));
+    if (launched && profiling) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, res->prof_a, res->prof_b) == hipSuccess) {
+        op_ms += ms;
+        ++samples;
+      }
+    }
+    if (res->host_state->done) break;
+    TATVA_TRY_HIP(hipGraphLaunch(exec, res->cap));
+    launched = true;
+  }
+  const PcgState &h = *res->host_state;
+  if (stats) {
+    stats->iterations = h.iter;
+    stats->status = h.done ? h.status : TATVA_ENOTCONV;
+    stats->b_norm = std::sqrt(h.bb);
+    stats->rel_residual = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+    stats->op_ms = op_ms;
+    stats->op_samples = samples;
+    stats->reserved = 0;
+  }
+  return TATVA_OK;
+}
+
+}  // namespace tatva
+
+using namespace tatva;
+
+extern "C" int tatva_spmv_csr(int64_t n, int64_t nnz, const int32_t *indptr,
+                              const int32_t *indices, const double *data, const double *x,
+                              double *y, void *stream) {
+  TATVA_REQUIRE(n >= 0 && nnz >= 0 && indptr && indices && data && x && y, "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  // The average row length picks the lanes per row.
+  return with_lpr(choose_lpr(n, nnz), [&](auto lc) -> int {
+    constexpr int L = decltype(lc)::value;
+    k_spmv<L><<<spmv_grid(n, L), kBlock, 0, s>>>(n, indptr, indices, data, x, y);
+    TATVA_CHECK_LAUNCH();
+    return TATVA_OK;
+  });
+}
+
+extern "C" int tatva_set_profiling(int enable) {
+  g_profiling.store(enable ? 1 : 0);
+  return TATVA_OK;
+}
+
+extern "C" int64_t tatva_pcg_csr_workspace(int64_t n) { return (int64_t)ws_layout(n, 4, sizeof(double)).total; }
+
+extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
+                             const int32_t *indices, const double *data, const double *b,
+                             double *x, double tol, int32_t maxiter, int32_t chunk,
+                             void *workspace, int64_t ws_bytes, tatva_pcg_stats *stats,
+                             void *stream) {
+  TATVA_REQUIRE(n > 0 && nnz > 0 && indptr && indices && data && b && x, "invalid arguments");
+  TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
+  TATVA_REQUIRE(tol >= 0.0, "tol must be non-negative");
+  const WsLayout L = ws_layout(n, 4, sizeof(double));
+  TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
+  int device = 0;
+  TATVA_TRY_HIP(hipGetDevice(&device));
+  DeviceRes *res = nullptr;
+  int rc = get_res(device, &res);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(res->mu);
+  char *w = (char *)workspace;
+  double *p = (double *)(w + L.off[0]);
+  double *r = (double *)(w + L.off[1]);
+  double *Ap = (double *)(w + L.off[2]);
+  double *dinv = (double *)(w + L.off[3]);
+  double *partials = (double *)(w + L.off[5]);
+  PcgState *st = (PcgState *)(w + L.off[6]);
+  unsigned int *counters = (unsigned int *)(w + L.off[7]);
+  hipStream_t caller = (hipStream_t)stream;
+  hipStream_t s = res->cap;
+  TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
+  TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
+  TATVA_TRY_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned int), s));
+  const int lpr = choose_lpr(n, nnz);
+  const int gv = reduce_grid(n);
+  const bool profiling = g_profiling.load() != 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  rc = with_lpr(lpr, [&](auto lc) -> int {
+    constexpr int LP = decltype(lc)::value;
+    const int gs = spmv_grid(n, LP);
+    k_state_reset<<<1, 64, 0, s>>>(st);
+    k_csr_diag_inv<<<grid_for(n), kBlock, 0, s>>>(n, indptr, indices, data, dinv, st);
+    k_csr_residual<LP><<<gs, kBlock, 0, s>>>(n, indptr, indices, data, x, b, r);
+    k_pcg_init<double><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, tol, maxiter, partials,
+                                             counters, st);
+    TATVA_CHECK_LAUNCH();
+    TATVA_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk; ++i) {
+      if (i == 0 && profiling) (void)hipEventRecord(res->prof_a, s);
+      k_pcg_spmv_dot<LP><<<gs, kBlock, 0, s>>>(n, indptr, indices, data, p, Ap, partials,
+                                               counters, st);
+      if (i == 0 && profiling) (void)hipEventRecord(res->prof_b, s);
+      k_pcg_update<double><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, partials,
+                                                 counters + 1, st);
+      k_pcg_pupdate<double><<<gv, kBlock, 0, s>>>(n, p, r, dinv, st);
+    }
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess) {
+      set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      return TATVA_EHIP;
+    }
+    TATVA_TRY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    return TATVA_OK;
+  });
+  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, stats);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
+  TATVA_TRY_HIP(hipStreamWaitEvent(caller, res->ev_out, 0));
+  return rc;
+}
+
+extern "C" int64_t tatva_pcg_matfree_workspace(tatva_mesh *m, int64_t n) {
+  (void)m;
+  return (int64_t)ws_layout(n, 5, sizeof(float)).total;
+}
+
+extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, const float *u,
+                                     const int32_t *dof_map, int64_t n, const float *diag,
+                                     const float *b, float *x, double tol, int32_t maxiter,
+                                     int32_t chunk, void *workspace, int64_t ws_bytes,
+                                     tatva_pcg_stats *stats, void *stream) {
+  TATVA_REQUIRE(m && d && u && diag && b && x && n > 0, "invalid arguments");
+  TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
+  const WsLayout L = ws_layout(n, 5, sizeof(float));
+  TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
+  TATVA_TRY_HIP(hipSetDevice(m->device));
+  DeviceRes *res = nullptr;
+  int rc = get_res(m->device, &res);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(res->mu);
+  char *w = (char *)workspace;
+  float *p = (float *)(w + L.off[0]);
+  float *r = (float *)(w + L.off[1]);
+  float *Ap = (float *)(w + L.off[2]);
+  float *dinv = (float *)(w + L.off[3]);
+  double *partials = (double *)(w + L.off[5]);
+  PcgState *st = (PcgState *)(w + L.off[6]);
+  unsigned int *counters = (unsigned int *)(w + L.off[7]);
+  hipStream_t caller = (hipStream_t)stream;
+  hipStream_t s = res->cap;
+  TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
+  TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
+  TATVA_TRY_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned int), s));
+  const int gv = reduce_grid(n);
+  const bool profiling = g_profiling.load() != 0;
+  k_state_reset<<<1, 64, 0, s>>>(st);
+  k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
+  // r = b - H x
+  rc = launch_hvp_f32(m, d, u, x, dof_map, Ap, s);
+  if (rc) return rc;
+  k_copy_residual_f32<float><<<grid_for(n), kBlock, 0, s>>>(n, b, Ap, r);
+  k_pcg_init<float><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, tol, maxiter, partials, counters, st);
+  TATVA_CHECK_LAUNCH();
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  TATVA_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < chunk && rc == TATVA_OK; ++i) {
+    if (i == 0 && profiling) (void)hipEventRecord(res->prof_a, s);
+    rc = launch_hvp_f32(m, d, u, p, dof_map, Ap, s, &st->done);
+    if (i == 0 && profiling) (void)hipEventRecord(res->prof_b, s);
+    k_pcg_dot_alpha<float><<<gv, kBlock, 0, s>>>(n, p, Ap, partials, counters, st);
+    k_pcg_update<float><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, partials, counters + 1, st);
+    k_pcg_pupdate<float><<<gv, kBlock, 0, s>>>(n, p, r, dinv, st);
+  }
+  hipError_t ce = hipStreamEndCapture(s, &graph);
+  if (rc == TATVA_OK && ce != hipSuccess) {
+    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    rc = TATVA_EHIP;
+  }
+  if (rc == TATVA_OK) {
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      rc = TATVA_EHIP;
+    }
+  }
+  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, stats);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
+  TATVA_TRY_HIP(hipStreamWaitEvent(caller, res->ev_out, 0));
+  return rc;
+}
+
+// ------------------------------------------------------- vector kernels ----
+
+namespace tatva {
+
+__global__ void k_scatter_free(int64_t n, const int32_t *__restrict__ idx, const double *__restrict__ in,
+                               double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[idx[i]] = in[i];
+}
+
+__global__ void k_gather_free(int64_t n, const int32_t *__restrict__ idx, const double *__restrict__ in,
+                              double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[idx[i]];
+}
+
+__global__ void k_set_dofs(int64_t n, const int32_t *__restrict__ idx, const double *__restrict__ vals,
+                           double value, double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[idx[i]] = vals ? vals[i] : value;
+}
+
+template <class T>
+__global__ void k_axpy_free(int64_t n, const int32_t *__restrict__ idx, double alpha,
+                            const T *__restrict__ du, double *__restrict__ u) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[idx[i]] += alpha * (double)du[i];
+}
+
+template <class A, class B>
+__global__ void k_cast(int64_t n, double alpha, const A *__restrict__ x, B *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = B(alpha * (double)x[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *__restrict__ a,
+                                                const double *__restrict__ b,
+                                                double *__restrict__ partials,
+                                                unsigned int *__restrict__ counter,
+                                                double *__restrict__ out) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += a[i] * b[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (last_block(counter)) {
+    const double t = block_sum_array(partials, gridDim.x);
+    if (threadIdx.x == 0) {
+      out[0] = t;
+      *counter = 0u;
+    }
+  }
+}
+
+}  // namespace tatva
+
+extern "C" int tatva_scatter_free(int64_t n, const int32_t *idx, const double *in, double *out,
+                                  void *stream) {
+  TATVA_REQUIRE(n >= 0 && (n == 0 || (idx && in && out)), "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  k_scatter_free<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, idx, in, out);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_set_dofs(int64_t n, const int32_t *idx, const double *vals, double value,
+                              double *out, void *stream) {
+  TATVA_REQUIRE(n >= 0 && (n == 0 || (idx && out)), "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  k_set_dofs<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, idx, vals, value, out);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_gather_free(int64_t n, const int32_t *idx, const double *in, double *out,
+                                 void *stream) {
+  TATVA_REQUIRE(n >= 0 && (n == 0 || (idx && in && out)), "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  k_gather_free<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, idx, in, out);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_axpy_free_f32(int64_t n, const int32_t *idx, double alpha, const float *du,
+                                   double *u, void *stream) {
+  TATVA_REQUIRE(n >= 0 && (n == 0 || (idx && du && u)), "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  k_axpy_free<float><<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, idx, alpha, du, u);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_axpy_free(int64_t n, const int32_t *idx, double alpha, const double *du,
+                               double *u, void *stream) {
+  TATVA_REQUIRE(n >= 0 && (n == 0 || (idx && du && u)), "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  k_axpy_free<double><<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, idx, alpha, du, u);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_cast_f64_f32(int64_t n, double alpha, const double *x, float *y, void *stream) {
+  TATVA_REQUIRE(n >= 0 && (n == 0 || (x && y)), "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  k_cast<double, float><<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, alpha, x, y);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_cast_f32_f64(int64_t n, double alpha, const float *x, double *y, void *stream) {
+  TATVA_REQUIRE(n >= 0 && (n == 0 || (x && y)), "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  k_cast<float, double><<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, alpha, x, y);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_dot(int64_t n, const double *x, const double *y, double *out, void *stream) {
+  TATVA_REQUIRE(n > 0 && x && y && out, "invalid arguments");
+  // The partials/counter live in a small per-device buffer owned by the library.
+  static double *partials[64] = {nullptr};
+  static unsigned int *counter[64] = {nullptr};
+  static std::mutex mu;
+  int device = 0;
+  TATVA_TRY_HIP(hipGetDevice(&device));
+  TATVA_REQUIRE(device >= 0 && device < 64, "device id out of range");
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!partials[device]) {
+      TATVA_TRY_HIP(hipMalloc(&partials[device], kMaxReduceBlocks * sizeof(double)));
+      TATVA_TRY_HIP(hipMalloc(&counter[device], sizeof(unsigned int)));
+      TATVA_TRY_HIP(hipMemset(counter[device], 0, sizeof(unsigned int)));
+    }
+  }
+  k_dot<<<reduce_grid(n), kBlock, 0, (hipStream_t)stream>>>(n, x, y, partials[device],
+                                                           counter[device], out);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+// rows[k] = row of entry k, col_colors[k] = colors[indices[k]]
+// (compute_rows_cols, tatva/sparse/base.py:108-136).  One wavefront per row.
+namespace tatva {
+__global__ __launch_bounds__(kBlock) void k_rows_cols(int64_t n, const int32_t *__restrict__ indptr,
+                                                      const int32_t *__restrict__ indices,
+                                                      const int32_t *__restrict__ colors,
+                                                      int32_t *__restrict__ rows,
+                                                      int32_t *__restrict__ col_colors) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (row >= n) return;
+  for (int32_t k = indptr[row] + lane; k < indptr[row + 1]; k += kWave) {
+    rows[k] = (int32_t)row;
+    col_colors[k] = colors[indices[k]];
+  }
+}
+}  // namespace tatva
+
+extern "C" int tatva_rows_cols(int64_t n, const int32_t *indptr, const int32_t *indices,
+                               const int32_t *colors, int32_t *rows, int32_t *col_colors,
+                               void *stream) {
+  TATVA_REQUIRE(n >= 0 && indptr && indices && colors && rows && col_colors, "invalid arguments");
+  if (n == 0) return TATVA_OK;
+  const int64_t blocks = (n + (kBlock / kWave) - 1) / (kBlock / kWave);
+  TATVA_REQUIRE(blocks < (int64_t(1) << 31), "too many rows");
+  k_rows_cols<<<(int)blocks, kBlock, 0, (hipStream_t)stream>>>(n, indptr, indices, colors, rows,
+                                                              col_colors);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}

@@ -1,0 +1,249 @@
+"""Operator: gather -> per-element / per-quadrature-point map -> weighted reduce.
+
+Mirrors ``tatva.Operator`` (tatva/operator.py:77-397) on gfx950 kernels: the
+mesh is uploaded once (coordinates, connectivity, node->element incidence),
+and ``eval``/``grad``/``integrate``/``get_integration_weights`` run as HIP
+kernels through the C ABI.  Inputs may be numpy arrays or torch tensors;
+outputs are FP64 torch tensors on the GPU.
+
+The reference's ``map``/``map_over_elements`` accept arbitrary JAX callables.
+A HIP kernel cannot execute those, so here they accept the built-in element
+functions (gradient, interpolation, Jacobian determinant) and the built-in
+energy densities; any other callable raises ``NotImplementedError`` rather
+than running on the host.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import weakref
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from . import _native as nat
+from .element import Element
+from .mesh import Mesh
+
+
+class _DeviceMesh:
+    """Owns the native ``tatva_mesh`` handle; freed with the Operator."""
+
+    def __init__(self, mesh: Mesh, element: Element):
+        L = nat.lib()
+        dev = nat.device()
+        h = ctypes.c_void_p()
+        coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+        elems = np.ascontiguousarray(mesh.elements, dtype=np.int32)
+        qp = np.ascontiguousarray(element.quad_points, dtype=np.float64)
+        qw = np.ascontiguousarray(element.quad_weights, dtype=np.float64)
+        nat.check(L.tatva_mesh_create(dev.index, element.kind, coords.ctypes.data, coords.shape[0],
+                                      coords.shape[1], elems.ctypes.data, elems.shape[0],
+                                      elems.shape[1], qw.shape[0], qp.ctypes.data, qw.ctypes.data,
+                                      ctypes.byref(h)))
+        self.handle = h.value
+        self.device = dev
+        self._fin = weakref.finalize(self, L.tatva_mesh_destroy, h.value)
+
+    def close(self):
+        self._fin()
+
+
+def _validate(mesh: Mesh, element: Element) -> None:
+    """``Operator.__check_init__`` (tatva/operator.py:132-170)."""
+    coords, elements = mesh.coords, mesh.elements
+    if coords.ndim != 2:
+        raise ValueError("Mesh coordinates must be a 2D array shaped (n_nodes, n_dim).")
+    if coords.shape[0] == 0:
+        raise ValueError("Mesh must contain at least one node.")
+    if elements.ndim != 2:
+        raise ValueError("Mesh elements must be a 2D array shaped (n_elements, n_nodes_per_element).")
+    if elements.shape[0] == 0:
+        raise ValueError("Mesh must contain at least one element.")
+    if not np.issubdtype(elements.dtype, np.integer):
+        raise TypeError("Mesh element connectivity must contain integer indices.")
+    if elements.min() < 0:
+        raise ValueError("Mesh element connectivity contains negative node indices.")
+    if elements.max() >= coords.shape[0]:
+        raise ValueError("Mesh element connectivity references nodes outside the mesh coordinates array.")
+    if coords.shape[1] != element.dim:
+        raise ValueError(f"{type(element).__name__} needs {element.dim}-D coordinates.")
+    if elements.shape[1] != element.nne:
+        raise ValueError(f"{type(element).__name__} has {element.nne} nodes per element.")
+
+
+@dataclass(frozen=True, eq=False)
+class Operator:
+    """``Operator(mesh, element, batch_size=None, cache_weights=False)``.
+
+    ``batch_size`` is accepted for API compatibility; the kernels stream all
+    elements in one launch.  ``cache_weights`` keeps detJ*w resident after the
+    first use (operator.py:119-130).
+    """
+
+    mesh: Mesh
+    element: Element
+    batch_size: int | None = None
+    cache_weights: bool = False
+    _dev: _DeviceMesh | None = field(default=None, init=False, repr=False)
+    _w: object = field(default=None, init=False, repr=False)
+
+    def __post_init__(self):
+        _validate(self.mesh, self.element)
+        if self.batch_size is None:
+            object.__setattr__(self, "batch_size", self.mesh.n_elements)
+
+    # ------------------------------------------------------------ plumbing --
+
+    @property
+    def handle(self) -> int:
+        if self._dev is None:
+            object.__setattr__(self, "_dev", _DeviceMesh(self.mesh, self.element))
+        return self._dev.handle
+
+    @property
+    def n_elements(self) -> int:
+        return self.mesh.n_elements
+
+    @property
+    def n_nodes(self) -> int:
+        return self.mesh.n_nodes
+
+    @property
+    def n_quad(self) -> int:
+        return int(self.element.quad_weights.shape[0])
+
+    def _nodal(self, values):
+        """(N,) or (N, ...) -> (device tensor (N, nc) contiguous, trailing shape)."""
+        t = nat.to_device(values)
+        if t.shape[0] != self.n_nodes:
+            raise ValueError(f"nodal array must have {self.n_nodes} rows, got {tuple(t.shape)}")
+        trailing = tuple(t.shape[1:])
+        nc = int(np.prod(trailing)) if trailing else 1
+        return t.reshape(self.n_nodes, nc).contiguous(), trailing, nc
+
+    # --------------------------------------------------------------- ops ----
+
+    def get_integration_weights(self):
+        """detJ * w_q per element and quadrature point, (E, Q) (operator.py:172-192)."""
+        if self.cache_weights and self._w is not None:
+            return self._w
+        t = nat.torch()
+        out = t.empty((self.n_elements, self.n_quad), dtype=t.float64, device=nat.device())
+        nat.check(nat.lib().tatva_weights(self.handle, nat.dptr(out), nat.stream()))
+        if self.cache_weights:
+            object.__setattr__(self, "_w", out)
+        return out
+
+    def eval(self, nodal_values):
+        """Values at quadrature points, (E, Q, ...) (operator.py:358-377)."""
+        u, trailing, nc = self._nodal(nodal_values)
+        t = nat.torch()
+        out = t.empty((self.n_elements, self.n_quad, nc), dtype=t.float64, device=u.device)
+        nat.check(nat.lib().tatva_eval(self.handle, nc, nat.dptr(u), nat.dptr(out), nat.stream()))
+        return out.reshape((self.n_elements, self.n_quad) + trailing)
+
+    def grad(self, nodal_values):
+        """Gradients at quadrature points, (E, Q, ..., d) (operator.py:379-397)."""
+        u, trailing, nc = self._nodal(nodal_values)
+        t = nat.torch()
+        d = self.element.dim
+        out = t.empty((self.n_elements, self.n_quad, nc, d), dtype=t.float64, device=u.device)
+        nat.check(nat.lib().tatva_grad(self.handle, nc, nat.dptr(u), nat.dptr(out), nat.stream()))
+        return out.reshape((self.n_elements, self.n_quad) + trailing + (d,))
+
+    def quads(self):
+        """Quadrature points in physical coordinates (operator.py:503-511)."""
+        return self.eval(self.mesh.coords)
+
+    def integrate_per_element(self, arg):
+        """Integral per element (operator.py:321-342): numeric, quad array
+        (shape[0] == n_elements) or nodal array."""
+        return self._integrate(arg, per_element=True)
+
+    def integrate(self, arg):
+        """Integral over the mesh (operator.py:307-319)."""
+        return self._integrate(arg, per_element=False)
+
+    def _integrate(self, arg, per_element: bool):
+        t = nat.torch()
+        if isinstance(arg, (int, float, np.number)):
+            vals = t.full((self.n_nodes, 1), float(arg), dtype=t.float64, device=nat.device())
+            nodal, trailing = 1, ()
+        else:
+            a = nat.to_device(arg)
+            if a.shape[0] == self.n_elements:
+                if a.ndim < 2 or a.shape[1] != self.n_quad:
+                    raise ValueError("quadrature arrays must be shaped (n_elements, n_quad, ...)")
+                trailing = tuple(a.shape[2:])
+                vals, nodal = a, 0
+            elif a.shape[0] == self.n_nodes:
+                trailing = tuple(a.shape[1:])
+                vals, nodal = a, 1
+            else:
+                raise ValueError("argument is neither a nodal nor a quadrature array")
+        nc = int(np.prod(trailing)) if trailing else 1
+        vals = vals.contiguous()
+        dev = nat.device()
+        pe = t.empty((self.n_elements, nc), dtype=t.float64, device=dev) if per_element else None
+        tot = None if per_element else t.empty((nc,), dtype=t.float64, device=dev)
+        nat.check(nat.lib().tatva_integrate(self.handle, nc, nodal, nat.dptr(vals), nat.dptr(pe),
+                                            nat.dptr(tot), nat.stream()))
+        if per_element:
+            return pe.reshape((self.n_elements,) + trailing)
+        return tot.reshape(trailing)
+
+    # ------------------------------------------------------------ mapping --
+
+    def map(self, func, *, element_quantity: Sequence[int] = ()):
+        """Map a built-in quadrature-point function over elements and quad points.
+
+        Supported ``func``: ``"gradient"``/``"grad"``, ``"interpolate"``/``"eval"``,
+        ``"det_jacobian"``, or a built-in density from ``tatva_amd.energies``
+        (mapped over nodal values: psi(grad u)).  Arbitrary callables raise.
+        """
+        from .energies import Density  # noqa: PLC0415
+
+        if element_quantity:
+            raise NotImplementedError("element_quantity needs a user callable; not supported on GPU")
+        if isinstance(func, Density):
+            return lambda u: func(self.grad(u), values=self.eval(u) if func.needs_values else None)
+        name = func if isinstance(func, str) else getattr(func, "__name__", None)
+        if name in ("gradient", "grad"):
+            return self.grad
+        if name in ("interpolate", "eval"):
+            return self.eval
+        if name == "det_jacobian":
+            def _det(_coords=None):
+                w = self.get_integration_weights()
+                return w / nat.to_device(self.element.quad_weights)[None, :]
+            return _det
+        raise NotImplementedError(
+            "Operator.map runs built-in kernels only (gradient, interpolate, det_jacobian, "
+            "tatva_amd.energies densities); arbitrary Python callables cannot run on the GPU path")
+
+    def map_over_elements(self, func, *, element_quantity: Sequence[int] = ()):
+        """Per-element map (operator.py:268-305): built-in ``"integrate"`` only."""
+        name = func if isinstance(func, str) else getattr(func, "__name__", None)
+        if name == "integrate":
+            return self.integrate_per_element
+        raise NotImplementedError(
+            "Operator.map_over_elements runs built-in kernels only; arbitrary Python callables "
+            "cannot run on the GPU path")
+
+    # ------------------------------------------------------------- energy --
+
+    def energy(self, density, *, external_force=None):
+        """Total-energy functional E(u) of a built-in density (see ``energies``)."""
+        from .energies import Energy  # noqa: PLC0415
+        return Energy(self, density, external_force=external_force)
+
+    def close(self) -> None:
+        """Release the device-resident mesh now (otherwise freed with the Operator)."""
+        if self._dev is not None:
+            self._dev.close()
+            object.__setattr__(self, "_dev", None)
+
+
+__all__ = ["Operator"]

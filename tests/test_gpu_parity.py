@@ -1,0 +1,330 @@
+"""GPU parity: every HIP kernel of the path against the oracle on the same
+seeded inputs.  FP64 results must agree to 1e-12 relative (the tolerance is
+written in each assertion); CSR indices and colors must be bit-exact."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import scipy.sparse.linalg as spla
+
+import tatva_amd as ta
+from oracle import fem
+from tatva_amd import energies, lifter, solvers, sparse
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+ORACLE_EL = {ta.Tri3: fem.Tri3, ta.Tetrahedron4: fem.Tet4, ta.Hexahedron8: fem.Hex8}
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def make_mesh(kind, n=4, jitter=0.2):
+    if kind == "tri3":
+        m = ta.Mesh.unit_square(n, n + 1)
+        rng = np.random.default_rng(11)
+        X = m.coords.copy()
+        inner = ~(np.isclose(X, 0) | np.isclose(X, 1)).any(axis=1)
+        X += jitter / n * rng.uniform(-1, 1, X.shape) * inner[:, None]
+        return m.set_coords(X), ta.Tri3()
+    if kind == "tet4":
+        return ta.Mesh.box((n, n - 1, n + 1), type="tetrahedron", jitter=jitter, seed=5), \
+            ta.Tetrahedron4()
+    m = ta.Mesh.unit_cube(n)
+    rng = np.random.default_rng(12)
+    X = m.coords.copy()
+    inner = ~(np.isclose(X, 0) | np.isclose(X, 1)).any(axis=1)
+    X += jitter / n * rng.uniform(-1, 1, X.shape) * inner[:, None]
+    return m.set_coords(X), ta.Hexahedron8()
+
+
+def oracle_density(d):
+    if isinstance(d, energies.Poisson):
+        return fem.Poisson(d.f)
+    if isinstance(d, energies.LinearElasticity):
+        return fem.LinearElasticity(d.mu, d.lmbda)
+    return fem.NeoHookean(d.mu, d.lmbda)
+
+
+def pair(kind, density, n=4, f_ext=False):
+    m, el = make_mesh(kind, n)
+    op = ta.Operator(m, el)
+    prob = fem.Problem(ORACLE_EL[type(el)](), m.coords, m.elements, oracle_density(density))
+    fe = None
+    if f_ext:
+        fe = np.random.default_rng(3).standard_normal(prob.n) * 1e-2
+        prob.f_ext = fe
+    E = op.energy(density, external_force=fe)
+    return op, E, prob
+
+
+# ------------------------------------------------------------------ KATs ----
+
+def test_operator_kats_on_gpu():
+    k = GOLDEN["operator_two_triangles"]
+    op = ta.Operator(ta.Mesh(np.array(k["nodes"]), np.array(k["elements"])), ta.Tri3())
+    np.testing.assert_allclose(to_np(op.eval(np.array(k["eval_input"]))), k["eval_expected"],
+                               rtol=RTOL)
+    X = np.array(k["nodes"])
+    np.testing.assert_allclose(to_np(op.grad(X @ np.array(k["grad_coeffs"]))), k["grad_expected"],
+                               rtol=RTOL)
+    np.testing.assert_allclose(to_np(op.integrate_per_element(np.ones(4))), k["areas_expected"],
+                               rtol=RTOL)
+    np.testing.assert_allclose(float(op.integrate(np.ones(4))), 1.0, rtol=RTOL)
+    np.testing.assert_allclose(float(op.integrate(1.0)), 1.0, rtol=RTOL)
+    q = np.full((2, 1), k["quad_value"])
+    np.testing.assert_allclose(to_np(op.integrate_per_element(q)), k["quad_integral_expected"],
+                               rtol=RTOL)
+    np.testing.assert_allclose(to_np(op.get_integration_weights()), [[0.5], [0.5]], rtol=RTOL)
+
+
+@pytest.mark.parametrize("cls", [ta.Tri3, ta.Tetrahedron4, ta.Hexahedron8])
+def test_element_linear_fields_exact_on_gpu(cls):
+    """tests/test_element.py:45-147 through Operator.grad on a one-element mesh."""
+    el = cls()
+    X = el._reference_nodes()
+    op = ta.Operator(ta.Mesh(X, np.arange(el.nne)[None, :]), el)
+    d = el.dim
+    rng = np.random.default_rng(0)
+    a = np.array([2.0, 3.0, 4.0][:d])
+    g = to_np(op.grad(X @ a))[0]
+    np.testing.assert_allclose(g, np.tile(a, (g.shape[0], 1)), atol=1e-12)
+    A = rng.standard_normal((d, d))
+    g = to_np(op.grad(X @ A.T))[0]
+    for q in range(g.shape[0]):
+        np.testing.assert_allclose(g[q], A, atol=1e-12)
+    B = rng.standard_normal((2, 2, d))
+    g = to_np(op.grad(np.einsum("ijk,nk->nij", B, X)))[0]
+    for q in range(g.shape[0]):
+        np.testing.assert_allclose(g[q], B, atol=1e-12)
+
+
+def test_lifter_kats_on_gpu():
+    k = GOLDEN["lifted_fixed_sum"]
+    L = lifter.Lifter(k["size"], lifter.Fixed(k["fixed"], 0.0))
+    u = L.lift_from_zeros(np.array(k["u_reduced"]))
+    np.testing.assert_array_equal(to_np(u), k["lift_expected"])
+    assert float(u.sum()) == k["sum_expected"]
+    np.testing.assert_array_equal(to_np(L.reduce(u)), k["u_reduced"])
+    k = GOLDEN["reduce_adjoint_fixed"]
+    L = lifter.Lifter(k["size"], lifter.Fixed(k["fixed"], 0.0))
+    np.testing.assert_array_equal(to_np(L.reduce_adjoint(np.array(k["r_full"]))), k["expected"])
+    L = lifter.Lifter(4, lifter.Fixed([0, 3], lifter.RuntimeValue("top")))
+    u = L.at["top"].set(np.array([1.5, -2.0])).lift(np.array([7.0, 8.0]), np.zeros(4))
+    np.testing.assert_array_equal(to_np(u), [1.5, 7.0, 8.0, -2.0])
+    dbl = lifter.lifted(lambda x: x * 2, output="primal")
+    np.testing.assert_array_equal(to_np(dbl(L.at["top"].set(0.0), np.array([10.0, 20.0]))),
+                                  [20.0, 40.0])
+
+
+# ------------------------------------------------------------ operator ------
+
+@pytest.mark.parametrize("kind", ["tri3", "tet4", "hex8"])
+def test_eval_grad_weights_integrate(kind):
+    m, el = make_mesh(kind, 4)
+    op = ta.Operator(m, el)
+    oe = ORACLE_EL[type(el)]()
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((m.n_nodes, 3))
+    assert rel_err(to_np(op.get_integration_weights()),
+                   fem.integration_weights(oe, m.coords, m.elements)) < RTOL
+    assert rel_err(to_np(op.eval(u)), fem.op_eval(oe, m.coords, m.elements, u)) < RTOL
+    assert rel_err(to_np(op.grad(u)), fem.op_grad(oe, m.coords, m.elements, u)) < RTOL
+    assert rel_err(to_np(op.grad(u[:, 0])), fem.op_grad(oe, m.coords, m.elements, u[:, 0])) < RTOL
+    assert rel_err(to_np(op.integrate_per_element(u)),
+                   fem.integrate_per_element(oe, m.coords, m.elements, u)) < RTOL
+    assert rel_err(to_np(op.integrate(u)), fem.integrate(oe, m.coords, m.elements, u)) < RTOL
+    qv = rng.standard_normal((m.n_elements, len(el.quad_weights), 2))
+    assert rel_err(to_np(op.integrate(qv)), fem.integrate(oe, m.coords, m.elements, qv)) < RTOL
+    assert rel_err(to_np(op.quads()), fem.op_eval(oe, m.coords, m.elements, m.coords)) < RTOL
+
+
+DENSITIES = [energies.Poisson(1.0), energies.LinearElasticity(0.38, 0.58),
+             energies.NeoHookean(1.0, 10.0)]
+CASES = [(k, d) for k in ("tri3", "tet4", "hex8") for d in DENSITIES]
+
+
+@pytest.mark.parametrize("kind,density", CASES, ids=[f"{k}-{type(d).__name__}" for k, d in CASES])
+def test_energy_residual_hvp_parity(kind, density):
+    op, E, prob = pair(kind, density, f_ext=True)
+    rng = np.random.default_rng(2)
+    u = 0.05 * rng.standard_normal(prob.n)
+    v = rng.standard_normal(prob.n)
+    e = float(E(u))
+    assert abs(e - prob.energy(u)) <= RTOL * max(abs(prob.energy(u)), 1e-300) * 10
+    assert rel_err(to_np(E.grad(u)), prob.residual(u)) < RTOL
+    assert rel_err(to_np(E.hvp(u, v)), prob.hvp(u, v)) < RTOL
+    assert rel_err(to_np(E.hessian_diag(u)), prob.hessian_diag(u)) < RTOL
+    r, y = ta.jvp(ta.grad(E), (u,), (v,))
+    assert rel_err(to_np(y), prob.hvp(u, v)) < RTOL
+    # FP32 Hessian-vector product: same operator in single precision
+    t = __import__("torch")
+    u32 = t.tensor(u, dtype=t.float32, device="cuda")
+    v32 = t.tensor(v, dtype=t.float32, device="cuda")
+    assert rel_err(to_np(E.hvp(u32, v32)), prob.hvp(u, v)) < 2e-5
+    # psi at quadrature points through the density callable
+    psi = density(op.grad(u.reshape(-1, prob.nc) if prob.nc > 1 else u),
+                  values=op.eval(u) if density.needs_values else None)
+    w = fem.integration_weights(prob.elem, prob.coords, prob.elements)
+    prob.f_ext = None
+    assert abs(float((psi * op.get_integration_weights()).sum()) - prob.energy(u)) \
+        <= 1e-11 * max(abs(prob.energy(u)), 1e-300)
+    assert w.shape == tuple(psi.shape[:2])
+
+
+def test_residual_and_hessian_are_deterministic():
+    op, E, prob = pair("hex8", energies.NeoHookean(1.0, 10.0), n=6)
+    u = 0.05 * np.random.default_rng(4).standard_normal(prob.n)
+    r1, r2 = to_np(E.grad(u)), to_np(E.grad(u))
+    assert np.array_equal(r1, r2)
+    S = sparse.pattern_from_mesh(op.mesh, 3)
+    cm = sparse.ColoredMatrix.from_csr(S)
+    J = sparse.jacfwd(E.grad, cm)
+    assert np.array_equal(to_np(J(u).data), to_np(J(u).data))
+
+
+# ------------------------------------------------------ colored Hessian -----
+
+def test_colored_hessian_tri3_kat():
+    """tests/test_sparse.py:49-80: colored == dense Hessian; linearized primal == grad."""
+    m = ta.Mesh.unit_square(8, 8)
+    op = ta.Operator(m, ta.Tri3())
+    E = op.energy(energies.LinearElasticity(1.0, 0.0))
+    S = sparse.pattern_from_mesh(m, 2)
+    cm = sparse.ColoredMatrix.from_csr(S)
+    u0 = np.zeros(m.n_nodes * 2)
+    K = sparse.jacfwd(ta.grad(E), cm, color_batch_size=10)(u0)
+    prim, K2 = sparse.linearized_jacfwd(ta.grad(E), cm, color_batch_size=10)(u0)
+    prob = fem.Problem(fem.Tri3(), m.coords, m.elements, fem.LinearElasticity(1.0, 0.0))
+    Hd = np.stack([prob.hvp(u0, np.eye(prob.n)[j]) for j in range(prob.n)], axis=1)
+    np.testing.assert_allclose(K.to_dense(), Hd, rtol=1e-7, atol=1e-12)
+    np.testing.assert_allclose(K2.to_dense(), Hd, rtol=1e-7, atol=1e-12)
+    np.testing.assert_allclose(to_np(prim), prob.residual(u0), atol=1e-15)
+    rows, cc = sparse.compute_rows_cols(cm)
+    ro, cco = fem.compute_rows_cols(S.indptr, S.indices, to_np(cm.colors))
+    np.testing.assert_array_equal(to_np(rows), ro)
+    np.testing.assert_array_equal(to_np(cc), cco)
+
+
+HESS_CASES = [("tri3", energies.Poisson(1.0)), ("tri3", energies.NeoHookean(1.0, 2.0)),
+              ("tet4", energies.NeoHookean(1.0, 10.0)), ("tet4", energies.LinearElasticity(0.4, 0.6)),
+              ("hex8", energies.LinearElasticity(0.38, 0.58)), ("hex8", energies.NeoHookean(1.0, 10.0))]
+
+
+@pytest.mark.parametrize("kind,density", HESS_CASES,
+                         ids=[f"{k}-{type(d).__name__}" for k, d in HESS_CASES])
+def test_colored_hessian_with_lifter_parity(kind, density):
+    op, E, prob = pair(kind, density, n=4)
+    nc = prob.nc
+    m = op.mesh
+    x0 = np.where(np.isclose(m.coords[:, 0], 0.0))[0]
+    x1 = np.where(np.isclose(m.coords[:, 0], 1.0))[0]
+    fixed = np.concatenate([(x0[:, None] * nc + np.arange(nc)).ravel(), x1 * nc])
+    L = lifter.Lifter(prob.n, lifter.Fixed(fixed, lifter.RuntimeValue("g", default=0.0)))
+    L = L.at["g"].set(0.01)
+    Lo = fem.Lifter(prob.n, fem.Fixed(fixed, 0.01))
+    S = L.adapt_sparsity(sparse.pattern_from_mesh(m, nc))
+    So = Lo.adapt_sparsity(fem.pattern_from_mesh(m.elements, m.n_nodes, nc))
+    So.sort_indices()
+    cm = sparse.ColoredMatrix.from_csr(S)
+    colors_o = fem.distance2_colors(So.indptr, So.indices, So.shape[0])
+    np.testing.assert_array_equal(to_np(cm.colors), colors_o)          # bit-exact
+    np.testing.assert_array_equal(to_np(cm.indptr), So.indptr)
+    np.testing.assert_array_equal(to_np(cm.indices), So.indices)
+    rng = np.random.default_rng(6)
+    ured = 0.03 * rng.standard_normal(L.size_reduced)
+    Ered = E.reduced(L)
+    K = sparse.jacfwd(Ered.grad, cm)(ured)
+    uf = Lo.lift_from_zeros(ured)
+    Ko = fem.hessian_csr(prob, Lo, uf, So, colors_o)   # literal one-JVP-per-color oracle
+    np.testing.assert_array_equal(to_np(K.indices), Ko.indices)
+    assert rel_err(to_np(K.data), Ko.data) < RTOL
+    prim, K2 = sparse.linearized_jacfwd(Ered.grad, cm)(ured)
+    assert rel_err(to_np(prim), Lo.reduce_adjoint(prob.residual(uf))) < RTOL
+    assert np.array_equal(to_np(K2.data), to_np(K.data))
+    # SpMV on the assembled matrix equals the matrix-free HVP
+    v = rng.standard_normal(L.size_reduced)
+    assert rel_err(to_np(K @ v), to_np(Ered.hvp(ured, v))) < 1e-11
+
+
+# ------------------------------------------------------------ solvers -------
+
+def test_pcg_matches_spsolve_and_oracle_iterations():
+    m = ta.Mesh.unit_cube(6)
+    op = ta.Operator(m, ta.Hexahedron8())
+    E = op.energy(energies.LinearElasticity(0.4, 0.6))
+    x0n = np.where(np.isclose(m.coords[:, 0], 0.0))[0]
+    L = lifter.Lifter(m.n_nodes * 3, lifter.Fixed((x0n[:, None] * 3 + np.arange(3)).ravel()))
+    cm = sparse.ColoredMatrix.from_csr(L.adapt_sparsity(sparse.pattern_from_mesh(m, 3)))
+    K = sparse.jacfwd(E.reduced(L).grad, cm)(np.zeros(L.size_reduced))
+    b = np.random.default_rng(5).standard_normal(L.size_reduced)
+    x, info = solvers.pcg(K, b, tol=1e-10, chunk=8)
+    assert info.converged and info.rel_residual <= 1e-10
+    Kh = K.to_csr()
+    xo, ito, relo = fem.pcg(Kh, b, tol=1e-10)
+    assert abs(info.iterations - ito) <= 2
+    xs = spla.spsolve(Kh.tocsc(), b)
+    assert rel_err(to_np(x), xs) < 1e-8
+    true_rel = np.linalg.norm(b - Kh @ to_np(x)) / np.linalg.norm(b)
+    assert true_rel < 1e-9
+    # zero right-hand side and maxiter limit
+    x, info = solvers.pcg(K, np.zeros_like(b))
+    assert info.converged and info.iterations == 0 and float(x.abs().max()) == 0.0
+    x, info = solvers.pcg(K, b, maxiter=3, chunk=2)
+    assert info.iterations == 3 and not info.converged
+
+
+def test_newton_poisson_tri3_matches_oracle():
+    n = 16
+    m = ta.Mesh.unit_square(n, n)
+    op = ta.Operator(m, ta.Tri3())
+    E = op.energy(energies.Poisson(1.0))
+    bnd = np.where(np.isclose(m.coords, 0).any(1) | np.isclose(m.coords, 1).any(1))[0]
+    L = lifter.Lifter(m.n_nodes, lifter.Fixed(bnd, 0.0))
+    solver = solvers.Newton(E, L, mode="csr", tol=1e-10, lin_tol=1e-12)
+    t = __import__("torch")
+    u = t.zeros(m.n_nodes, dtype=t.float64, device="cuda")
+    st = solver.solve(u)
+    assert st.converged and st.newton_iterations == 1
+    prob = fem.Problem(fem.Tri3(), m.coords, m.elements, fem.Poisson(1.0))
+    Lo = fem.Lifter(m.n_nodes, fem.Fixed(bnd))
+    So = Lo.adapt_sparsity(fem.pattern_from_mesh(m.elements, m.n_nodes, 1))
+    co = fem.distance2_colors(So.indptr, So.indices, So.shape[0])
+    uo, its, cg, hist = fem.newton(prob, Lo, np.zeros(m.n_nodes), So, co, tol=1e-12, lin_tol=1e-12)
+    assert rel_err(to_np(u), uo) < 1e-10
+
+
+def test_newton_mixed_precision_neohookean():
+    """FP32 matrix-free CG inside FP64 Newton reaches 1e-8 relative residual and the
+    FP64 colored-CSR Newton solution within 1e-6 relative."""
+    t = __import__("torch")
+    m = ta.Mesh.unit_cube(5)
+    op = ta.Operator(m, ta.Hexahedron8())
+    E = op.energy(energies.NeoHookean(1.0, 10.0))
+    x0n = np.where(np.isclose(m.coords[:, 0], 0.0))[0]
+    x1n = np.where(np.isclose(m.coords[:, 0], 1.0))[0]
+    fixed = np.concatenate([(x0n[:, None] * 3 + np.arange(3)).ravel(), x1n * 3])
+    L = lifter.Lifter(m.n_nodes * 3, lifter.Fixed(fixed, lifter.RuntimeValue("s")))
+    vals = lambda s: np.concatenate([np.zeros(x0n.size * 3), np.full(x1n.size, s)])  # noqa: E731
+    um = t.zeros(m.n_nodes * 3, dtype=t.float64, device="cuda")
+    uc = um.clone()
+    mixed = solvers.Newton(E, L.at["s"].set(vals(0.0)), mode="mixed", tol=1e-8, inner_tol=1e-3)
+    csr = solvers.Newton(E, L.at["s"].set(vals(0.0)), mode="csr", tol=1e-10, lin_tol=1e-12)
+    for s in (0.05, 0.1):
+        st_m = mixed.solve(um, L.at["s"].set(vals(s)))
+        st_c = csr.solve(uc, L.at["s"].set(vals(s)))
+        assert st_m.converged and st_c.converged
+        assert st_m.residual_history[-1] <= 1e-8 * st_m.residual_history[0]
+    assert rel_err(to_np(um), to_np(uc)) < 1e-6

@@ -1,0 +1,347 @@
+"""Pins the oracle (oracle/fem.py) before it is trusted as the parity reference.
+
+Each check is chosen so that a plausible slip in the restatement (dropped term,
+wrong sign or index, transposed operand, wrong node order or quadrature) fails:
+reference known-answer tests, exact rational integration, finite differences of
+the energy, invariants of the Hessian and brute force on tiny inputs.
+"""
+
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import scipy.sparse.linalg as spla
+
+from oracle import fem
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+# --------------------------------------------------------------- KATs ------
+
+def test_operator_kats():
+    k = GOLDEN["operator_two_triangles"]
+    X = np.array(k["nodes"])
+    E = np.array(k["elements"], dtype=np.int32)
+    t = fem.Tri3()
+    np.testing.assert_allclose(fem.op_eval(t, X, E, np.array(k["eval_input"])),
+                               k["eval_expected"])
+    np.testing.assert_allclose(fem.op_grad(t, X, E, X @ np.array(k["grad_coeffs"])),
+                               k["grad_expected"])
+    np.testing.assert_allclose(fem.integrate_per_element(t, X, E, np.ones(4)), k["areas_expected"])
+    np.testing.assert_allclose(fem.integrate(t, X, E, np.ones(4)), sum(k["areas_expected"]))
+    q = np.full((2, 1), k["quad_value"])
+    np.testing.assert_allclose(fem.integrate_per_element(t, X, E, q), k["quad_integral_expected"])
+
+
+@pytest.mark.parametrize("cls", [fem.Tri3, fem.Tet4, fem.Hex8])
+def test_element_linear_fields_exact(cls):
+    """tests/test_element.py:45-147 on the oracle: exact gradients of linear fields."""
+    el = cls()
+    X = el.ref_nodes
+    E = np.arange(el.nne, dtype=np.int32)[None, :]
+    d = el.dim
+    rng = np.random.default_rng(0)
+    a = np.array([2.0, 3.0, 4.0][:d])
+    np.testing.assert_allclose(fem.op_grad(el, X, E, X @ a)[0], np.tile(a, (len(el.quad_weights), 1)),
+                               atol=1e-12)
+    A = rng.standard_normal((d, d))
+    g = fem.op_grad(el, X, E, X @ A.T)[0]
+    for q in range(g.shape[0]):
+        np.testing.assert_allclose(g[q], A, atol=1e-12)
+    B = rng.standard_normal((2, 2, d))
+    T = np.einsum("ijk,nk->nij", B, X)
+    g = fem.op_grad(el, X, E, T)[0]
+    for q in range(g.shape[0]):
+        np.testing.assert_allclose(g[q], B, atol=1e-12)
+
+
+def test_lifter_kats():
+    k = GOLDEN["lifter_fixed_periodic"]
+    L = fem.Lifter(k["size"], fem.Fixed(k["fixed"]), fem.Periodic(k["periodic_dofs"], k["periodic_master"]))
+    u = L.lift_from_zeros(np.array(k["u_reduced"]))
+    np.testing.assert_array_equal(u, k["lift_expected"])
+    np.testing.assert_array_equal(L.reduce(u), k["u_reduced"])
+    k = GOLDEN["lifter_no_constraints"]
+    L = fem.Lifter(k["size"])
+    np.testing.assert_array_equal(L.lift_from_zeros(np.arange(4.0)), k["lift_expected"])
+    for name in ("reduce_adjoint_fixed", "reduce_adjoint_periodic", "reduce_adjoint_periodic_fixed"):
+        k = GOLDEN[name]
+        cons = []
+        if "periodic_dofs" in k:
+            cons.append(fem.Periodic(k["periodic_dofs"], k["periodic_master"]))
+        if "fixed" in k:
+            cons.append(fem.Fixed(k["fixed"]))
+        L = fem.Lifter(k["size"], *cons)
+        np.testing.assert_array_equal(L.reduce_adjoint(np.array(k["r_full"])), k["expected"])
+
+
+def _dense_hessian(prob, u):
+    K = prob.element_stiffness(u)
+    n = prob.n
+    nc = prob.nc
+    dofs = (prob.elements[:, :, None] * nc + np.arange(nc)).reshape(len(prob.elements), -1)
+    H = np.zeros((n, n))
+    for e in range(len(dofs)):
+        H[np.ix_(dofs[e], dofs[e])] += K[e]
+    return H
+
+
+def test_sparse_kat_tri3_elasticity():
+    """tests/test_sparse.py:49-80: colored Hessian == dense Hessian; primal == grad."""
+    X, E = fem.unit_square_tri3(8, 8)
+    prob = fem.Problem(fem.Tri3(), X, E, fem.LinearElasticity(1.0, 0.0))
+    S = fem.pattern_from_mesh(E, X.shape[0], 2)
+    colors = fem.distance2_colors_py(S.indptr, S.indices, S.shape[0])
+    u0 = np.zeros(prob.n)
+    data = fem.colored_jacfwd(lambda s: prob.hvp(u0, s), S.indptr, S.indices, colors)
+    K = sps.csr_matrix((data, S.indices, S.indptr), shape=S.shape).toarray()
+    # dense Hessian by brute force: H e_j for every j
+    Hd = np.stack([prob.hvp(u0, np.eye(prob.n)[j]) for j in range(prob.n)], axis=1)
+    np.testing.assert_allclose(K, Hd, rtol=1e-7, atol=1e-12)
+    np.testing.assert_allclose(K, _dense_hessian(prob, u0), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(prob.residual(u0), np.zeros(prob.n), atol=1e-15)
+
+
+# ----------------------------------------------------------- closed forms --
+
+def _exact_hex8_le_stiffness(mu, lmbda):
+    """Unit-cube Hex8 K_e by exact rational integration of products of 1-D linears."""
+    signs = [(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1),
+             (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)]
+
+    def I(a, i, b, j):
+        """integral over [0,1]^3 of dN_a/dx_i dN_b/dx_j."""
+        val = Fraction(1)
+        for d in range(3):
+            sa, sb = signs[a][d], signs[b][d]
+            if d == i and d == j:
+                val *= sa * sb
+            elif d == i:
+                val *= Fraction(sa, 2)
+            elif d == j:
+                val *= Fraction(sb, 2)
+            else:
+                val *= Fraction(1, 3) if sa == sb else Fraction(1, 6)
+        return val
+
+    K = np.zeros((24, 24))
+    for a in range(8):
+        for i in range(3):
+            for b in range(8):
+                for k in range(3):
+                    lap = sum(I(a, j, b, j) for j in range(3))
+                    v = mu * ((lap if i == k else 0) + I(a, k, b, i)) + lmbda * I(a, i, b, k)
+                    K[a * 3 + i, b * 3 + k] = float(v)
+    return K
+
+
+def test_hex8_le_stiffness_exact():
+    mu, lm = 0.3846153846153846, 0.5769230769230769
+    X, E = fem.cube_hex8(1)
+    prob = fem.Problem(fem.Hex8(), X, E, fem.LinearElasticity(mu, lm))
+    K = prob.element_stiffness(np.zeros(prob.n))[0]
+    np.testing.assert_allclose(K, _exact_hex8_le_stiffness(mu, lm), rtol=1e-13, atol=1e-14)
+
+
+def test_tri3_poisson_load_is_area_over_three():
+    X, E = fem.unit_square_tri3(4, 4)
+    prob = fem.Problem(fem.Tri3(), X, E, fem.Poisson(1.0))
+    r = prob.residual(np.zeros(prob.n))
+    load = np.zeros(prob.n)
+    area = fem.integrate_per_element(fem.Tri3(), X, E, np.ones(X.shape[0]))
+    np.add.at(load, E, np.repeat(area[:, None] / 3, 3, axis=1))
+    np.testing.assert_allclose(-r, load, rtol=1e-14)
+
+
+# ------------------------------------------------------- finite differences --
+
+CASES = [
+    ("tri3", fem.Poisson(1.0)), ("tri3", fem.LinearElasticity(1.0, 0.5)),
+    ("tri3", fem.NeoHookean(1.0, 2.0)),
+    ("tet4", fem.Poisson(0.7)), ("tet4", fem.LinearElasticity(0.4, 0.6)),
+    ("tet4", fem.NeoHookean(1.0, 10.0)),
+    ("hex8", fem.Poisson(1.0)), ("hex8", fem.LinearElasticity(0.38, 0.58)),
+    ("hex8", fem.NeoHookean(1.0, 10.0)),
+]
+
+
+def small_problem(kind, density, jitter=0.15):
+    if kind == "tri3":
+        X, E = fem.unit_square_tri3(3, 2)
+        X = X + jitter * 0.3 * np.random.default_rng(1).uniform(-1, 1, X.shape) * \
+            (~(np.isclose(X, 0) | np.isclose(X, 1))).all(axis=1, keepdims=True) / 3
+        el = fem.Tri3()
+    elif kind == "tet4":
+        X, E = fem.box_tet4(2, 2, 1, jitter=jitter, seed=3)
+        el = fem.Tet4()
+    else:
+        X, E = fem.cube_hex8(2)
+        rng = np.random.default_rng(4)
+        inner = ~(np.isclose(X, 0) | np.isclose(X, 1)).any(axis=1)
+        X = X + jitter * 0.5 * rng.uniform(-1, 1, X.shape) * inner[:, None]
+        el = fem.Hex8()
+    return fem.Problem(el, X, E, density)
+
+
+@pytest.mark.parametrize("kind,density", CASES)
+def test_residual_matches_energy_differences(kind, density):
+    prob = small_problem(kind, density)
+    rng = np.random.default_rng(7)
+    u = 0.05 * rng.standard_normal(prob.n)
+    r = prob.residual(u)
+    h = 1e-6
+    fd = np.array([(prob.energy(u + h * e) - prob.energy(u - h * e)) / (2 * h)
+                   for e in np.eye(prob.n)])
+    np.testing.assert_allclose(r, fd, rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("kind,density", CASES)
+def test_hvp_matches_residual_differences(kind, density):
+    prob = small_problem(kind, density)
+    rng = np.random.default_rng(8)
+    u = 0.05 * rng.standard_normal(prob.n)
+    v = rng.standard_normal(prob.n)
+    h = 1e-6
+    fd = (prob.residual(u + h * v) - prob.residual(u - h * v)) / (2 * h)
+    np.testing.assert_allclose(prob.hvp(u, v), fd, rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("kind,density", CASES)
+def test_taylor_second_order(kind, density):
+    prob = small_problem(kind, density)
+    rng = np.random.default_rng(9)
+    u = 0.05 * rng.standard_normal(prob.n)
+    v = rng.standard_normal(prob.n)
+    E0, g = prob.energy(u), prob.residual(u) @ v
+    Hv = prob.hvp(u, v) @ v
+    errs = []
+    for h in (1e-2, 5e-3, 2.5e-3):
+        errs.append(abs(prob.energy(u + h * v) - E0 - h * g - 0.5 * h * h * Hv))
+    # third-order remainder: halving h divides the error by ~8 (exactly 0 if quadratic)
+    if errs[0] > 1e-13:
+        assert errs[1] < errs[0] / 5 and errs[2] < errs[1] / 5
+
+
+@pytest.mark.parametrize("kind,density", CASES)
+def test_stiffness_symmetric_and_consistent(kind, density):
+    prob = small_problem(kind, density)
+    u = 0.05 * np.random.default_rng(10).standard_normal(prob.n)
+    K = prob.element_stiffness(u)
+    np.testing.assert_allclose(K, np.swapaxes(K, 1, 2), rtol=1e-12, atol=1e-14)
+    H = _dense_hessian(prob, u)
+    v = np.random.default_rng(11).standard_normal(prob.n)
+    np.testing.assert_allclose(H @ v, prob.hvp(u, v), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(np.diag(H), prob.hessian_diag(u), rtol=1e-12, atol=1e-14)
+
+
+# ------------------------------------------------------------- invariants --
+
+def test_poisson_constant_in_kernel_and_le_rigid_modes():
+    prob = small_problem("tet4", fem.Poisson(1.0))
+    np.testing.assert_allclose(prob.hvp(np.zeros(prob.n), np.ones(prob.n)), 0.0, atol=1e-13)
+    prob = small_problem("hex8", fem.LinearElasticity(0.4, 0.6))
+    X = prob.coords
+    modes = [np.tile(np.eye(3)[i], X.shape[0]) for i in range(3)]
+    for axis in range(3):   # infinitesimal rotations
+        w = np.zeros(3)
+        w[axis] = 1.0
+        modes.append(np.cross(w, X).ravel())
+    for m in modes:
+        np.testing.assert_allclose(prob.hvp(np.zeros(prob.n), m), 0.0, atol=1e-12)
+
+
+def test_nh_at_rest_equals_linear_elasticity():
+    for kind in ("tri3", "tet4", "hex8"):
+        a = small_problem(kind, fem.NeoHookean(1.3, 4.0))
+        b = small_problem(kind, fem.LinearElasticity(1.3, 4.0))
+        v = np.random.default_rng(12).standard_normal(a.n)
+        np.testing.assert_allclose(a.hvp(np.zeros(a.n), v), b.hvp(np.zeros(b.n), v), rtol=1e-12,
+                                   atol=1e-13)
+        np.testing.assert_allclose(a.residual(np.zeros(a.n)), 0.0, atol=1e-14)
+
+
+def test_linear_newton_converges_in_one_step():
+    X, E = fem.cube_hex8(3)
+    prob = fem.Problem(fem.Hex8(), X, E, fem.LinearElasticity(0.4, 0.6),
+                       f_ext=fem.face_traction_loads(X, 3))
+    fixed = np.where(np.isclose(X[:, 0], 0.0))[0]
+    L = fem.Lifter(prob.n, fem.Fixed((fixed[:, None] * 3 + np.arange(3)).ravel()))
+    S = L.adapt_sparsity(fem.pattern_from_mesh(E, X.shape[0], 3))
+    colors = fem.distance2_colors(S.indptr, S.indices, S.shape[0])
+    u, its, cg, hist = fem.newton(prob, L, np.zeros(prob.n), S, colors, tol=1e-9, lin_tol=1e-12)
+    assert its == 1 and hist[-1] <= 1e-9 * hist[0]
+
+
+# ------------------------------------------------------------ brute force --
+
+@pytest.mark.parametrize("maker,ndpn", [(lambda: fem.unit_square_tri3(3, 4), 2),
+                                        (lambda: fem.box_tet4(2, 1, 2), 3),
+                                        (lambda: fem.cube_hex8(2), 1)])
+def test_pattern_equals_set_of_element_pairs(maker, ndpn):
+    X, E = maker()
+    S = fem.pattern_from_mesh(E, X.shape[0], ndpn)
+    pairs = set()
+    for el in E:
+        dofs = [int(n) * ndpn + c for n in el for c in range(ndpn)]
+        pairs.update((a, b) for a in dofs for b in dofs)
+    rows = np.repeat(np.arange(S.shape[0]), np.diff(S.indptr))
+    assert set(zip(rows.tolist(), S.indices.tolist())) == pairs
+    for i in range(S.shape[0]):
+        seg = S.indices[S.indptr[i]:S.indptr[i + 1]]
+        assert np.all(np.diff(seg) > 0)
+
+
+@pytest.mark.parametrize("maker,ndpn", [(lambda: fem.unit_square_tri3(5, 4), 2),
+                                        (lambda: fem.box_tet4(3, 2, 2), 3),
+                                        (lambda: fem.cube_hex8(3), 3)])
+def test_coloring_valid_and_c_matches_python(maker, ndpn):
+    X, E = maker()
+    S = fem.pattern_from_mesh(E, X.shape[0], ndpn)
+    cp = fem.distance2_colors_py(S.indptr, S.indices, S.shape[0])
+    cc = fem.distance2_colors(S.indptr, S.indices, S.shape[0])
+    np.testing.assert_array_equal(cp, cc)
+    for i in range(S.shape[0]):      # distance-2: the columns of every row differ in color
+        cols = S.indices[S.indptr[i]:S.indptr[i + 1]]
+        assert len(set(cc[cols].tolist())) == len(cols)
+
+
+def test_greedy_is_first_fit():
+    """Every color c > 0 of node i is blocked by each smaller color among earlier A^2 neighbours."""
+    X, E = fem.box_tet4(2, 2, 2)
+    S = fem.pattern_from_mesh(E, X.shape[0], 3)
+    adj = fem.distance2_adjacency(S.shape[0], S.indptr, S.indices)
+    col = fem.distance2_colors(S.indptr, S.indices, S.shape[0])
+    for i in range(S.shape[0]):
+        earlier = {int(col[j]) for j in adj[i] if j < i}
+        assert col[i] not in earlier
+        assert all(c in earlier for c in range(col[i]))
+
+
+def test_pcg_matches_spsolve():
+    X, E = fem.cube_hex8(3)
+    prob = fem.Problem(fem.Hex8(), X, E, fem.LinearElasticity(0.4, 0.6))
+    fixed = np.where(np.isclose(X[:, 0], 0.0))[0]
+    L = fem.Lifter(prob.n, fem.Fixed((fixed[:, None] * 3 + np.arange(3)).ravel()))
+    S = L.adapt_sparsity(fem.pattern_from_mesh(E, X.shape[0], 3))
+    colors = fem.distance2_colors(S.indptr, S.indices, S.shape[0])
+    K = fem.hessian_csr(prob, L, np.zeros(prob.n), S, colors)
+    b = np.random.default_rng(5).standard_normal(K.shape[0])
+    x, it, rel = fem.pcg(K, b, tol=1e-12)
+    np.testing.assert_allclose(x, spla.spsolve(K.tocsc(), b), rtol=1e-9, atol=1e-12)
+    assert rel <= 1e-12
+
+
+def test_tet4_generator_matches_reference_layout():
+    """tests/test_sparse_tracer.py:29-70 connectivity; positive volumes, total volume 1."""
+    X, E = fem.box_tet4(3, 2, 2)
+    assert E.shape == (3 * 2 * 2 * 6, 4)
+    np.testing.assert_array_equal(E[0], [0, 1, 5, 17])      # n0, n1, n3, n7 of cell (0,0,0)
+    w = fem.integration_weights(fem.Tet4(), X, E)
+    assert np.all(w > 0)
+    np.testing.assert_allclose(w.sum(), 1.0, rtol=1e-14)
+    Xj, _ = fem.box_tet4(4, 4, 4, jitter=0.2, seed=0)
+    assert np.all(fem.integration_weights(fem.Tet4(), Xj, fem.box_tet4(4, 4, 4)[1]) > 0)
