@@ -170,14 +170,30 @@ int tatva_rows_cols(int64_t n, const int32_t *indptr, const int32_t *indices,
 int tatva_spmv_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
                    const double *data, const double *x, double *y, void *stream);
 
+/* Node-block view of a CSR pattern (no counterpart in the reference): when the
+ * rows come in groups of block_size (one node) sharing one column list of
+ * aligned block_size-wide blocks, writes one block column per block into
+ * block_cols (nnz / block_size^2 entries) and sets *ok = 1; otherwise *ok = 0.
+ * The CSR arrays are not changed.  Setup call: returns after the check. */
+int tatva_csr_block_view(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
+                         int32_t block_size, int32_t *block_cols, int32_t *ok, void *stream);
+
+/* y = A x through the node-block view (same result as tatva_spmv_csr). */
+int tatva_spmv_bcsr(int64_t n, const int32_t *indptr, const int32_t *block_cols,
+                    int32_t block_size, const double *data, const double *x, double *y,
+                    void *stream);
+
 /* Jacobi-PCG on a CSR matrix in FP64 (no counterpart in the reference; it
  * replaces the user-side linear solve, cf. spsolve in tatva/utils.py:186-198).
- * x holds x0 on entry.  Iterations run in hipGraph-captured chunks of `chunk`. */
+ * x holds x0 on entry.  Iterations run in hipGraph-captured chunks of `chunk`.
+ * With block_cols (from tatva_csr_block_view) the operator reads the CSR data
+ * through the node-block view. */
 int64_t tatva_pcg_csr_workspace(int64_t n);
 int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
-                  const double *data, const double *b, double *x, double tol, int32_t maxiter,
-                  int32_t chunk, void *workspace, int64_t workspace_bytes,
-                  tatva_pcg_stats *stats, void *stream);
+                  const double *data, const int32_t *block_cols, int32_t block_size,
+                  const double *b, double *x, double tol, int32_t maxiter, int32_t chunk,
+                  void *workspace, int64_t workspace_bytes, tatva_pcg_stats *stats,
+                  void *stream);
 
 /* Jacobi-PCG with the matrix-free FP32 Hessian-vector product of (mesh, density)
  * at the FP32 state u (full vector); b, x, diag are reduced FP32 vectors. */

@@ -66,8 +66,12 @@ _SIGS = {
     "tatva_rows_cols": (c_int, [c_i64, c_void, c_void, c_void, c_void, c_void, c_void]),
     "tatva_spmv_csr": (c_int, [c_i64, c_i64, c_void, c_void, c_void, c_void, c_void, c_void]),
     "tatva_pcg_csr_workspace": (c_i64, [c_i64]),
-    "tatva_pcg_csr": (c_int, [c_i64, c_i64, c_void, c_void, c_void, c_void, c_void, c_dbl, c_i32,
-                              c_i32, c_void, c_i64, ctypes.POINTER(PcgStats), c_void]),
+    "tatva_pcg_csr": (c_int, [c_i64, c_i64, c_void, c_void, c_void, c_void, c_i32, c_void, c_void,
+                              c_dbl, c_i32, c_i32, c_void, c_i64, ctypes.POINTER(PcgStats),
+                              c_void]),
+    "tatva_csr_block_view": (c_int, [c_i64, c_i64, c_void, c_void, c_i32, c_void,
+                                     ctypes.POINTER(c_i32), c_void]),
+    "tatva_spmv_bcsr": (c_int, [c_i64, c_void, c_void, c_i32, c_void, c_void, c_void, c_void]),
     "tatva_pcg_matfree_workspace": (c_i64, [c_void, c_i64]),
     "tatva_pcg_matfree_f32": (c_int, [c_void, ctypes.POINTER(Density), c_void, c_void, c_i64, c_void,
                                       c_void, c_void, c_dbl, c_i32, c_i32, c_void, c_i64,

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <atomic>
+#include <functional>
 #include <mutex>
 
 #include "common.hpp"
@@ -101,6 +102,145 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv_dot(int64_t n, const int32_
       }
       *counter = 0u;
     }
+  }
+}
+
+// ------------------------------------------------------ node-block SpMV ----
+//
+// FEM patterns with B dofs per node have rows in groups of B (one node) that
+// share one column list made of aligned B-wide blocks.  The CSR arrays stay as
+// they are; a derived list holds one int32 block column per BxB block, so the
+// operator reads 8 + 4/B^2 bytes per non-zero instead of 12.  For block row I:
+//   base = indptr[B I], L = indptr[B I + 1] - base (row length, same for its
+//   B rows), row c of the node starts at base + c L, and its j-th block
+//   column is bcol[base / B^2 + j].
+// 32 lanes per block row; lane j owns block j (B x B values, B x entries).
+constexpr int kBLanes = 32;
+
+template <int B>
+__device__ __forceinline__ void bcsr_block_row(int64_t I, int lane, const int32_t *__restrict__ indptr,
+                                               const int32_t *__restrict__ bcol,
+                                               const double *__restrict__ data,
+                                               const double *__restrict__ x, double acc[B]) {
+  const int32_t base = indptr[B * I];
+  const int32_t L = indptr[B * I + 1] - base;
+  const int32_t nb = L / B;
+  const int32_t k0 = base / (B * B);
+#pragma unroll
+  for (int c = 0; c < B; ++c) acc[c] = 0.0;
+  for (int32_t j = lane; j < nb; j += kBLanes) {
+    const int64_t m = bcol[k0 + j];
+    double xv[B];
+#pragma unroll
+    for (int cm = 0; cm < B; ++cm) xv[cm] = x[m * B + cm];
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+      const double *row = data + base + (int64_t)c * L + (int64_t)j * B;
+#pragma unroll
+      for (int cm = 0; cm < B; ++cm) acc[c] += row[cm] * xv[cm];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < B; ++c)
+#pragma unroll
+    for (int off = kBLanes / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, kBLanes);
+}
+
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_bspmv(int64_t nbr, const int32_t *__restrict__ indptr,
+                                                  const int32_t *__restrict__ bcol,
+                                                  const double *__restrict__ data,
+                                                  const double *__restrict__ x,
+                                                  const double *__restrict__ b, double *__restrict__ y) {
+  constexpr int G = kBlock / kBLanes;
+  const int lane = threadIdx.x % kBLanes;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t stride = (int64_t)gridDim.x * G;
+  for (int64_t I = lb * G + threadIdx.x / kBLanes; I < nbr; I += stride) {
+    double acc[B];
+    bcsr_block_row<B>(I, lane, indptr, bcol, data, x, acc);
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < B; ++c) y[B * I + c] = b ? b[B * I + c] - acc[c] : acc[c];
+    }
+  }
+}
+
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_pcg_bspmv_dot(int64_t nbr, const int32_t *__restrict__ indptr,
+                                                          const int32_t *__restrict__ bcol,
+                                                          const double *__restrict__ data,
+                                                          const double *__restrict__ p,
+                                                          double *__restrict__ Ap,
+                                                          double *__restrict__ partials,
+                                                          unsigned int *__restrict__ counter,
+                                                          PcgState *__restrict__ st) {
+  if (st->done) return;
+  constexpr int G = kBlock / kBLanes;
+  const int lane = threadIdx.x % kBLanes;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t stride = (int64_t)gridDim.x * G;
+  double dot = 0.0;
+  for (int64_t I = lb * G + threadIdx.x / kBLanes; I < nbr; I += stride) {
+    double acc[B];
+    bcsr_block_row<B>(I, lane, indptr, bcol, data, p, acc);
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        Ap[B * I + c] = acc[c];
+        dot += acc[c] * p[B * I + c];
+      }
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = dot;
+  if (last_block(counter)) {
+    const double pAp = block_sum_array(partials, gridDim.x);
+    if (threadIdx.x == 0) {
+      if (!(pAp > 0.0) || !isfinite(pAp)) {
+        st->done = 1;
+        st->status = TATVA_EBREAKDOWN;
+      } else {
+        st->alpha = st->rz / pAp;
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+// Derives the block column list; flags *bad if the pattern is not made of
+// aligned BxB blocks shared by the B rows of each node.
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_block_view(int64_t nbr, const int32_t *__restrict__ indptr,
+                                                       const int32_t *__restrict__ indices,
+                                                       int32_t *__restrict__ bcol,
+                                                       int *__restrict__ bad) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nbr) return;
+  const int32_t base = indptr[B * I];
+  const int32_t L = indptr[B * I + 1] - base;
+  if (L % B != 0 || base % (B * B) != 0) {
+    *bad = 1;
+    return;
+  }
+  for (int c = 1; c < B; ++c)
+    if (indptr[B * I + c] != base + c * L || indptr[B * I + c + 1] != base + (c + 1) * L) {
+      *bad = 1;
+      return;
+    }
+  for (int32_t j = 0; j < L / B; ++j) {
+    const int32_t m = indices[base + j * B];
+    if (m % B != 0) {
+      *bad = 1;
+      return;
+    }
+    for (int c = 0; c < B; ++c)
+      for (int cm = 0; cm < B; ++cm)
+        if (indices[base + c * L + j * B + cm] != m + cm) {
+          *bad = 1;
+          return;
+        }
+    bcol[base / (B * B) + j] = m / B;
   }
 }
 
@@ -407,11 +547,12 @@ static WsLayout ws_layout(int64_t n, int nvec, size_t esz) {
 }
 
 // Runs captured chunks until the device state reports done; fills stats.
-// Runs captured chunks until the device state reports done; fills stats.  With
-// profiling, the chunk's first operator application is bracketed by
-// prof_a/prof_b and its duration is read after each chunk.
+// With profiling, the graph holds the chunk minus its first operator
+// application, which `sampled_op` launches eagerly on the same stream between
+// prof_a and prof_b; its duration is read after each chunk.
 static int run_chunks(DeviceRes *res, hipGraphExec_t exec, PcgState *st, int maxiter, int chunk,
-                      bool profiling, tatva_pcg_stats *stats) {
+                      bool profiling, const std::function<int()> &sampled_op,
+                      tatva_pcg_stats *stats) {
   const int max_launch = (maxiter + chunk - 1) / chunk + 1;
   double op_ms = 0.0;
   int samples = 0;
@@ -422,12 +563,17 @@ static int run_chunks(DeviceRes *res, hipGraphExec_t exec, PcgState *st, int max
     TATVA_TRY_HIP(hipStreamSynchronize(res->cap));
     if (launched && profiling) {
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, res->prof_a, res->prof_b) == hipSuccess) {
-        op_ms += ms;
-        ++samples;
-      }
+      TATVA_TRY_HIP(hipEventElapsedTime(&ms, res->prof_a, res->prof_b));
+      op_ms += ms;
+      ++samples;
     }
     if (res->host_state->done) break;
+    if (profiling) {
+      TATVA_TRY_HIP(hipEventRecord(res->prof_a, res->cap));
+      const int rc = sampled_op();
+      if (rc) return rc;
+      TATVA_TRY_HIP(hipEventRecord(res->prof_b, res->cap));
+    }
     TATVA_TRY_HIP(hipGraphLaunch(exec, res->cap));
     launched = true;
   }
@@ -463,6 +609,56 @@ extern "C" int tatva_spmv_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   });
 }
 
+extern "C" int tatva_csr_block_view(int64_t n, int64_t nnz, const int32_t *indptr,
+                                    const int32_t *indices, int32_t block_size,
+                                    int32_t *block_cols, int32_t *ok, void *stream) {
+  TATVA_REQUIRE(n > 0 && nnz >= 0 && indptr && indices && block_cols && ok, "invalid arguments");
+  TATVA_REQUIRE(block_size == 2 || block_size == 3, "block_size must be 2 or 3");
+  *ok = 0;
+  if (n % block_size != 0 || nnz % (block_size * block_size) != 0) return TATVA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  // setup call: a 4-byte flag, freed before returning
+  int *bad = nullptr;
+  TATVA_TRY_HIP(hipMalloc(&bad, sizeof(int)));
+  int h_bad = 0;
+  hipError_t e = hipMemsetAsync(bad, 0, sizeof(int), s);
+  const int64_t nbr = n / block_size;
+  if (e == hipSuccess) {
+    if (block_size == 3)
+      k_block_view<3><<<grid_for(nbr), kBlock, 0, s>>>(nbr, indptr, indices, block_cols, bad);
+    else
+      k_block_view<2><<<grid_for(nbr), kBlock, 0, s>>>(nbr, indptr, indices, block_cols, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(bad);
+  if (e != hipSuccess) {
+    set_error(std::string("block view: ") + hipGetErrorString(e));
+    return TATVA_EHIP;
+  }
+  *ok = h_bad ? 0 : 1;
+  return TATVA_OK;
+}
+
+extern "C" int tatva_spmv_bcsr(int64_t n, const int32_t *indptr, const int32_t *block_cols,
+                               int32_t block_size, const double *data, const double *x, double *y,
+                               void *stream) {
+  TATVA_REQUIRE(n > 0 && indptr && block_cols && data && x && y, "invalid arguments");
+  TATVA_REQUIRE((block_size == 2 || block_size == 3) && n % block_size == 0,
+                "block_size must be 2 or 3 and divide n");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nbr = n / block_size;
+  if (block_size == 3)
+    k_bspmv<3><<<spmv_grid(nbr, kBLanes), kBlock, 0, s>>>(nbr, indptr, block_cols, data, x,
+                                                         nullptr, y);
+  else
+    k_bspmv<2><<<spmv_grid(nbr, kBLanes), kBlock, 0, s>>>(nbr, indptr, block_cols, data, x,
+                                                         nullptr, y);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
 extern "C" int tatva_set_profiling(int enable) {
   g_profiling.store(enable ? 1 : 0);
   return TATVA_OK;
@@ -471,12 +667,16 @@ extern "C" int tatva_set_profiling(int enable) {
 extern "C" int64_t tatva_pcg_csr_workspace(int64_t n) { return (int64_t)ws_layout(n, 4, sizeof(double)).total; }
 
 extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
-                             const int32_t *indices, const double *data, const double *b,
+                             const int32_t *indices, const double *data,
+                             const int32_t *block_cols, int32_t block_size, const double *b,
                              double *x, double tol, int32_t maxiter, int32_t chunk,
                              void *workspace, int64_t ws_bytes, tatva_pcg_stats *stats,
                              void *stream) {
   TATVA_REQUIRE(n > 0 && nnz > 0 && indptr && indices && data && b && x, "invalid arguments");
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
+  if (block_cols)
+    TATVA_REQUIRE((block_size == 2 || block_size == 3) && n % block_size == 0,
+                  "block view needs block_size 2 or 3 dividing n");
   TATVA_REQUIRE(tol >= 0.0, "tol must be non-negative");
   const WsLayout L = ws_layout(n, 4, sizeof(double));
   TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
@@ -502,23 +702,41 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   const int lpr = choose_lpr(n, nnz);
   const int gv = reduce_grid(n);
   const bool profiling = g_profiling.load() != 0;
+  const int nblk = block_cols ? block_size : 1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  rc = with_lpr(lpr, [&](auto lc) -> int {
+  // The operator: node-block SpMV when a block view is given, else CSR.
+  auto apply_residual = [&](auto lc) {
     constexpr int LP = decltype(lc)::value;
-    const int gs = spmv_grid(n, LP);
+    if (nblk == 3)
+      k_bspmv<3><<<spmv_grid(n / 3, kBLanes), kBlock, 0, s>>>(n / 3, indptr, block_cols, data, x, b, r);
+    else if (nblk == 2)
+      k_bspmv<2><<<spmv_grid(n / 2, kBLanes), kBlock, 0, s>>>(n / 2, indptr, block_cols, data, x, b, r);
+    else
+      k_csr_residual<LP><<<spmv_grid(n, LP), kBlock, 0, s>>>(n, indptr, indices, data, x, b, r);
+  };
+  auto apply_spmv_dot = [&](auto lc) {
+    constexpr int LP = decltype(lc)::value;
+    if (nblk == 3)
+      k_pcg_bspmv_dot<3><<<spmv_grid(n / 3, kBLanes), kBlock, 0, s>>>(
+          n / 3, indptr, block_cols, data, p, Ap, partials, counters, st);
+    else if (nblk == 2)
+      k_pcg_bspmv_dot<2><<<spmv_grid(n / 2, kBLanes), kBlock, 0, s>>>(
+          n / 2, indptr, block_cols, data, p, Ap, partials, counters, st);
+    else
+      k_pcg_spmv_dot<LP><<<spmv_grid(n, LP), kBlock, 0, s>>>(n, indptr, indices, data, p, Ap,
+                                                             partials, counters, st);
+  };
+  rc = with_lpr(lpr, [&](auto lc) -> int {
     k_state_reset<<<1, 64, 0, s>>>(st);
     k_csr_diag_inv<<<grid_for(n), kBlock, 0, s>>>(n, indptr, indices, data, dinv, st);
-    k_csr_residual<LP><<<gs, kBlock, 0, s>>>(n, indptr, indices, data, x, b, r);
+    apply_residual(lc);
     k_pcg_init<double><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, tol, maxiter, partials,
                                              counters, st);
     TATVA_CHECK_LAUNCH();
     TATVA_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < chunk; ++i) {
-      if (i == 0 && profiling) (void)hipEventRecord(res->prof_a, s);
-      k_pcg_spmv_dot<LP><<<gs, kBlock, 0, s>>>(n, indptr, indices, data, p, Ap, partials,
-                                               counters, st);
-      if (i == 0 && profiling) (void)hipEventRecord(res->prof_b, s);
+      if (!(i == 0 && profiling)) apply_spmv_dot(lc);  // sampled launch runs eagerly
       k_pcg_update<double><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, partials,
                                                  counters + 1, st);
       k_pcg_pupdate<double><<<gv, kBlock, 0, s>>>(n, p, r, dinv, st);
@@ -531,7 +749,14 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
     TATVA_TRY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     return TATVA_OK;
   });
-  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, stats);
+  const std::function<int()> sampled = [&]() -> int {
+    return with_lpr(lpr, [&](auto lc) -> int {
+      apply_spmv_dot(lc);
+      TATVA_CHECK_LAUNCH();
+      return TATVA_OK;
+    });
+  };
+  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, sampled, stats);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
@@ -585,9 +810,8 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   hipGraphExec_t exec = nullptr;
   TATVA_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < chunk && rc == TATVA_OK; ++i) {
-    if (i == 0 && profiling) (void)hipEventRecord(res->prof_a, s);
-    rc = launch_hvp_f32(m, d, u, p, dof_map, Ap, s, &st->done);
-    if (i == 0 && profiling) (void)hipEventRecord(res->prof_b, s);
+    if (!(i == 0 && profiling))  // sampled launch runs eagerly
+      rc = launch_hvp_f32(m, d, u, p, dof_map, Ap, s, &st->done);
     k_pcg_dot_alpha<float><<<gv, kBlock, 0, s>>>(n, p, Ap, partials, counters, st);
     k_pcg_update<float><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, partials, counters + 1, st);
     k_pcg_pupdate<float><<<gv, kBlock, 0, s>>>(n, p, r, dinv, st);
@@ -604,7 +828,10 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
       rc = TATVA_EHIP;
     }
   }
-  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, stats);
+  const std::function<int()> sampled = [&]() -> int {
+    return launch_hvp_f32(m, d, u, p, dof_map, Ap, s, &st->done);
+  };
+  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, sampled, stats);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));

@@ -71,6 +71,8 @@ class ColoredMatrix:
     n_colors: int = 0
     _host_indptr: np.ndarray | None = field(default=None, repr=False)
     _host_indices: np.ndarray | None = field(default=None, repr=False)
+    # pattern-derived data (block views), shared by copies made with replace()
+    _cache: dict = field(default_factory=dict, repr=False)
 
     @classmethod
     def from_csr(cls, csr_matrix, colors=None) -> "ColoredMatrix":
@@ -115,11 +117,37 @@ class ColoredMatrix:
         """Dense host copy (sparse/base.py:102-105)."""
         return self.to_csr().toarray()
 
-    def matvec(self, x):
-        """y = A x on the GPU (CSR SpMV kernel)."""
+    def block_view(self, block_size: int):
+        """Node-block view of the pattern: (block_cols tensor, block_size), or None
+        when rows do not come in groups sharing aligned block columns.  Cached."""
+        cache = self._cache
+        if block_size in cache:
+            return cache[block_size]
+        view = None
+        n = self.shape[0]
+        if block_size in (2, 3) and n % block_size == 0 and self.nnz % (block_size ** 2) == 0:
+            t = nat.torch()
+            bc = t.empty(self.nnz // block_size ** 2, dtype=t.int32, device=self.indices.device)
+            ok = ctypes.c_int32()
+            nat.check(nat.lib().tatva_csr_block_view(n, self.nnz, nat.dptr(self.indptr),
+                                                     nat.dptr(self.indices), block_size,
+                                                     nat.dptr(bc), ctypes.byref(ok), nat.stream()))
+            if ok.value:
+                view = (bc, block_size)
+        cache[block_size] = view
+        return view
+
+    def matvec(self, x, *, block_size: int | None = None):
+        """y = A x on the GPU (CSR SpMV, or the node-block view when block_size is given)."""
         t = nat.torch()
         xv = nat.to_device(x).reshape(-1).contiguous()
         y = t.empty(self.shape[0], dtype=t.float64, device=xv.device)
+        view = self.block_view(block_size) if block_size else None
+        if view is not None:
+            nat.check(nat.lib().tatva_spmv_bcsr(self.shape[0], nat.dptr(self.indptr),
+                                                nat.dptr(view[0]), view[1], nat.dptr(self.data),
+                                                nat.dptr(xv), nat.dptr(y), nat.stream()))
+            return y
         nat.check(nat.lib().tatva_spmv_csr(self.shape[0], self.nnz, nat.dptr(self.indptr),
                                            nat.dptr(self.indices), nat.dptr(self.data),
                                            nat.dptr(xv), nat.dptr(y), nat.stream()))
