@@ -96,27 +96,46 @@ __device__ __forceinline__ T block_sum(T v) {
   return s;
 }
 
-// Returns true in every thread of the last block to arrive.  Callers write
-// their block partials before calling; the last block then reads all partials
-// in a fixed order, so the result does not depend on arrival order.
-__device__ __forceinline__ bool last_block(unsigned int *counter) {
-  __shared__ bool is_last;
-  __threadfence();
+// Deterministic block sum whose result every thread receives.
+template <class T>
+__device__ __forceinline__ T block_sum_all(T v) {
+  __shared__ T smem[kBlock / kWave + 1];
+  v = wave_sum(v);
+  const int lane = threadIdx.x % kWave;
+  const int wid = threadIdx.x / kWave;
+  if (lane == 0) smem[wid] = v;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    is_last = (prev == gridDim.x - 1);
+    T s = T(0);
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) s += smem[w];
+    smem[kBlock / kWave] = s;
   }
   __syncthreads();
-  if (is_last) __threadfence();
-  return is_last;
+  const T s = smem[kBlock / kWave];
+  __syncthreads();
+  return s;
 }
 
-// Sum of partials[0..n) by one block in a fixed order; result valid in thread 0.
+// Sum of per-block partials written by an earlier kernel, in a fixed order;
+// every thread receives the result.  Cross-block reductions go through a
+// kernel boundary (which publishes the partials to every XCD) instead of an
+// in-kernel device-scope fence and atomic: on CDNA4 with one L2 per XCD such a
+// fence writes back the L2 and costs far more than re-reading a few KB of
+// partials in each block.
 template <class T>
-__device__ __forceinline__ T block_sum_array(const T *partials, int n) {
+__device__ __forceinline__ T sum_partials(const T *partials, int n) {
   T v = T(0);
   for (int i = threadIdx.x; i < n; i += blockDim.x) v += partials[i];
-  return block_sum(v);
+  return block_sum_all(v);
+}
+
+// out[0] = sum (subtract == 0) or out[0] -= sum of partials[0..n): one block.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_finalize_sum(const T *__restrict__ partials, int n,
+                                                         T *__restrict__ out, int subtract) {
+  const T s = sum_partials(partials, n);
+  if (threadIdx.x == 0) out[0] = subtract ? out[0] - s : s;
 }
 
 }  // namespace tatva

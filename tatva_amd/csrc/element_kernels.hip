@@ -154,46 +154,29 @@ __global__ __launch_bounds__(kBlock) void k_integrate_elem(int64_t n_elem, const
   }
 }
 
-// out[0] = sum_i x[i*stride + offset] (grid-stride, deterministic).
+// partials[block] = sum over the block's grid-stride share of x[i*stride + offset];
+// k_finalize_sum then adds the partials in a fixed order.
 __global__ __launch_bounds__(kBlock) void k_strided_sum(int64_t n, const double *__restrict__ x,
                                                         int stride, int offset,
-                                                        double *__restrict__ partials,
-                                                        unsigned int *__restrict__ counter,
-                                                        double *__restrict__ out) {
+                                                        double *__restrict__ partials) {
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     s += x[i * stride + offset];
   s = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
-  if (last_block(counter)) {
-    const double t = block_sum_array(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      out[0] = t;
-      *counter = 0u;
-    }
-  }
 }
 
-// out[0] -= sum_i a[i] * b[i]
-__global__ __launch_bounds__(kBlock) void k_dot_sub(int64_t n, const double *__restrict__ a,
-                                                    const double *__restrict__ b,
-                                                    double *__restrict__ partials,
-                                                    unsigned int *__restrict__ counter,
-                                                    double *__restrict__ out) {
+// partials[block] = block share of sum_i a[i] * b[i]
+__global__ __launch_bounds__(kBlock) void k_dot_partials(int64_t n, const double *__restrict__ a,
+                                                         const double *__restrict__ b,
+                                                         double *__restrict__ partials) {
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     s += a[i] * b[i];
   s = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
-  if (last_block(counter)) {
-    const double t = block_sum_array(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      out[0] -= t;
-      *counter = 0u;
-    }
-  }
 }
 
 // ----------------------------------------------------------- densities -----
@@ -220,9 +203,7 @@ template <int KIND, class Dens>
 __global__ __launch_bounds__(kBlock) void k_energy(int64_t n_elem, const double *__restrict__ coords,
                                                    const int32_t *__restrict__ conn, QuadRule quad,
                                                    Dens dens, const double *__restrict__ u,
-                                                   double *__restrict__ partials,
-                                                   unsigned int *__restrict__ counter,
-                                                   double *__restrict__ out) {
+                                                   double *__restrict__ partials) {
   using El = Elem<KIND>;
   constexpr int D = El::D, NNE = El::NNE, NC = Dens::NC;
   double acc = 0.0;
@@ -263,13 +244,6 @@ __global__ __launch_bounds__(kBlock) void k_energy(int64_t n_elem, const double 
   }
   acc = block_sum(acc);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
-  if (last_block(counter)) {
-    const double t = block_sum_array(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      out[0] = t;
-      *counter = 0u;
-    }
-  }
 }
 
 // Element residual vectors r_e[a][c] = sum_q w (P(G) B)_ac - f w N_a.
@@ -601,8 +575,6 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     return fail(err, "hipMalloc scratch");
   if ((err = hipMalloc(&m->partials, kMaxReduceBlocks * 4 * sizeof(double))) != hipSuccess)
     return fail(err, "hipMalloc partials");
-  if ((err = hipMalloc(&m->counters, 8 * sizeof(unsigned int))) != hipSuccess)
-    return fail(err, "hipMalloc counters");
   if ((err = hipMemcpy(m->coords, coords, n_nodes * dim * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(err, "copy coords");
   if ((err = hipMemcpy(m->conn, elements, total * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
@@ -611,8 +583,6 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     return fail(err, "copy inv_ptr");
   if ((err = hipMemcpy(m->inv_list, list.data(), total * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(err, "copy inv_list");
-  if ((err = hipMemset(m->counters, 0, 8 * sizeof(unsigned int))) != hipSuccess)
-    return fail(err, "zero counters");
   *out = m;
   return TATVA_OK;
 }
@@ -627,7 +597,6 @@ extern "C" int tatva_mesh_destroy(tatva_mesh *m) {
   if (m->inv_list) (void)hipFree(m->inv_list);
   if (m->scratch) (void)hipFree(m->scratch);
   if (m->partials) (void)hipFree(m->partials);
-  if (m->counters) (void)hipFree(m->counters);
   delete m;
   return TATVA_OK;
 }
@@ -699,8 +668,8 @@ extern "C" int tatva_integrate(tatva_mesh *m, int nc, int nodal, const double *v
   if (total) {
     const int g = reduce_grid(m->n_elements);
     for (int c = 0; c < nc; ++c) {
-      k_strided_sum<<<g, kBlock, 0, s>>>(m->n_elements, pe, nc, c, m->partials, m->counters,
-                                         total + c);
+      k_strided_sum<<<g, kBlock, 0, s>>>(m->n_elements, pe, nc, c, m->partials);
+      k_finalize_sum<double><<<1, kBlock, 0, s>>>(m->partials, g, total + c, 0);
       TATVA_CHECK_LAUNCH();
     }
   }
@@ -737,8 +706,10 @@ extern "C" int tatva_energy(tatva_mesh *m, const tatva_density *d, const double 
     constexpr int K = decltype(kc)::value;
     return with_density<Elem<K>::D, double>(d, [&](auto dens) -> int {
       using Dens = decltype(dens);
-      k_energy<K, Dens><<<reduce_grid(m->n_elements), kBlock, 0, s>>>(
-          m->n_elements, m->coords, m->conn, m->quad, dens, u, m->partials, m->counters, out);
+      const int g = reduce_grid(m->n_elements);
+      k_energy<K, Dens><<<g, kBlock, 0, s>>>(m->n_elements, m->coords, m->conn, m->quad, dens, u,
+                                             m->partials);
+      k_finalize_sum<double><<<1, kBlock, 0, s>>>(m->partials, g, out, 0);
       TATVA_CHECK_LAUNCH();
       return TATVA_OK;
     });
@@ -746,7 +717,9 @@ extern "C" int tatva_energy(tatva_mesh *m, const tatva_density *d, const double 
   if (rc) return rc;
   if (f_ext) {
     const int64_t n = m->n_nodes * density_ncomp(m, d);
-    k_dot_sub<<<reduce_grid(n), kBlock, 0, s>>>(n, f_ext, u, m->partials, m->counters, out);
+    const int g = reduce_grid(n);
+    k_dot_partials<<<g, kBlock, 0, s>>>(n, f_ext, u, m->partials);
+    k_finalize_sum<double><<<1, kBlock, 0, s>>>(m->partials, g, out, 1);
     TATVA_CHECK_LAUNCH();
   }
   return TATVA_OK;

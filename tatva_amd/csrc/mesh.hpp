@@ -25,7 +25,6 @@ struct tatva_mesh {
   void *scratch;         // E * nne * dim doubles: per-element vectors before the node gather
   int64_t scratch_bytes;
   double *partials;      // reduction partials, kMaxReduceBlocks * 4
-  unsigned int *counters;  // last-block counters, 8
   int max_incident;      // largest number of elements sharing a node
 };
 

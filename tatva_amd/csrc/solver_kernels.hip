@@ -1,19 +1,24 @@
 // CSR SpMV, Jacobi-PCG (FP64 CSR and FP32 matrix-free) with hipGraph-captured
 // iteration chunks, and the small vector kernels of the Newton loop.
 //
-// PCG iteration (one chunk step = 3 kernels for CSR, 4 for matrix-free):
-//   Ap = A p, p.Ap            -> last block: alpha = rz / p.Ap
-//   x += alpha p, r -= alpha Ap, z = D^-1 r, r.z, r.r
-//                             -> last block: convergence test, beta = rz'/rz
-//   p = D^-1 r + beta p
-// Scalars live in a device-side state record; every kernel returns at once
-// when the state says "done", so a captured chunk may overrun convergence
-// without changing the result.  Reductions use fixed trees: deterministic.
+// One PCG iteration i = three kernels (four with the matrix-free operator):
+//   K1  Ap = A p; each block writes its share of p.Ap            -> part_pap
+//   K2  every block re-sums part_pap and part_rz[q^1] (fixed order) to get
+//       alpha = rz_i / p.Ap; x += alpha p, r -= alpha Ap; shares of
+//       r.(D^-1 r) and r.r                                    -> part_rz[q], part_rr[q]
+//   K3  every block re-sums part_rz[q], part_rz[q^1], part_rr[q]: convergence
+//       test, beta = rz_{i+1}/rz_i, p = D^-1 r + beta p
+// with q = i mod 2 (a kernel argument; chunks have an even length).  There is
+// no in-kernel fence or atomic: cross-block sums cross a kernel boundary, and
+// every block computes identical scalars, so the iteration is deterministic.
+// Block 0 alone writes the state record (done/status/iter/rr) that later
+// kernels test and the host reads between chunks; each kernel returns at once
+// when the state says done, so a chunk may overrun convergence harmlessly.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
-#include <atomic>
 #include <functional>
 #include <mutex>
 
@@ -23,9 +28,6 @@
 namespace tatva {
 
 struct PcgState {
-  double rz;
-  double alpha;
-  double beta;
   double rr;
   double bb;
   double tol2;
@@ -34,6 +36,27 @@ struct PcgState {
   int status;
   int maxiter;
 };
+
+// Partial-sum buffers of one solve: pap, rz[2], rr[2], bb (kMaxReduceBlocks each).
+struct PcgParts {
+  double *pap;
+  double *rz0, *rz1;
+  double *rr0, *rr1;
+  double *bb;
+  double *rz(int q) const { return q ? rz1 : rz0; }
+  double *rr(int q) const { return q ? rr1 : rr0; }
+};
+
+// Vector kernels use at most this many blocks (4 per CU): the partials each
+// block re-sums stay a few KB.
+constexpr int kVecBlocks = 1024;
+
+inline int vec_grid(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > kVecBlocks) b = kVecBlocks;
+  return (int)b;
+}
 
 // ----------------------------------------------------------------- SpMV ----
 
@@ -51,31 +74,32 @@ __device__ __forceinline__ double csr_row_dot(int64_t row, int lane, const int32
   return s;
 }
 
+// y = A x, or y = b - A x when b != NULL.
 template <int LPR>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t *__restrict__ indptr,
                                                  const int32_t *__restrict__ indices,
                                                  const double *__restrict__ data,
-                                                 const double *__restrict__ x, double *__restrict__ y) {
+                                                 const double *__restrict__ x,
+                                                 const double *__restrict__ b, double *__restrict__ y) {
   constexpr int G = kBlock / LPR;
   const int lane = threadIdx.x % LPR;
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const int64_t stride = (int64_t)gridDim.x * G;
   for (int64_t row = lb * G + threadIdx.x / LPR; row < n; row += stride) {
     const double s = csr_row_dot<LPR>(row, lane, indptr, indices, data, x);
-    if (lane == 0) y[row] = s;
+    if (lane == 0) y[row] = b ? b[row] - s : s;
   }
 }
 
-// Ap = A p fused with the p.Ap reduction; the last block sets alpha.
+// K1 (CSR): Ap = A p and the block's share of p.Ap.
 template <int LPR>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv_dot(int64_t n, const int32_t *__restrict__ indptr,
-                                                         const int32_t *__restrict__ indices,
-                                                         const double *__restrict__ data,
-                                                         const double *__restrict__ p,
-                                                         double *__restrict__ Ap,
-                                                         double *__restrict__ partials,
-                                                         unsigned int *__restrict__ counter,
-                                                         PcgState *__restrict__ st) {
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int64_t n, const int32_t *__restrict__ indptr,
+                                                     const int32_t *__restrict__ indices,
+                                                     const double *__restrict__ data,
+                                                     const double *__restrict__ p,
+                                                     double *__restrict__ Ap,
+                                                     double *__restrict__ part_pap,
+                                                     const PcgState *__restrict__ st) {
   if (st->done) return;
   constexpr int G = kBlock / LPR;
   const int lane = threadIdx.x % LPR;
@@ -90,19 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv_dot(int64_t n, const int32_
     }
   }
   dot = block_sum(dot);
-  if (threadIdx.x == 0) partials[blockIdx.x] = dot;
-  if (last_block(counter)) {
-    const double pAp = block_sum_array(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      if (!(pAp > 0.0) || !isfinite(pAp)) {
-        st->done = 1;
-        st->status = TATVA_EBREAKDOWN;
-      } else {
-        st->alpha = st->rz / pAp;
-      }
-      *counter = 0u;
-    }
-  }
+  if (threadIdx.x == 0) part_pap[blockIdx.x] = dot;
 }
 
 // ------------------------------------------------------ node-block SpMV ----
@@ -166,15 +178,15 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(int64_t nbr, const int32_t *__
   }
 }
 
+// K1 (node-block view): Ap = A p and the block's share of p.Ap.
 template <int B>
-__global__ __launch_bounds__(kBlock) void k_pcg_bspmv_dot(int64_t nbr, const int32_t *__restrict__ indptr,
-                                                          const int32_t *__restrict__ bcol,
-                                                          const double *__restrict__ data,
-                                                          const double *__restrict__ p,
-                                                          double *__restrict__ Ap,
-                                                          double *__restrict__ partials,
-                                                          unsigned int *__restrict__ counter,
-                                                          PcgState *__restrict__ st) {
+__global__ __launch_bounds__(kBlock) void k_pcg_bspmv(int64_t nbr, const int32_t *__restrict__ indptr,
+                                                      const int32_t *__restrict__ bcol,
+                                                      const double *__restrict__ data,
+                                                      const double *__restrict__ p,
+                                                      double *__restrict__ Ap,
+                                                      double *__restrict__ part_pap,
+                                                      const PcgState *__restrict__ st) {
   if (st->done) return;
   constexpr int G = kBlock / kBLanes;
   const int lane = threadIdx.x % kBLanes;
@@ -193,19 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_bspmv_dot(int64_t nbr, const int
     }
   }
   dot = block_sum(dot);
-  if (threadIdx.x == 0) partials[blockIdx.x] = dot;
-  if (last_block(counter)) {
-    const double pAp = block_sum_array(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      if (!(pAp > 0.0) || !isfinite(pAp)) {
-        st->done = 1;
-        st->status = TATVA_EBREAKDOWN;
-      } else {
-        st->alpha = st->rz / pAp;
-      }
-      *counter = 0u;
-    }
-  }
+  if (threadIdx.x == 0) part_pap[blockIdx.x] = dot;
 }
 
 // Derives the block column list; flags *bad if the pattern is not made of
@@ -244,114 +244,157 @@ __global__ __launch_bounds__(kBlock) void k_block_view(int64_t nbr, const int32_
   }
 }
 
-// p.Ap for the matrix-free operator (FP32 vectors, FP64 accumulation).
+// ------------------------------------------------------------ PCG steps ----
+
+// K1 for a matrix-free operator: the block's share of p.Ap.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pcg_dot_alpha(int64_t n, const T *__restrict__ p,
-                                                          const T *__restrict__ Ap,
-                                                          double *__restrict__ partials,
-                                                          unsigned int *__restrict__ counter,
-                                                          PcgState *__restrict__ st) {
+__global__ __launch_bounds__(kBlock) void k_pcg_dot(int64_t n, const T *__restrict__ p,
+                                                    const T *__restrict__ Ap,
+                                                    double *__restrict__ part_pap,
+                                                    const PcgState *__restrict__ st) {
   if (st->done) return;
   double dot = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     dot += (double)p[i] * (double)Ap[i];
   dot = block_sum(dot);
-  if (threadIdx.x == 0) partials[blockIdx.x] = dot;
-  if (last_block(counter)) {
-    const double pAp = block_sum_array(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      if (!(pAp > 0.0) || !isfinite(pAp)) {
-        st->done = 1;
-        st->status = TATVA_EBREAKDOWN;
-      } else {
-        st->alpha = st->rz / pAp;
-      }
-      *counter = 0u;
-    }
-  }
+  if (threadIdx.x == 0) part_pap[blockIdx.x] = dot;
 }
 
+// K2: alpha, x and r updates, shares of r.z and r.r for parity q.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, T *__restrict__ x, T *__restrict__ r,
                                                        const T *__restrict__ p,
                                                        const T *__restrict__ Ap,
                                                        const T *__restrict__ dinv,
-                                                       double *__restrict__ partials,
-                                                       unsigned int *__restrict__ counter,
+                                                       const double *__restrict__ part_pap, int n_pap,
+                                                       const double *__restrict__ part_rz_old,
+                                                       double *__restrict__ part_rz_new,
+                                                       double *__restrict__ part_rr_new,
                                                        PcgState *__restrict__ st) {
   if (st->done) return;
-  const T alpha = T(st->alpha);
-  double rz = 0.0, rr = 0.0;
+  const int iter = st->iter;
+  const double pap = sum_partials(part_pap, n_pap);
+  const double rz = sum_partials(part_rz_old, gridDim.x);
+  if (!(pap > 0.0) || !isfinite(pap) || iter >= st->maxiter) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->status = (iter >= st->maxiter) ? TATVA_ENOTCONV : TATVA_EBREAKDOWN;
+    }
+    return;
+  }
+  const T alpha = T(rz / pap);
+  double srz = 0.0, srr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * Ap[i];
     r[i] = ri;
-    rz += (double)ri * (double)(dinv[i] * ri);
-    rr += (double)ri * (double)ri;
+    srz += (double)ri * (double)(dinv[i] * ri);
+    srr += (double)ri * (double)ri;
   }
-  rz = block_sum(rz);
-  rr = block_sum(rr);
+  srz = block_sum(srz);
+  srr = block_sum(srr);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = rz;
-    partials[kMaxReduceBlocks + blockIdx.x] = rr;
-  }
-  if (last_block(counter)) {
-    const double rz_new = block_sum_array(partials, gridDim.x);
-    const double rr_new = block_sum_array(partials + kMaxReduceBlocks, gridDim.x);
-    if (threadIdx.x == 0) {
-      st->iter += 1;
-      st->rr = rr_new;
-      if (rr_new <= st->tol2) {
-        st->done = 1;
-        st->status = TATVA_OK;
-      } else if (!isfinite(rr_new)) {
-        st->done = 1;
-        st->status = TATVA_EBREAKDOWN;
-      } else if (st->iter >= st->maxiter) {
-        st->done = 1;
-        st->status = TATVA_ENOTCONV;
-      } else {
-        st->beta = rz_new / st->rz;
-        st->rz = rz_new;
-      }
-      *counter = 0u;
-    }
+    part_rz_new[blockIdx.x] = srz;
+    part_rr_new[blockIdx.x] = srr;
   }
 }
 
+// K3: convergence test and p = D^-1 r + beta p.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict__ p,
                                                         const T *__restrict__ r,
                                                         const T *__restrict__ dinv,
-                                                        const PcgState *__restrict__ st) {
+                                                        const double *__restrict__ part_rz_old,
+                                                        const double *__restrict__ part_rz_new,
+                                                        const double *__restrict__ part_rr_new,
+                                                        PcgState *__restrict__ st) {
   if (st->done) return;
-  const T beta = T(st->beta);
+  const double rz_old = sum_partials(part_rz_old, gridDim.x);
+  const double rz_new = sum_partials(part_rz_new, gridDim.x);
+  const double rr = sum_partials(part_rr_new, gridDim.x);
+  const bool converged = rr <= st->tol2;
+  const bool bad = !isfinite(rr) || !isfinite(rz_new);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter += 1;
+    st->rr = rr;
+    if (converged) {
+      st->done = 1;
+      st->status = TATVA_OK;
+    } else if (bad) {
+      st->done = 1;
+      st->status = TATVA_EBREAKDOWN;
+    }
+  }
+  if (converged || bad) return;
+  const T beta = T(rz_new / rz_old);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
-// r = b - A x (CSR)
-template <int LPR>
-__global__ __launch_bounds__(kBlock) void k_csr_residual(int64_t n, const int32_t *__restrict__ indptr,
-                                                         const int32_t *__restrict__ indices,
-                                                         const double *__restrict__ data,
-                                                         const double *__restrict__ x,
-                                                         const double *__restrict__ b,
-                                                         double *__restrict__ r) {
-  constexpr int G = kBlock / LPR;
-  const int lane = threadIdx.x % LPR;
-  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t stride = (int64_t)gridDim.x * G;
-  for (int64_t row = lb * G + threadIdx.x / LPR; row < n; row += stride) {
-    const double s = csr_row_dot<LPR>(row, lane, indptr, indices, data, x);
-    if (lane == 0) r[row] = b[row] - s;
+// r given: z = D^-1 r, p = z, shares of r.z (into the parity-1 slot, read by
+// iteration 0 as "previous"), r.r and b.b.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T *__restrict__ r,
+                                                     const T *__restrict__ dinv,
+                                                     const T *__restrict__ b, T *__restrict__ p,
+                                                     double *__restrict__ part_rz,
+                                                     double *__restrict__ part_rr,
+                                                     double *__restrict__ part_bb) {
+  double rz = 0.0, rr = 0.0, bb = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T ri = r[i];
+    const T z = dinv[i] * ri;
+    p[i] = z;
+    rz += (double)ri * (double)z;
+    rr += (double)ri * (double)ri;
+    bb += (double)b[i] * (double)b[i];
+  }
+  rz = block_sum(rz);
+  rr = block_sum(rr);
+  bb = block_sum(bb);
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = rz;
+    part_rr[blockIdx.x] = rr;
+    part_bb[blockIdx.x] = bb;
   }
 }
 
-// dinv[row] = 1 / A[row, row]; a missing or zero diagonal flags st->status.
+// One block: state from the initial sums (b == 0 leaves x as given, as the
+// oracle does; an initial residual already below tol needs no iteration).
+__global__ __launch_bounds__(kBlock) void k_pcg_init_state(const double *__restrict__ part_rr,
+                                                           const double *__restrict__ part_bb,
+                                                           int nparts, double tol, int maxiter,
+                                                           PcgState *__restrict__ st) {
+  const double rr = sum_partials(part_rr, nparts);
+  const double bb = sum_partials(part_bb, nparts);
+  if (threadIdx.x == 0) {
+    const int bad = st->status;  // set by the diagonal kernel
+    st->rr = rr;
+    st->bb = bb;
+    st->tol2 = tol * tol * bb;
+    st->iter = 0;
+    st->maxiter = maxiter;
+    int done = 1, status = TATVA_OK;
+    if (bad != TATVA_OK) {
+      status = bad;
+    } else if (bb == 0.0 || rr <= st->tol2) {
+      status = TATVA_OK;
+    } else if (maxiter <= 0) {
+      status = TATVA_ENOTCONV;
+    } else {
+      done = 0;
+    }
+    st->status = status;
+    st->done = done;
+  }
+}
+
+// dinv[row] = 1 / A[row, row] by binary search of the sorted row; a missing or
+// zero diagonal flags st->status.
 __global__ __launch_bounds__(kBlock) void k_csr_diag_inv(int64_t n, const int32_t *__restrict__ indptr,
                                                          const int32_t *__restrict__ indices,
                                                          const double *__restrict__ data,
@@ -359,9 +402,18 @@ __global__ __launch_bounds__(kBlock) void k_csr_diag_inv(int64_t n, const int32_
                                                          PcgState *__restrict__ st) {
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
+  int32_t lo = indptr[row], hi = indptr[row + 1] - 1;
   double d = 0.0;
-  for (int32_t k = indptr[row]; k < indptr[row + 1]; ++k)
-    if (indices[k] == row) d = data[k];
+  while (lo <= hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    const int32_t c = indices[mid];
+    if (c == row) {
+      d = data[mid];
+      break;
+    }
+    if (c < row) lo = mid + 1;
+    else hi = mid - 1;
+  }
   if (d != 0.0 && isfinite(d)) {
     dinv[row] = 1.0 / d;
   } else {
@@ -385,70 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag_f32(int64_t n, const float 
   }
 }
 
-// r given; z = D^-1 r, p = z, r.z, r.r, b.b -> state; done if already converged.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T *__restrict__ r,
-                                                     const T *__restrict__ dinv,
-                                                     const T *__restrict__ b, T *__restrict__ p,
-                                                     double tol, int maxiter,
-                                                     double *__restrict__ partials,
-                                                     unsigned int *__restrict__ counter,
-                                                     PcgState *__restrict__ st) {
-  double rz = 0.0, rr = 0.0, bb = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const T ri = r[i];
-    const T z = dinv[i] * ri;
-    p[i] = z;
-    rz += (double)ri * (double)z;
-    rr += (double)ri * (double)ri;
-    bb += (double)b[i] * (double)b[i];
-  }
-  rz = block_sum(rz);
-  rr = block_sum(rr);
-  bb = block_sum(bb);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = rz;
-    partials[kMaxReduceBlocks + blockIdx.x] = rr;
-    partials[2 * kMaxReduceBlocks + blockIdx.x] = bb;
-  }
-  if (last_block(counter)) {
-    const double a = block_sum_array(partials, gridDim.x);
-    const double c = block_sum_array(partials + kMaxReduceBlocks, gridDim.x);
-    const double d = block_sum_array(partials + 2 * kMaxReduceBlocks, gridDim.x);
-    if (threadIdx.x == 0) {
-      const int bad = st->status;  // set by the diagonal kernel
-      st->rz = a;
-      st->rr = c;
-      st->bb = d;
-      st->tol2 = tol * tol * d;
-      st->alpha = 0.0;
-      st->beta = 0.0;
-      st->iter = 0;
-      st->maxiter = maxiter;
-      // b == 0 leaves x as given (as the oracle does); an initial residual
-      // already below tol needs no iteration.
-      int done = 1, status = TATVA_OK;
-      if (bad != TATVA_OK) {
-        status = bad;
-      } else if (d == 0.0 || c <= st->tol2) {
-        status = TATVA_OK;
-      } else if (maxiter <= 0) {
-        status = TATVA_ENOTCONV;
-      } else {
-        done = 0;
-      }
-      st->status = status;
-      st->done = done;
-      *counter = 0u;
-    }
-  }
-}
-
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_copy_residual_f32(int64_t n, const T *__restrict__ b,
-                                                              const T *__restrict__ Ax,
-                                                              T *__restrict__ r) {
+__global__ __launch_bounds__(kBlock) void k_sub(int64_t n, const T *__restrict__ b,
+                                                const T *__restrict__ Ax, T *__restrict__ r) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) r[i] = b[i] - Ax[i];
 }
@@ -522,28 +513,34 @@ static int spmv_grid(int64_t n, int lpr) {
 }
 
 struct WsLayout {
-  size_t off[8];
+  size_t vec[5];
+  size_t parts;
+  size_t state;
   size_t total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// vectors: nvec vectors of n elements of size esz, then partials, state, counters.
+// nvec vectors of n elements of size esz, 6 partial buffers, the state record.
 static WsLayout ws_layout(int64_t n, int nvec, size_t esz) {
   WsLayout L{};
   size_t o = 0;
   for (int i = 0; i < nvec; ++i) {
-    L.off[i] = o;
+    L.vec[i] = o;
     o = align256(o + (size_t)n * esz);
   }
-  L.off[5] = o;  // partials: 3 * kMaxReduceBlocks doubles
-  o = align256(o + 3 * kMaxReduceBlocks * sizeof(double));
-  L.off[6] = o;  // state
+  L.parts = o;
+  o = align256(o + 6 * kMaxReduceBlocks * sizeof(double));
+  L.state = o;
   o = align256(o + sizeof(PcgState));
-  L.off[7] = o;  // counters
-  o = align256(o + 8 * sizeof(unsigned int));
   L.total = o;
   return L;
+}
+
+static PcgParts parts_at(char *w, const WsLayout &L) {
+  double *b = (double *)(w + L.parts);
+  return PcgParts{b, b + kMaxReduceBlocks, b + 2 * kMaxReduceBlocks, b + 3 * kMaxReduceBlocks,
+                  b + 4 * kMaxReduceBlocks, b + 5 * kMaxReduceBlocks};
 }
 
 // Runs captured chunks until the device state reports done; fills stats.
@@ -590,9 +587,30 @@ static int run_chunks(DeviceRes *res, hipGraphExec_t exec, PcgState *st, int max
   return TATVA_OK;
 }
 
+// Captures one chunk (even length) of `step(q)` calls, q = iteration parity.
+static int capture_chunk(hipStream_t s, int chunk, const std::function<int(int, bool)> &step,
+                         bool profiling, hipGraph_t *graph, hipGraphExec_t *exec) {
+  TATVA_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = TATVA_OK;
+  for (int i = 0; i < chunk && rc == TATVA_OK; ++i) rc = step(i & 1, !(i == 0 && profiling));
+  hipError_t ce = hipStreamEndCapture(s, graph);
+  if (rc) return rc;
+  if (ce != hipSuccess) {
+    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    return TATVA_EHIP;
+  }
+  TATVA_TRY_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+  return TATVA_OK;
+}
+
 }  // namespace tatva
 
 using namespace tatva;
+
+extern "C" int tatva_set_profiling(int enable) {
+  g_profiling.store(enable ? 1 : 0);
+  return TATVA_OK;
+}
 
 extern "C" int tatva_spmv_csr(int64_t n, int64_t nnz, const int32_t *indptr,
                               const int32_t *indices, const double *data, const double *x,
@@ -603,7 +621,7 @@ extern "C" int tatva_spmv_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   // The average row length picks the lanes per row.
   return with_lpr(choose_lpr(n, nnz), [&](auto lc) -> int {
     constexpr int L = decltype(lc)::value;
-    k_spmv<L><<<spmv_grid(n, L), kBlock, 0, s>>>(n, indptr, indices, data, x, y);
+    k_spmv<L><<<spmv_grid(n, L), kBlock, 0, s>>>(n, indptr, indices, data, x, nullptr, y);
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   });
@@ -659,11 +677,6 @@ extern "C" int tatva_spmv_bcsr(int64_t n, const int32_t *indptr, const int32_t *
   return TATVA_OK;
 }
 
-extern "C" int tatva_set_profiling(int enable) {
-  g_profiling.store(enable ? 1 : 0);
-  return TATVA_OK;
-}
-
 extern "C" int64_t tatva_pcg_csr_workspace(int64_t n) { return (int64_t)ws_layout(n, 4, sizeof(double)).total; }
 
 extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
@@ -680,6 +693,7 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   TATVA_REQUIRE(tol >= 0.0, "tol must be non-negative");
   const WsLayout L = ws_layout(n, 4, sizeof(double));
   TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
+  chunk += chunk & 1;  // parity ping-pong needs an even chunk
   int device = 0;
   TATVA_TRY_HIP(hipGetDevice(&device));
   DeviceRes *res = nullptr;
@@ -687,76 +701,71 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   if (rc) return rc;
   std::lock_guard<std::mutex> lock(res->mu);
   char *w = (char *)workspace;
-  double *p = (double *)(w + L.off[0]);
-  double *r = (double *)(w + L.off[1]);
-  double *Ap = (double *)(w + L.off[2]);
-  double *dinv = (double *)(w + L.off[3]);
-  double *partials = (double *)(w + L.off[5]);
-  PcgState *st = (PcgState *)(w + L.off[6]);
-  unsigned int *counters = (unsigned int *)(w + L.off[7]);
+  double *p = (double *)(w + L.vec[0]);
+  double *r = (double *)(w + L.vec[1]);
+  double *Ap = (double *)(w + L.vec[2]);
+  double *dinv = (double *)(w + L.vec[3]);
+  const PcgParts P = parts_at(w, L);
+  PcgState *st = (PcgState *)(w + L.state);
   hipStream_t caller = (hipStream_t)stream;
   hipStream_t s = res->cap;
   TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
   TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
-  TATVA_TRY_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned int), s));
   const int lpr = choose_lpr(n, nnz);
-  const int gv = reduce_grid(n);
+  const int gv = vec_grid(n);
   const bool profiling = g_profiling.load() != 0;
   const int nblk = block_cols ? block_size : 1;
+  // operator launch geometry: node-block view when given, else CSR
+  const int g_op = nblk > 1 ? spmv_grid(n / nblk, kBLanes)
+                            : with_lpr(lpr, [&](auto lc) -> int { return spmv_grid(n, decltype(lc)::value); });
+  auto apply_op = [&](const double *xin, const double *bin, double *yout) -> int {
+    if (nblk == 3)
+      k_bspmv<3><<<g_op, kBlock, 0, s>>>(n / 3, indptr, block_cols, data, xin, bin, yout);
+    else if (nblk == 2)
+      k_bspmv<2><<<g_op, kBlock, 0, s>>>(n / 2, indptr, block_cols, data, xin, bin, yout);
+    else
+      with_lpr(lpr, [&](auto lc) -> int {
+        k_spmv<decltype(lc)::value><<<g_op, kBlock, 0, s>>>(n, indptr, indices, data, xin, bin, yout);
+        return 0;
+      });
+    TATVA_CHECK_LAUNCH();
+    return TATVA_OK;
+  };
+  auto apply_op_dot = [&]() -> int {
+    if (nblk == 3)
+      k_pcg_bspmv<3><<<g_op, kBlock, 0, s>>>(n / 3, indptr, block_cols, data, p, Ap, P.pap, st);
+    else if (nblk == 2)
+      k_pcg_bspmv<2><<<g_op, kBlock, 0, s>>>(n / 2, indptr, block_cols, data, p, Ap, P.pap, st);
+    else
+      with_lpr(lpr, [&](auto lc) -> int {
+        k_pcg_spmv<decltype(lc)::value><<<g_op, kBlock, 0, s>>>(n, indptr, indices, data, p, Ap,
+                                                                P.pap, st);
+        return 0;
+      });
+    TATVA_CHECK_LAUNCH();
+    return TATVA_OK;
+  };
+  k_state_reset<<<1, 64, 0, s>>>(st);
+  k_csr_diag_inv<<<grid_for(n), kBlock, 0, s>>>(n, indptr, indices, data, dinv, st);
+  rc = apply_op(x, b, r);  // r = b - A x
+  if (rc) return rc;
+  k_pcg_init<double><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, P.rz(1), P.rr(1), P.bb);
+  k_pcg_init_state<<<1, kBlock, 0, s>>>(P.rr(1), P.bb, gv, tol, maxiter, st);
+  TATVA_CHECK_LAUNCH();
+  const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
+    if (with_op) {
+      const int e = apply_op_dot();
+      if (e) return e;
+    }
+    k_pcg_update<double><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, P.pap, g_op, P.rz(q ^ 1),
+                                               P.rz(q), P.rr(q), st);
+    k_pcg_pupdate<double><<<gv, kBlock, 0, s>>>(n, p, r, dinv, P.rz(q ^ 1), P.rz(q), P.rr(q), st);
+    return TATVA_OK;
+  };
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  // The operator: node-block SpMV when a block view is given, else CSR.
-  auto apply_residual = [&](auto lc) {
-    constexpr int LP = decltype(lc)::value;
-    if (nblk == 3)
-      k_bspmv<3><<<spmv_grid(n / 3, kBLanes), kBlock, 0, s>>>(n / 3, indptr, block_cols, data, x, b, r);
-    else if (nblk == 2)
-      k_bspmv<2><<<spmv_grid(n / 2, kBLanes), kBlock, 0, s>>>(n / 2, indptr, block_cols, data, x, b, r);
-    else
-      k_csr_residual<LP><<<spmv_grid(n, LP), kBlock, 0, s>>>(n, indptr, indices, data, x, b, r);
-  };
-  auto apply_spmv_dot = [&](auto lc) {
-    constexpr int LP = decltype(lc)::value;
-    if (nblk == 3)
-      k_pcg_bspmv_dot<3><<<spmv_grid(n / 3, kBLanes), kBlock, 0, s>>>(
-          n / 3, indptr, block_cols, data, p, Ap, partials, counters, st);
-    else if (nblk == 2)
-      k_pcg_bspmv_dot<2><<<spmv_grid(n / 2, kBLanes), kBlock, 0, s>>>(
-          n / 2, indptr, block_cols, data, p, Ap, partials, counters, st);
-    else
-      k_pcg_spmv_dot<LP><<<spmv_grid(n, LP), kBlock, 0, s>>>(n, indptr, indices, data, p, Ap,
-                                                             partials, counters, st);
-  };
-  rc = with_lpr(lpr, [&](auto lc) -> int {
-    k_state_reset<<<1, 64, 0, s>>>(st);
-    k_csr_diag_inv<<<grid_for(n), kBlock, 0, s>>>(n, indptr, indices, data, dinv, st);
-    apply_residual(lc);
-    k_pcg_init<double><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, tol, maxiter, partials,
-                                             counters, st);
-    TATVA_CHECK_LAUNCH();
-    TATVA_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; ++i) {
-      if (!(i == 0 && profiling)) apply_spmv_dot(lc);  // sampled launch runs eagerly
-      k_pcg_update<double><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, partials,
-                                                 counters + 1, st);
-      k_pcg_pupdate<double><<<gv, kBlock, 0, s>>>(n, p, r, dinv, st);
-    }
-    hipError_t ce = hipStreamEndCapture(s, &graph);
-    if (ce != hipSuccess) {
-      set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-      return TATVA_EHIP;
-    }
-    TATVA_TRY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    return TATVA_OK;
-  });
-  const std::function<int()> sampled = [&]() -> int {
-    return with_lpr(lpr, [&](auto lc) -> int {
-      apply_spmv_dot(lc);
-      TATVA_CHECK_LAUNCH();
-      return TATVA_OK;
-    });
-  };
-  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, sampled, stats);
+  rc = capture_chunk(s, chunk, step, profiling, &graph, &exec);
+  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, apply_op_dot, stats);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
@@ -778,60 +787,52 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
   const WsLayout L = ws_layout(n, 5, sizeof(float));
   TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
+  chunk += chunk & 1;
   TATVA_TRY_HIP(hipSetDevice(m->device));
   DeviceRes *res = nullptr;
   int rc = get_res(m->device, &res);
   if (rc) return rc;
   std::lock_guard<std::mutex> lock(res->mu);
   char *w = (char *)workspace;
-  float *p = (float *)(w + L.off[0]);
-  float *r = (float *)(w + L.off[1]);
-  float *Ap = (float *)(w + L.off[2]);
-  float *dinv = (float *)(w + L.off[3]);
-  double *partials = (double *)(w + L.off[5]);
-  PcgState *st = (PcgState *)(w + L.off[6]);
-  unsigned int *counters = (unsigned int *)(w + L.off[7]);
+  float *p = (float *)(w + L.vec[0]);
+  float *r = (float *)(w + L.vec[1]);
+  float *Ap = (float *)(w + L.vec[2]);
+  float *dinv = (float *)(w + L.vec[3]);
+  const PcgParts P = parts_at(w, L);
+  PcgState *st = (PcgState *)(w + L.state);
   hipStream_t caller = (hipStream_t)stream;
   hipStream_t s = res->cap;
   TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
   TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
-  TATVA_TRY_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned int), s));
-  const int gv = reduce_grid(n);
+  const int gv = vec_grid(n);
   const bool profiling = g_profiling.load() != 0;
   k_state_reset<<<1, 64, 0, s>>>(st);
   k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
   // r = b - H x
   rc = launch_hvp_f32(m, d, u, x, dof_map, Ap, s);
   if (rc) return rc;
-  k_copy_residual_f32<float><<<grid_for(n), kBlock, 0, s>>>(n, b, Ap, r);
-  k_pcg_init<float><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, tol, maxiter, partials, counters, st);
+  k_sub<float><<<grid_for(n), kBlock, 0, s>>>(n, b, Ap, r);
+  k_pcg_init<float><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, P.rz(1), P.rr(1), P.bb);
+  k_pcg_init_state<<<1, kBlock, 0, s>>>(P.rr(1), P.bb, gv, tol, maxiter, st);
   TATVA_CHECK_LAUNCH();
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  TATVA_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < chunk && rc == TATVA_OK; ++i) {
-    if (!(i == 0 && profiling))  // sampled launch runs eagerly
-      rc = launch_hvp_f32(m, d, u, p, dof_map, Ap, s, &st->done);
-    k_pcg_dot_alpha<float><<<gv, kBlock, 0, s>>>(n, p, Ap, partials, counters, st);
-    k_pcg_update<float><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, partials, counters + 1, st);
-    k_pcg_pupdate<float><<<gv, kBlock, 0, s>>>(n, p, r, dinv, st);
-  }
-  hipError_t ce = hipStreamEndCapture(s, &graph);
-  if (rc == TATVA_OK && ce != hipSuccess) {
-    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-    rc = TATVA_EHIP;
-  }
-  if (rc == TATVA_OK) {
-    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-      rc = TATVA_EHIP;
-    }
-  }
-  const std::function<int()> sampled = [&]() -> int {
+  const std::function<int()> apply_op = [&]() -> int {
     return launch_hvp_f32(m, d, u, p, dof_map, Ap, s, &st->done);
   };
-  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, sampled, stats);
+  const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
+    if (with_op) {
+      const int e = apply_op();
+      if (e) return e;
+    }
+    k_pcg_dot<float><<<gv, kBlock, 0, s>>>(n, p, Ap, P.pap, st);
+    k_pcg_update<float><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, P.pap, gv, P.rz(q ^ 1),
+                                              P.rz(q), P.rr(q), st);
+    k_pcg_pupdate<float><<<gv, kBlock, 0, s>>>(n, p, r, dinv, P.rz(q ^ 1), P.rz(q), P.rr(q), st);
+    return TATVA_OK;
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  rc = capture_chunk(s, chunk, step, profiling, &graph, &exec);
+  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, apply_op, stats);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
@@ -874,23 +875,30 @@ __global__ void k_cast(int64_t n, double alpha, const A *__restrict__ x, B *__re
   if (i < n) y[i] = B(alpha * (double)x[i]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *__restrict__ a,
-                                                const double *__restrict__ b,
-                                                double *__restrict__ partials,
-                                                unsigned int *__restrict__ counter,
-                                                double *__restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_dot_part(int64_t n, const double *__restrict__ a,
+                                                     const double *__restrict__ b,
+                                                     double *__restrict__ partials) {
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     s += a[i] * b[i];
   s = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
-  if (last_block(counter)) {
-    const double t = block_sum_array(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      out[0] = t;
-      *counter = 0u;
-    }
+}
+
+// rows[k] = row of entry k, col_colors[k] = colors[indices[k]]
+// (compute_rows_cols, tatva/sparse/base.py:108-136).  One wavefront per row.
+__global__ __launch_bounds__(kBlock) void k_rows_cols(int64_t n, const int32_t *__restrict__ indptr,
+                                                      const int32_t *__restrict__ indices,
+                                                      const int32_t *__restrict__ colors,
+                                                      int32_t *__restrict__ rows,
+                                                      int32_t *__restrict__ col_colors) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (row >= n) return;
+  for (int32_t k = indptr[row] + lane; k < indptr[row + 1]; k += kWave) {
+    rows[k] = (int32_t)row;
+    col_colors[k] = colors[indices[k]];
   }
 }
 
@@ -959,44 +967,25 @@ extern "C" int tatva_cast_f32_f64(int64_t n, double alpha, const float *x, doubl
 
 extern "C" int tatva_dot(int64_t n, const double *x, const double *y, double *out, void *stream) {
   TATVA_REQUIRE(n > 0 && x && y && out, "invalid arguments");
-  // The partials/counter live in a small per-device buffer owned by the library.
+  // The partials live in a small per-device buffer owned by the library
+  // (allocated on the first call on a device, kept until the process ends).
   static double *partials[64] = {nullptr};
-  static unsigned int *counter[64] = {nullptr};
   static std::mutex mu;
   int device = 0;
   TATVA_TRY_HIP(hipGetDevice(&device));
   TATVA_REQUIRE(device >= 0 && device < 64, "device id out of range");
   {
     std::lock_guard<std::mutex> lock(mu);
-    if (!partials[device]) {
+    if (!partials[device])
       TATVA_TRY_HIP(hipMalloc(&partials[device], kMaxReduceBlocks * sizeof(double)));
-      TATVA_TRY_HIP(hipMalloc(&counter[device], sizeof(unsigned int)));
-      TATVA_TRY_HIP(hipMemset(counter[device], 0, sizeof(unsigned int)));
-    }
   }
-  k_dot<<<reduce_grid(n), kBlock, 0, (hipStream_t)stream>>>(n, x, y, partials[device],
-                                                           counter[device], out);
+  hipStream_t s = (hipStream_t)stream;
+  const int g = reduce_grid(n);
+  k_dot_part<<<g, kBlock, 0, s>>>(n, x, y, partials[device]);
+  k_finalize_sum<double><<<1, kBlock, 0, s>>>(partials[device], g, out, 0);
   TATVA_CHECK_LAUNCH();
   return TATVA_OK;
 }
-
-// rows[k] = row of entry k, col_colors[k] = colors[indices[k]]
-// (compute_rows_cols, tatva/sparse/base.py:108-136).  One wavefront per row.
-namespace tatva {
-__global__ __launch_bounds__(kBlock) void k_rows_cols(int64_t n, const int32_t *__restrict__ indptr,
-                                                      const int32_t *__restrict__ indices,
-                                                      const int32_t *__restrict__ colors,
-                                                      int32_t *__restrict__ rows,
-                                                      int32_t *__restrict__ col_colors) {
-  const int lane = threadIdx.x % kWave;
-  const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  if (row >= n) return;
-  for (int32_t k = indptr[row] + lane; k < indptr[row + 1]; k += kWave) {
-    rows[k] = (int32_t)row;
-    col_colors[k] = colors[indices[k]];
-  }
-}
-}  // namespace tatva
 
 extern "C" int tatva_rows_cols(int64_t n, const int32_t *indptr, const int32_t *indices,
                                const int32_t *colors, int32_t *rows, int32_t *col_colors,
