@@ -31,27 +31,12 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def traction_loads(coords, n, t=(1e-3, 0.0, 0.0)):
-    """Consistent nodal loads of a uniform traction on the x = 1 face of a
-    structured n^3 unit-cube Hex8 mesh: t h^2 {1/4 corner, 1/2 edge, 1 interior}."""
-    import numpy as np
-
-    h = 1.0 / n
-    f = np.zeros(coords.shape[0] * 3)
-    face = np.where(np.isclose(coords[:, 0], 1.0))[0]
-    j = np.rint(coords[face, 1] / h).astype(int)
-    k = np.rint(coords[face, 2] / h).astype(int)
-    w = np.where((j == 0) | (j == n), 0.5, 1.0) * np.where((k == 0) | (k == n), 0.5, 1.0)
-    for c in range(3):
-        f[face * 3 + c] = t[c] * h * h * w
-    return f
-
-
 def build_problem(n):
     import numpy as np
 
     import tatva_amd as ta
     from tatva_amd import energies, lifter, solvers, sparse
+    from tatva_amd.serve import traction_loads
 
     mesh = ta.Mesh.unit_cube(n)
     op = ta.Operator(mesh, ta.Hexahedron8())
@@ -68,6 +53,148 @@ def build_problem(n):
     solver = solvers.Newton(E, L, mode="csr", tol=1e-9, lin_tol=1e-10, chunk=64, colored_matrix=cm)
     return dict(mesh=mesh, op=op, E=E, L=L, cm=cm, solver=solver, f=f, dens=dens,
                 pattern_s=t1 - t0, coloring_s=t2 - t1)
+
+
+def stretch_problem(n, kind="hex8", mu=1.0, lmbda=10.0, jitter=0.0, seed=0):
+    """Compressible Neo-Hookean cube [0,1]^3: x = 0 clamped, u_x prescribed on x = 1
+    (the load), u_y, u_z free there."""
+    import numpy as np
+
+    import tatva_amd as ta
+    from tatva_amd import energies, lifter
+
+    if kind == "hex8":
+        mesh = ta.Mesh.unit_cube(n)
+        el = ta.Hexahedron8()
+    else:
+        mesh = ta.Mesh.unit_cube(n, type="tetrahedron", jitter=jitter, seed=seed)
+        el = ta.Tetrahedron4()
+    op = ta.Operator(mesh, el)
+    E = op.energy(energies.NeoHookean(mu, lmbda))
+    x0 = np.where(np.isclose(mesh.coords[:, 0], 0.0))[0]
+    x1 = np.where(np.isclose(mesh.coords[:, 0], 1.0))[0]
+    dofs = np.concatenate([(x0[:, None] * 3 + np.arange(3)).ravel(), x1 * 3])
+    L = lifter.Lifter(mesh.n_nodes * 3, lifter.Fixed(dofs, lifter.RuntimeValue("load")))
+
+    def values(d):
+        return np.concatenate([np.zeros(x0.size * 3), np.full(x1.size, d)])
+
+    return dict(mesh=mesh, op=op, E=E, L=L, values=values)
+
+
+def solve_load_steps(P, solver, n_steps, stretch):
+    """Load-stepped Newton: each step adds a homogeneous stretch increment
+    (u_x += dx * x) as predictor, sets u_x = stretch * s / n_steps on x = 1 and
+    solves.  Returns (u, [NewtonStats])."""
+    import torch
+
+    mesh = P["mesh"]
+    ramp = torch.zeros(mesh.n_nodes * 3, dtype=torch.float64, device="cuda")
+    ramp[0::3] = torch.as_tensor(mesh.coords[:, 0], device="cuda")
+    u = torch.zeros(mesh.n_nodes * 3, dtype=torch.float64, device="cuda")
+    out = []
+    for s in range(1, n_steps + 1):
+        u.add_(ramp, alpha=stretch / n_steps)
+        L = P["L"].at["load"].set(P["values"](stretch * s / n_steps))
+        out.append(solver.solve(u, L))
+    return u, out
+
+
+def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=False):
+    """The same problem solved by FP64 colored-CSR Newton and by mixed Newton
+    (FP32 matrix-free PCG inside FP64 Newton/iterative refinement), both to
+    ||r|| <= tol ||r_0||.  ``run(solver)`` performs the solve and returns
+    (u, [NewtonStats]); it is called once untimed first."""
+    import torch
+
+    from tatva_amd import solvers
+
+    res = {}
+    sols = {}
+    for mode in ("csr", "mixed"):
+        kw = dict(lin_tol=1e-10, colored_matrix=colored_matrix) if mode == "csr" \
+            else dict(inner_tol=1e-3)
+        solver = solvers.Newton(E, L0, mode=mode, tol=tol, chunk=64, **kw)
+        run(solver)     # warm-up (workspaces)
+        solvers.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u, stats = run(solver)
+        torch.cuda.synchronize()
+        solvers.set_profiling(False)
+        dt = time.perf_counter() - t0
+        n_free = solver.red.size
+        cg = sum(s.cg_iterations for s in stats)
+        op_ms = sum(s.op_ms for s in stats)
+        op_n = sum(s.op_samples for s in stats)
+        res[mode] = {
+            "seconds": dt, "newton_iterations": sum(s.newton_iterations for s in stats),
+            "cg_iterations": cg, "dof_iters_per_s": n_free * cg / dt,
+            "converged": all(s.converged for s in stats),
+            "worst_rel_residual": max(s.residual_history[-1] / s.residual_history[0]
+                                      for s in stats),
+            "operator_us": (op_ms / op_n * 1e3) if op_n else None,
+        }
+        sols[mode] = u
+    a, b = sols["mixed"], sols["csr"]
+    res["mixed_vs_fp64_rel_error"] = float((a - b).abs().max() / b.abs().max())
+    if keep_solutions:
+        res["_u_mixed"], res["_u_csr"] = a, b
+    res["dof_iters_speedup"] = res["mixed"]["dof_iters_per_s"] / res["csr"]["dof_iters_per_s"]
+    res["time_to_solution_speedup"] = res["csr"]["seconds"] / res["mixed"]["seconds"]
+    return res
+
+
+def c1_latency(reps=20):
+    """C1: Tri3 Poisson on a 64x64 unit square, u = 0 on the boundary, FP64
+    colored CSR + PCG to 1e-10; median wall time per Newton solve."""
+    import numpy as np
+    import torch
+
+    import tatva_amd as ta
+    from tatva_amd import energies, lifter, solvers
+
+    mesh = ta.Mesh.unit_square(64, 64)
+    op = ta.Operator(mesh, ta.Tri3())
+    E = op.energy(energies.Poisson(1.0))
+    bnd = np.where(np.isclose(mesh.coords, 0).any(1) | np.isclose(mesh.coords, 1).any(1))[0]
+    L = lifter.Lifter(mesh.n_nodes, lifter.Fixed(bnd, 0.0))
+    solver = solvers.Newton(E, L, mode="csr", tol=1e-9, lin_tol=1e-10, chunk=32)
+    times, its = [], 0
+    for r in range(reps + 2):
+        u = torch.zeros(mesh.n_nodes, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(u)
+        torch.cuda.synchronize()
+        if r >= 2:
+            times.append(time.perf_counter() - t0)
+            its = st.cg_iterations
+    return {"median_ms": float(np.median(times)) * 1e3, "p99_ms": float(np.percentile(times, 99)) * 1e3,
+            "cg_iterations": its, "n_free": solver.red.size}
+
+
+def serve_stream(count, qps, seed=0, max_slots=64, chunk=8):
+    """Continuous-batching server on the mixed Hex8 LE / Neo-Hookean request
+    stream (n log-uniform in [8, 48]); one warm-up request first."""
+    from tatva_amd import serve
+
+    srv = serve.Server(max_slots=max_slots, chunk=chunk)
+    srv.run(serve.make_stream(2, qps=0.0, seed=seed + 1, n_range=(8, 12)), realtime=False)
+    reqs = serve.make_stream(count, qps=qps, seed=seed)
+    rep = srv.run(reqs, realtime=True)
+    srv.close()
+    s = rep.summary()
+    s.update({
+        "qps": qps, "max_slots": max_slots, "chunk": chunk,
+        "workload": f"{count} requests, Poisson arrivals at {qps}/s, hex8 cubes n in [8, 48] "
+                    "log-uniform, E ~ LogNormal(0, 0.5), nu ~ U(0.2, 0.45), linear elastic "
+                    "(traction 1e-3) / Neo-Hookean (2 load steps of u_x += 0.03) 50/50, "
+                    "Newton to 1e-8 relative, FP32 batched PCG to 1e-3 per Newton step",
+        "total_dofs": int(sum(r.n_dofs for r in reqs)),
+        "all_converged": all(r.converged for r in rep.results),
+    })
+    return s
 
 
 def cpu_baseline(K_host, b_host, seconds=12.0, max_iters=400):
@@ -98,6 +225,60 @@ def cpu_baseline(K_host, b_host, seconds=12.0, max_iters=400):
     return A.shape[0] * it / dt, it, dt
 
 
+def chunked_oracle_residual(elem, coords, elements, dens, u, chunk=262144):
+    """The oracle's FP64 residual of a large mesh, accumulated over element
+    chunks (residuals are sums of per-element contributions)."""
+    import numpy as np
+
+    from oracle import fem  # the checker only
+
+    r = np.zeros(u.size)
+    for a in range(0, elements.shape[0], chunk):
+        r += fem.Problem(elem, coords, elements[a:a + chunk], dens).residual(u)
+    return r
+
+
+def large_mixed(workload, oracle_check=True):
+    """C3 (Tet4 Neo-Hookean 96^3 x 6, jittered, 10 load steps to stretch 1.3) or
+    C5 (Hex8 Neo-Hookean 160^3, 5 load steps to stretch 1.3): mixed Newton vs
+    the FP64 colored-CSR Newton on the same problem, both to 1e-8 relative."""
+    import numpy as np
+
+    if workload == "c3":
+        n, kind, steps, jitter = 96, "tet4", 10, 0.2
+    else:
+        n, kind, steps, jitter = 160, "hex8", 5, 0.0
+    stretch = 0.3
+    t0 = time.perf_counter()
+    NH = stretch_problem(n, kind=kind, jitter=jitter, seed=0)
+    L0 = NH["L"].at["load"].set(NH["values"](0.0))
+    setup = time.perf_counter() - t0
+    res = mixed_vs_fp64(NH["E"], L0, lambda sv: solve_load_steps(NH, sv, steps, stretch),
+                        keep_solutions=True)
+    mesh = NH["mesh"]
+    res["workload"] = (f"{kind} Neo-Hookean {n}^3{' x 6 tets' if kind == 'tet4' else ''}, "
+                       f"mu=1, lambda=10, jitter={jitter}h (seed 0), x=0 clamped, u_x on x=1 "
+                       f"in {steps} load steps to {stretch}, Newton to 1e-8 relative per step")
+    res["elements"] = mesh.n_elements
+    res["nodes"] = mesh.n_nodes
+    res["setup_s"] = setup
+    if oracle_check:
+        from oracle import fem  # the checker only
+
+        el = fem.Tet4() if kind == "tet4" else fem.Hex8()
+        L_end = NH["L"].at["load"].set(NH["values"](stretch))
+        free = L_end.free_dofs
+        for mode in ("mixed", "csr"):
+            uh = res.pop(f"_u_{mode}").cpu().numpy()
+            r = chunked_oracle_residual(el, mesh.coords, mesh.elements,
+                                        fem.NeoHookean(1.0, 10.0), uh)[free]
+            res[mode]["oracle_final_residual_norm"] = float(np.linalg.norm(r))
+    else:
+        res.pop("_u_mixed", None)
+        res.pop("_u_csr", None)
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +287,15 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=64, help="cells per side of the Hex8 cube")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed-vs-FP64 and C1 latency sections")
+    ap.add_argument("--nh-n", type=int, default=48, help="cells per side, Neo-Hookean comparison")
+    ap.add_argument("--nh-steps", type=int, default=2, help="load steps, Neo-Hookean comparison")
+    ap.add_argument("--nh-stretch", type=float, default=0.1, help="final u_x on x = 1")
+    ap.add_argument("--workload", choices=("c2", "c3", "c5"), default="c2",
+                    help="c2: the default line; c3/c5: large mixed-vs-FP64 Neo-Hookean runs")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-qps", type=float, default=10.0)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path's "
@@ -118,6 +308,10 @@ def main(argv=None):
     from tatva_amd import solvers
 
     torch.cuda.set_device(0)
+    if args.workload != "c2":
+        r = large_mixed(args.workload, oracle_check=not args.no_cpu_baseline)
+        print(json.dumps({"workload": args.workload, **r}), flush=True)
+        return 0
     t_setup = time.perf_counter()
     P = build_problem(args.n)
     torch.cuda.synchronize()
@@ -164,6 +358,27 @@ def main(argv=None):
         spmv_bytes = 12 * nnz + 4 * (n_free + 1) + 16 * n_free
     achieved = spmv_bytes / spmv_s / 1e9 if op_samples else None
     converged = all(s.converged for s in stats)
+
+    extras = {}
+    if not args.no_extras:
+        # mixed vs FP64 on the same C2 problem (linear: Newton = iterative refinement)
+        def run_c2(sv):
+            uu = torch.zeros(ndof, dtype=torch.float64, device="cuda")
+            return uu, [sv.solve(uu)]
+
+        extras["mixed_vs_fp64_c2"] = mixed_vs_fp64(P["E"], L, run_c2, colored_matrix=cm)
+        # mixed vs FP64 on a load-stepped Neo-Hookean Hex8 cube (C5 form, smaller)
+        NH = stretch_problem(args.nh_n)
+        L0 = NH["L"].at["load"].set(NH["values"](0.0))
+        extras["mixed_vs_fp64_neo_hookean"] = mixed_vs_fp64(
+            NH["E"], L0, lambda sv: solve_load_steps(NH, sv, args.nh_steps, args.nh_stretch))
+        extras["mixed_vs_fp64_neo_hookean"]["workload"] = (
+            f"hex8 Neo-Hookean {args.nh_n}^3, mu=1, lambda=10, {args.nh_steps} load steps to "
+            f"u_x={args.nh_stretch} on x=1, Newton to 1e-8 relative per step")
+        del NH
+        extras["c1_poisson_tri3_64"] = c1_latency()
+        if args.serve_requests > 0:
+            extras["serve_c4"] = serve_stream(args.serve_requests, args.serve_qps)
 
     # checks on the final state: true linear residual and the FP64 oracle's residual
     res_free = P["E"].reduced(L).residual_full_state(u)
@@ -236,6 +451,7 @@ def main(argv=None):
         "setup_s": setup_s,
         "pattern_s": P["pattern_s"],
         "coloring_s": P["coloring_s"],
+        **extras,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -203,6 +203,35 @@ int tatva_pcg_matfree_f32(tatva_mesh *mesh, const tatva_density *density, const 
                           float *x, double tol, int32_t maxiter, int32_t chunk, void *workspace,
                           int64_t workspace_bytes, tatva_pcg_stats *stats, void *stream);
 
+/* ------------------------------------------------------- batched PCG ------ */
+
+/* A batch runs many matrix-free FP32 Jacobi-PCG solves (each with its own
+ * mesh, density, constraints, state, tolerance) in shared launches: one tile
+ * assembly and one gather launch per (element kind, density kind, quadrature)
+ * class and two vector launches per iteration for all slots.  No counterpart
+ * in the reference (it is the serving loop's solver; the linear algebra is the
+ * same as tatva_pcg_matfree_f32).  A slot's arithmetic does not depend on the
+ * other slots, so its result is identical alone and in any mix. */
+typedef struct tatva_batch tatva_batch;
+int tatva_batch_create(int device, int32_t capacity, tatva_batch **out);
+int tatva_batch_destroy(tatva_batch *batch);
+/* Bytes of the per-slot workspace for a solve of n reduced dofs on mesh. */
+int64_t tatva_batch_slot_workspace(tatva_mesh *mesh, int32_t ncomp, int64_t n);
+/* (Re)starts slot: H(u) x = b from x (in place; updated by the iterations).
+ * u (full FP32 state), dof_map, diag, b, x and the workspace must stay valid
+ * until the slot is released or restarted.  Set-up kernels run on stream. */
+int tatva_batch_start(tatva_batch *batch, int32_t slot, tatva_mesh *mesh,
+                      const tatva_density *density, const float *u, const int32_t *dof_map,
+                      int64_t n, const float *diag, const float *b, float *x, double tol,
+                      int32_t maxiter, void *workspace, int64_t workspace_bytes, void *stream);
+int tatva_batch_release(tatva_batch *batch, int32_t slot);
+/* iters PCG iterations of every active slot (converged slots idle). */
+int tatva_batch_iterate(tatva_batch *batch, int32_t iters, void *stream);
+/* Synchronises stream and writes capacity stats records (reserved = done flag;
+ * inactive slots are zero) and the number of active slots still iterating. */
+int tatva_batch_poll(tatva_batch *batch, tatva_pcg_stats *stats, int32_t *n_running,
+                     void *stream);
+
 /* ------------------------------------------------------- vector kernels ---- */
 
 /* out_full[free[i]] = in_red[i] for i < n_red (Lifter.lift, lifter/base.py:201-221;

@@ -76,6 +76,15 @@ _SIGS = {
     "tatva_pcg_matfree_f32": (c_int, [c_void, ctypes.POINTER(Density), c_void, c_void, c_i64, c_void,
                                       c_void, c_void, c_dbl, c_i32, c_i32, c_void, c_i64,
                                       ctypes.POINTER(PcgStats), c_void]),
+    "tatva_batch_create": (c_int, [c_int, c_i32, ctypes.POINTER(c_void)]),
+    "tatva_batch_destroy": (c_int, [c_void]),
+    "tatva_batch_slot_workspace": (c_i64, [c_void, c_i32, c_i64]),
+    "tatva_batch_start": (c_int, [c_void, c_i32, c_void, ctypes.POINTER(Density), c_void, c_void,
+                                  c_i64, c_void, c_void, c_void, c_dbl, c_i32, c_void, c_i64,
+                                  c_void]),
+    "tatva_batch_release": (c_int, [c_void, c_i32]),
+    "tatva_batch_iterate": (c_int, [c_void, c_i32, c_void]),
+    "tatva_batch_poll": (c_int, [c_void, c_void, ctypes.POINTER(c_i32), c_void]),
     "tatva_scatter_free": (c_int, [c_i64, c_void, c_void, c_void, c_void]),
     "tatva_set_dofs": (c_int, [c_i64, c_void, c_void, c_dbl, c_void, c_void]),
     "tatva_gather_free": (c_int, [c_i64, c_void, c_void, c_void, c_void]),

@@ -1,0 +1,554 @@
+// Batched matrix-free FP32 Jacobi-PCG: one launch per kernel step covers every
+// active solve of a batch (the serving loop's requests, each with its own mesh,
+// material, constraints and tolerance), so requests of very different sizes
+// share the launches instead of queueing behind each other.
+//
+// Per iteration (tick t), with one class = slots of equal (element kind,
+// density kind, quadrature rule):
+//   per class  A  tile assembly of H(u_s) p_s into the slot's tile partials
+//   per class  G  slot gather -> Ap_s (reduced) and the block shares of p.Ap
+//   all slots  U  alpha, x += alpha p, r -= alpha Ap, shares of r.z and r.r
+//   all slots  P  convergence test, beta, p = D^-1 r + beta p
+// A block finds its slot through a per-block table (one load) and works on
+// that slot's arrays only; the slot's own blocks re-sum its partials across
+// the kernel boundary exactly as the single-solve PCG does.  Every slot runs
+// the same block decomposition whatever else is in the batch, so its
+// arithmetic, and its result, do not depend on the mix.  The parity of the
+// partial buffers is (t - start tick of the slot) mod 2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "assembly.hpp"
+#include "common.hpp"
+#include "mesh.hpp"
+#include "pcg.hpp"
+
+namespace tatva {
+
+// Gather blocks of one slot: grid-stride over nodes beyond kMaxReduceBlocks.
+inline int gather_grid(int64_t n_nodes) {
+  int64_t b = (n_nodes + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
+  return (int)b;
+}
+
+struct BatchSlot {
+  const double *coords;
+  const int32_t *conn;
+  TileSet tiles;
+  int64_t n_nodes;
+  int64_t n;
+  tatva_density dens;
+  const int32_t *dof_map;  // full -> reduced (or NULL: identity)
+  const float *u;          // FP32 full state (linearisation point)
+  float *partial;          // tile partials, n_slots * ncomp
+  float *x, *r, *p, *Ap, *dinv;
+  double *pap, *rz0, *rz1, *rr0, *rr1, *bb;
+  PcgState *st;
+  int32_t g_blocks, v_blocks, start_tick;
+  int32_t a_blk0, g_blk0, v_blk0;
+};
+
+template <class Dens>
+__device__ __forceinline__ Dens make_density(const tatva_density &d) {
+  if constexpr (Dens::kSource) return Dens{float(d.p0)};
+  else return Dens{float(d.p0), float(d.p1)};
+}
+
+template <int KIND, class Dens>
+__global__ __launch_bounds__(kTile) void k_batch_assemble(const BatchSlot *__restrict__ slots,
+                                                          const int32_t *__restrict__ bmap,
+                                                          QuadRule quad) {
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  const BatchSlot &S = slots[bmap[b]];
+  if (S.st->done) return;
+  extern __shared__ unsigned char lds_raw[];
+  const Dens dens = make_density<Dens>(S.dens);
+  tile_assemble<KIND, Dens, float, kModeHvp>(b - S.a_blk0, S.coords, S.conn, quad, dens, S.u, S.p,
+                                             S.dof_map, S.tiles, S.partial,
+                                             reinterpret_cast<float *>(lds_raw));
+}
+
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_batch_gather(const BatchSlot *__restrict__ slots,
+                                                         const int32_t *__restrict__ bmap) {
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  const BatchSlot &S = slots[bmap[b]];
+  if (S.st->done) return;
+  const int lb = b - S.g_blk0;
+  const int32_t *__restrict__ dof_map = S.dof_map;
+  const float *__restrict__ p = S.p;
+  float *__restrict__ Ap = S.Ap;
+  double dot = 0.0;
+  for (int64_t i = (int64_t)lb * kBlock + threadIdx.x; i < S.n_nodes;
+       i += (int64_t)S.g_blocks * kBlock) {
+    float acc[NC];
+    slot_sum<NC, float>(i, S.tiles.slot_ptr, S.tiles.slot_list, S.partial, acc);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int64_t dof = i * NC + c;
+      const int64_t r = dof_map ? (int64_t)dof_map[dof] : dof;
+      if (r >= 0) {
+        Ap[r] = acc[c];
+        dot += (double)acc[c] * (double)p[r];
+      }
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0) S.pap[lb] = dot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_update(const BatchSlot *__restrict__ slots,
+                                                         const int32_t *__restrict__ bmap,
+                                                         int tick) {
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  const BatchSlot &S = slots[bmap[b]];
+  PcgState *st = S.st;
+  if (st->done) return;
+  const int lb = b - S.v_blk0;
+  const int q = (tick - S.start_tick) & 1;
+  const int iter = st->iter;
+  const double pap = sum_partials(S.pap, S.g_blocks);
+  const double rz = sum_partials(q ? S.rz0 : S.rz1, S.v_blocks);
+  if (!(pap > 0.0) || !isfinite(pap) || iter >= st->maxiter) {
+    if (lb == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->status = (iter >= st->maxiter) ? TATVA_ENOTCONV : TATVA_EBREAKDOWN;
+    }
+    return;
+  }
+  const float alpha = float(rz / pap);
+  float *__restrict__ x = S.x;
+  float *__restrict__ r = S.r;
+  const float *__restrict__ p = S.p;
+  const float *__restrict__ Ap = S.Ap;
+  const float *__restrict__ dinv = S.dinv;
+  double srz = 0.0, srr = 0.0;
+  for (int64_t i = (int64_t)lb * kBlock + threadIdx.x; i < S.n; i += (int64_t)S.v_blocks * kBlock) {
+    x[i] += alpha * p[i];
+    const float ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    srz += (double)ri * (double)(dinv[i] * ri);
+    srr += (double)ri * (double)ri;
+  }
+  srz = block_sum(srz);
+  srr = block_sum(srr);
+  if (threadIdx.x == 0) {
+    (q ? S.rz1 : S.rz0)[lb] = srz;
+    (q ? S.rr1 : S.rr0)[lb] = srr;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_pupdate(const BatchSlot *__restrict__ slots,
+                                                          const int32_t *__restrict__ bmap,
+                                                          int tick) {
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  const BatchSlot &S = slots[bmap[b]];
+  PcgState *st = S.st;
+  if (st->done) return;
+  const int lb = b - S.v_blk0;
+  const int q = (tick - S.start_tick) & 1;
+  const double rz_old = sum_partials(q ? S.rz0 : S.rz1, S.v_blocks);
+  const double rz_new = sum_partials(q ? S.rz1 : S.rz0, S.v_blocks);
+  const double rr = sum_partials(q ? S.rr1 : S.rr0, S.v_blocks);
+  const bool converged = rr <= st->tol2;
+  const bool bad = !isfinite(rr) || !isfinite(rz_new);
+  if (lb == 0 && threadIdx.x == 0) {
+    st->iter += 1;
+    st->rr = rr;
+    if (converged) {
+      st->done = 1;
+      st->status = TATVA_OK;
+    } else if (bad) {
+      st->done = 1;
+      st->status = TATVA_EBREAKDOWN;
+    }
+  }
+  if (converged || bad) return;
+  const float beta = float(rz_new / rz_old);
+  float *__restrict__ p = S.p;
+  const float *__restrict__ r = S.r;
+  const float *__restrict__ dinv = S.dinv;
+  for (int64_t i = (int64_t)lb * kBlock + threadIdx.x; i < S.n; i += (int64_t)S.v_blocks * kBlock)
+    p[i] = dinv[i] * r[i] + beta * p[i];
+}
+
+// Per-slot workspace: p, r, Ap, dinv (n floats each), tile partials
+// (n_slots * ncomp floats), the partial sums pap (gather blocks) and
+// rz0/rz1/rr0/rr1/bb (vector blocks).
+struct SlotLayout {
+  size_t p, r, Ap, dinv, partial, pap, rz0, rz1, rr0, rr1, bb, total;
+};
+
+static size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static SlotLayout slot_layout(const tatva_mesh *m, int ncomp, int64_t n) {
+  SlotLayout L{};
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = al256(o + bytes);
+    return at;
+  };
+  L.p = take(n * sizeof(float));
+  L.r = take(n * sizeof(float));
+  L.Ap = take(n * sizeof(float));
+  L.dinv = take(n * sizeof(float));
+  L.partial = take((size_t)std::max<int64_t>(1, m->n_slots) * ncomp * sizeof(float));
+  L.pap = take(gather_grid(m->n_nodes) * sizeof(double));
+  const size_t vb = vec_grid(n) * sizeof(double);
+  L.rz0 = take(vb);
+  L.rz1 = take(vb);
+  L.rr0 = take(vb);
+  L.rr1 = take(vb);
+  L.bb = take(vb);
+  L.total = o;
+  return L;
+}
+
+}  // namespace tatva
+
+using namespace tatva;
+
+struct tatva_batch {
+  struct Meta {
+    bool active = false;
+    int kind = 0, dkind = 0, ncomp = 0;
+    QuadRule quad{};
+    size_t lds = 0;
+    tatva_mesh *mesh = nullptr;
+  };
+  struct Launch {
+    int kind, dkind, ncomp;
+    QuadRule quad;
+    size_t lds;
+    int a_off, a_n, g_off, g_n;
+  };
+  int device = 0;
+  int capacity = 0;
+  std::mutex mu;
+  std::vector<BatchSlot> slots;  // host copy of the device table
+  std::vector<Meta> meta;
+  std::vector<Launch> launches;
+  int v_off = 0, v_n = 0;
+  bool dirty = true;
+  int64_t tick = 0;
+  BatchSlot *d_slots = nullptr;
+  PcgState *d_states = nullptr;
+  PcgState *h_states = nullptr;  // pinned
+  int32_t *d_bmap = nullptr;
+  size_t bmap_cap = 0;
+  BatchSlot *h_stage_slots = nullptr;  // pinned staging of the uploads
+  int32_t *h_stage_bmap = nullptr;
+  size_t stage_bmap_cap = 0;
+  hipEvent_t uploaded = nullptr;
+  bool upload_pending = false;
+};
+
+namespace {
+
+int batch_fail(hipError_t e, const char *what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return TATVA_EHIP;
+}
+
+bool same_quad(const QuadRule &a, const QuadRule &b) {
+  return std::memcmp(&a, &b, sizeof(QuadRule)) == 0;
+}
+
+// Re-plans the launches over the active slots and uploads the slot table and
+// the block map (pinned staging; the previous upload must have completed
+// before the staging is rewritten).
+int batch_upload(tatva_batch *B, hipStream_t s) {
+  if (!B->dirty) return TATVA_OK;
+  if (B->upload_pending) {
+    TATVA_TRY_HIP(hipEventSynchronize(B->uploaded));
+    B->upload_pending = false;
+  }
+  B->launches.clear();
+  std::vector<int32_t> bmap;
+  // classes in slot order of their first member
+  for (int i = 0; i < B->capacity; ++i) {
+    const auto &mi = B->meta[i];
+    if (!mi.active) continue;
+    bool found = false;
+    for (auto &L : B->launches)
+      if (L.kind == mi.kind && L.dkind == mi.dkind && same_quad(L.quad, mi.quad)) found = true;
+    if (found) continue;
+    tatva_batch::Launch L{mi.kind, mi.dkind, mi.ncomp, mi.quad, 0, 0, 0, 0, 0};
+    B->launches.push_back(L);
+  }
+  for (auto &L : B->launches) {
+    L.a_off = (int)bmap.size();
+    for (int i = 0; i < B->capacity; ++i) {
+      const auto &mi = B->meta[i];
+      if (!mi.active || mi.kind != L.kind || mi.dkind != L.dkind || !same_quad(mi.quad, L.quad))
+        continue;
+      B->slots[i].a_blk0 = (int32_t)bmap.size() - L.a_off;
+      bmap.insert(bmap.end(), B->slots[i].tiles.n_tiles, i);
+      L.lds = std::max(L.lds, mi.lds);
+    }
+    L.a_n = (int)bmap.size() - L.a_off;
+    L.g_off = (int)bmap.size();
+    for (int i = 0; i < B->capacity; ++i) {
+      const auto &mi = B->meta[i];
+      if (!mi.active || mi.kind != L.kind || mi.dkind != L.dkind || !same_quad(mi.quad, L.quad))
+        continue;
+      B->slots[i].g_blk0 = (int32_t)bmap.size() - L.g_off;
+      bmap.insert(bmap.end(), B->slots[i].g_blocks, i);
+    }
+    L.g_n = (int)bmap.size() - L.g_off;
+  }
+  B->v_off = (int)bmap.size();
+  for (int i = 0; i < B->capacity; ++i) {
+    if (!B->meta[i].active) continue;
+    B->slots[i].v_blk0 = (int32_t)bmap.size() - B->v_off;
+    bmap.insert(bmap.end(), B->slots[i].v_blocks, i);
+  }
+  B->v_n = (int)bmap.size() - B->v_off;
+  if (bmap.size() >= (size_t)1 << 31) {
+    set_error("batch block map too large");
+    return TATVA_EINVAL;
+  }
+  if (bmap.size() > B->bmap_cap) {
+    if (B->d_bmap) (void)hipFree(B->d_bmap);
+    if (B->h_stage_bmap) (void)hipHostFree(B->h_stage_bmap);
+    B->d_bmap = nullptr;
+    B->h_stage_bmap = nullptr;
+    B->bmap_cap = 0;
+    const size_t cap = std::max<size_t>(bmap.size() * 2, 4096);
+    hipError_t e = hipMalloc(&B->d_bmap, cap * sizeof(int32_t));
+    if (e != hipSuccess) return batch_fail(e, "hipMalloc block map");
+    e = hipHostMalloc(&B->h_stage_bmap, cap * sizeof(int32_t), hipHostMallocDefault);
+    if (e != hipSuccess) return batch_fail(e, "hipHostMalloc block map");
+    B->bmap_cap = cap;
+  }
+  std::memcpy(B->h_stage_slots, B->slots.data(), B->capacity * sizeof(BatchSlot));
+  if (!bmap.empty()) std::memcpy(B->h_stage_bmap, bmap.data(), bmap.size() * sizeof(int32_t));
+  TATVA_TRY_HIP(hipMemcpyAsync(B->d_slots, B->h_stage_slots, B->capacity * sizeof(BatchSlot),
+                               hipMemcpyHostToDevice, s));
+  if (!bmap.empty())
+    TATVA_TRY_HIP(hipMemcpyAsync(B->d_bmap, B->h_stage_bmap, bmap.size() * sizeof(int32_t),
+                                 hipMemcpyHostToDevice, s));
+  TATVA_TRY_HIP(hipEventRecord(B->uploaded, s));
+  B->upload_pending = true;
+  B->dirty = false;
+  return TATVA_OK;
+}
+
+template <class F>
+int with_class(int kind, int dkind, F &&f) {
+  auto dens = [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    constexpr int D = Elem<K>::D;
+    switch (dkind) {
+      case TATVA_DENSITY_POISSON: return f(kc, Poisson<D, float>{0.0f});
+      case TATVA_DENSITY_LINEAR_ELASTIC: return f(kc, LinearElastic<D, float>{0.0f, 0.0f});
+      case TATVA_DENSITY_NEO_HOOKEAN: return f(kc, NeoHookean<D, float>{0.0f, 0.0f});
+      default: set_error("unknown density kind"); return TATVA_EINVAL;
+    }
+  };
+  switch (kind) {
+    case TATVA_TRI3: return dens(std::integral_constant<int, TATVA_TRI3>{});
+    case TATVA_TET4: return dens(std::integral_constant<int, TATVA_TET4>{});
+    case TATVA_HEX8: return dens(std::integral_constant<int, TATVA_HEX8>{});
+    default: set_error("unknown element kind"); return TATVA_EINVAL;
+  }
+}
+
+}  // namespace
+
+extern "C" int tatva_batch_create(int device, int32_t capacity, tatva_batch **out) {
+  TATVA_REQUIRE(out != nullptr, "out is NULL");
+  *out = nullptr;
+  TATVA_REQUIRE(capacity >= 1 && capacity <= 4096, "capacity must be in [1, 4096]");
+  TATVA_TRY_HIP(hipSetDevice(device));
+  tatva_batch *B = new tatva_batch();
+  B->device = device;
+  B->capacity = capacity;
+  B->slots.assign(capacity, BatchSlot{});
+  B->meta.assign(capacity, tatva_batch::Meta{});
+  auto fail = [&](hipError_t e, const char *what) {
+    const int rc = batch_fail(e, what);
+    tatva_batch_destroy(B);
+    return rc;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&B->d_slots, capacity * sizeof(BatchSlot))) != hipSuccess)
+    return fail(e, "hipMalloc slots");
+  if ((e = hipMalloc(&B->d_states, capacity * sizeof(PcgState))) != hipSuccess)
+    return fail(e, "hipMalloc states");
+  if ((e = hipMemset(B->d_states, 0, capacity * sizeof(PcgState))) != hipSuccess)
+    return fail(e, "hipMemset states");
+  if ((e = hipHostMalloc(&B->h_states, capacity * sizeof(PcgState), hipHostMallocDefault)) != hipSuccess)
+    return fail(e, "hipHostMalloc states");
+  if ((e = hipHostMalloc(&B->h_stage_slots, capacity * sizeof(BatchSlot), hipHostMallocDefault)) != hipSuccess)
+    return fail(e, "hipHostMalloc slots");
+  if ((e = hipEventCreateWithFlags(&B->uploaded, hipEventDisableTiming)) != hipSuccess)
+    return fail(e, "hipEventCreate");
+  std::memset(B->h_states, 0, capacity * sizeof(PcgState));
+  *out = B;
+  return TATVA_OK;
+}
+
+extern "C" int tatva_batch_destroy(tatva_batch *B) {
+  if (!B) return TATVA_OK;
+  (void)hipSetDevice(B->device);
+  (void)hipDeviceSynchronize();
+  if (B->d_slots) (void)hipFree(B->d_slots);
+  if (B->d_states) (void)hipFree(B->d_states);
+  if (B->d_bmap) (void)hipFree(B->d_bmap);
+  if (B->h_states) (void)hipHostFree(B->h_states);
+  if (B->h_stage_slots) (void)hipHostFree(B->h_stage_slots);
+  if (B->h_stage_bmap) (void)hipHostFree(B->h_stage_bmap);
+  if (B->uploaded) (void)hipEventDestroy(B->uploaded);
+  delete B;
+  return TATVA_OK;
+}
+
+extern "C" int64_t tatva_batch_slot_workspace(tatva_mesh *m, int32_t ncomp, int64_t n) {
+  if (!m || n <= 0 || ncomp < 1 || ncomp > 3) return -1;
+  return (int64_t)slot_layout(m, ncomp, n).total;
+}
+
+extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
+                                 const tatva_density *d, const float *u, const int32_t *dof_map,
+                                 int64_t n, const float *diag, const float *b, float *x,
+                                 double tol, int32_t maxiter, void *workspace,
+                                 int64_t workspace_bytes, void *stream) {
+  TATVA_REQUIRE(B && m && d && u && diag && b && x && n > 0, "invalid arguments");
+  TATVA_REQUIRE(slot >= 0 && slot < B->capacity, "slot out of range");
+  TATVA_REQUIRE(m->device == B->device, "mesh and batch live on different devices");
+  TATVA_REQUIRE(d->kind >= TATVA_DENSITY_POISSON && d->kind <= TATVA_DENSITY_NEO_HOOKEAN,
+                "unknown density kind");
+  const int ncomp = density_ncomp(m, d);
+  const int64_t n_full = m->n_nodes * ncomp;
+  TATVA_REQUIRE(n <= n_full, "reduced size exceeds the mesh dofs");
+  TATVA_REQUIRE(dof_map || n == n_full, "without dof_map n must equal the number of dofs");
+  const SlotLayout L = slot_layout(m, ncomp, n);
+  TATVA_REQUIRE(workspace && workspace_bytes >= (int64_t)L.total, "batch slot workspace too small");
+  std::lock_guard<std::mutex> lock(B->mu);
+  TATVA_TRY_HIP(hipSetDevice(B->device));
+  hipStream_t s = (hipStream_t)stream;
+  char *w = (char *)workspace;
+  BatchSlot S{};
+  S.coords = m->coords;
+  S.conn = m->conn;
+  S.tiles = m->tiles;
+  S.n_nodes = m->n_nodes;
+  S.n = n;
+  S.dens = *d;
+  S.dof_map = dof_map;
+  S.u = u;
+  S.partial = (float *)(w + L.partial);
+  S.x = x;
+  S.r = (float *)(w + L.r);
+  S.p = (float *)(w + L.p);
+  S.Ap = (float *)(w + L.Ap);
+  S.dinv = (float *)(w + L.dinv);
+  S.pap = (double *)(w + L.pap);
+  S.rz0 = (double *)(w + L.rz0);
+  S.rz1 = (double *)(w + L.rz1);
+  S.rr0 = (double *)(w + L.rr0);
+  S.rr1 = (double *)(w + L.rr1);
+  S.bb = (double *)(w + L.bb);
+  S.st = B->d_states + slot;
+  S.g_blocks = gather_grid(m->n_nodes);
+  S.v_blocks = vec_grid(n);
+  if (B->tick >= (int64_t)1 << 30) B->tick &= 1;  // keeps tick and start_tick in int range
+  S.start_tick = (int32_t)B->tick;
+  const int rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, S.r, S.p, S.Ap, S.dinv, S.rz1,
+                                   S.rr1, S.bb, S.v_blocks, tol, maxiter, S.st, s);
+  if (rc) return rc;
+  B->slots[slot] = S;
+  auto &mi = B->meta[slot];
+  mi.active = true;
+  mi.kind = m->kind;
+  mi.dkind = d->kind;
+  mi.ncomp = ncomp;
+  mi.quad = m->quad;
+  mi.lds = (size_t)m->tiles.max_local * ncomp * sizeof(float);
+  mi.mesh = m;
+  B->dirty = true;
+  return TATVA_OK;
+}
+
+extern "C" int tatva_batch_release(tatva_batch *B, int32_t slot) {
+  TATVA_REQUIRE(B, "batch is NULL");
+  TATVA_REQUIRE(slot >= 0 && slot < B->capacity, "slot out of range");
+  std::lock_guard<std::mutex> lock(B->mu);
+  if (B->meta[slot].active) {
+    B->meta[slot] = tatva_batch::Meta{};
+    B->slots[slot] = BatchSlot{};
+    B->dirty = true;
+  }
+  return TATVA_OK;
+}
+
+extern "C" int tatva_batch_iterate(tatva_batch *B, int32_t iters, void *stream) {
+  TATVA_REQUIRE(B, "batch is NULL");
+  TATVA_REQUIRE(iters >= 0, "iters must be >= 0");
+  std::lock_guard<std::mutex> lock(B->mu);
+  TATVA_TRY_HIP(hipSetDevice(B->device));
+  hipStream_t s = (hipStream_t)stream;
+  int rc = batch_upload(B, s);
+  if (rc) return rc;
+  if (B->v_n == 0) return TATVA_OK;
+  for (int it = 0; it < iters; ++it) {
+    const int tick = (int)B->tick;
+    for (const auto &L : B->launches) {
+      rc = with_class(L.kind, L.dkind, [&](auto kc, auto dens_tag) -> int {
+        constexpr int K = decltype(kc)::value;
+        using Dens = decltype(dens_tag);
+        k_batch_assemble<K, Dens><<<L.a_n, kTile, L.lds, s>>>(B->d_slots, B->d_bmap + L.a_off,
+                                                               L.quad);
+        k_batch_gather<Dens::NC><<<L.g_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + L.g_off);
+        return TATVA_OK;
+      });
+      if (rc) return rc;
+    }
+    k_batch_update<<<B->v_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + B->v_off, tick);
+    k_batch_pupdate<<<B->v_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + B->v_off, tick);
+    TATVA_CHECK_LAUNCH();
+    B->tick += 1;
+  }
+  return TATVA_OK;
+}
+
+extern "C" int tatva_batch_poll(tatva_batch *B, tatva_pcg_stats *stats, int32_t *n_running,
+                                void *stream) {
+  TATVA_REQUIRE(B && stats, "invalid arguments");
+  std::lock_guard<std::mutex> lock(B->mu);
+  TATVA_TRY_HIP(hipSetDevice(B->device));
+  hipStream_t s = (hipStream_t)stream;
+  TATVA_TRY_HIP(hipMemcpyAsync(B->h_states, B->d_states, B->capacity * sizeof(PcgState),
+                               hipMemcpyDeviceToHost, s));
+  TATVA_TRY_HIP(hipStreamSynchronize(s));
+  int running = 0;
+  for (int i = 0; i < B->capacity; ++i) {
+    const PcgState &h = B->h_states[i];
+    tatva_pcg_stats &o = stats[i];
+    std::memset(&o, 0, sizeof(o));
+    if (!B->meta[i].active) {
+      o.status = TATVA_OK;
+      continue;
+    }
+    if (!h.done) ++running;
+    o.iterations = h.iter;
+    o.status = h.done ? h.status : TATVA_ENOTCONV;
+    o.b_norm = std::sqrt(h.bb);
+    o.rel_residual = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+    o.reserved = h.done;
+  }
+  if (n_running) *n_running = running;
+  return TATVA_OK;
+}

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -154,6 +155,41 @@ __global__ __launch_bounds__(kBlock) void k_integrate_elem(int64_t n_elem, const
   }
 }
 
+// partials[block] = block share of sum_e sum_q vals[e, q, c] * detJ * w_q for
+// one component c (grid-stride over elements; no per-element buffer).
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_integrate_comp(int64_t n_elem, const double *__restrict__ coords,
+                                                           const int32_t *__restrict__ conn,
+                                                           QuadRule quad, int nc, int c, int nodal,
+                                                           const double *__restrict__ vals,
+                                                           double *__restrict__ partials) {
+  using El = Elem<KIND>;
+  constexpr int D = El::D, NNE = El::NNE;
+  double s = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_elem;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t nodes[NNE];
+    double X[NNE][D];
+    load_element<KIND>(e, conn, coords, nodes, X);
+    for (int q = 0; q < quad.nq; ++q) {
+      const double w = qp_detJ<KIND>(X, quad.xi[q]) * quad.w[q];
+      double v;
+      if (nodal) {
+        double N[NNE];
+        El::N(quad.xi[q], N);
+        v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NNE; ++a) v += N[a] * vals[(int64_t)nodes[a] * nc + c];
+      } else {
+        v = vals[(e * quad.nq + q) * nc + c];
+      }
+      s += v * w;
+    }
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 // partials[block] = sum over the block's grid-stride share of x[i*stride + offset];
 // k_finalize_sum then adds the partials in a fixed order.
 __global__ __launch_bounds__(kBlock) void k_strided_sum(int64_t n, const double *__restrict__ x,
@@ -246,66 +282,6 @@ __global__ __launch_bounds__(kBlock) void k_energy(int64_t n_elem, const double 
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
-// Element residual vectors r_e[a][c] = sum_q w (P(G) B)_ac - f w N_a.
-template <int KIND, class Dens>
-__global__ __launch_bounds__(kBlock) void k_elem_residual(int64_t n_elem, const double *__restrict__ coords,
-                                                          const int32_t *__restrict__ conn,
-                                                          QuadRule quad, Dens dens,
-                                                          const double *__restrict__ u,
-                                                          double *__restrict__ out) {
-  using El = Elem<KIND>;
-  constexpr int D = El::D, NNE = El::NNE, NC = Dens::NC;
-  const int64_t e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e >= n_elem) return;
-  int32_t nodes[NNE];
-  double X[NNE][D];
-  load_element<KIND>(e, conn, coords, nodes, X);
-  double ue[NNE][NC], re[NNE][NC];
-#pragma unroll
-  for (int a = 0; a < NNE; ++a)
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      ue[a][c] = u[(int64_t)nodes[a] * NC + c];
-      re[a][c] = 0.0;
-    }
-  for (int q = 0; q < quad.nq; ++q) {
-    double B[D][NNE];
-    const double det = qp_geometry<KIND>(X, quad.xi[q], B);
-    const double w = det * quad.w[q];
-    double G[NC][D], P[NC][D];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        double s = 0.0;
-#pragma unroll
-        for (int a = 0; a < NNE; ++a) s += B[j][a] * ue[a][c];
-        G[c][j] = s;
-      }
-    dens.P(G, P);
-#pragma unroll
-    for (int a = 0; a < NNE; ++a)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < D; ++j) s += P[c][j] * B[j][a];
-        re[a][c] += w * s;
-      }
-    if constexpr (Dens::kSource) {
-      double N[NNE];
-      El::N(quad.xi[q], N);
-#pragma unroll
-      for (int a = 0; a < NNE; ++a) re[a][0] -= w * dens.f * N[a];
-    }
-  }
-  double *dst = out + e * NNE * NC;
-#pragma unroll
-  for (int a = 0; a < NNE; ++a)
-#pragma unroll
-    for (int c = 0; c < NC; ++c) dst[a * NC + c] = re[a][c];
-}
-
 // Element stiffness, one thread per (element, column j = b*NC + k):
 // column j of K_e is the element JVP of the residual with the unit tangent
 // on local dof j.  Layout: Ke[e][row][col], row/col = a*NC + c.
@@ -370,70 +346,6 @@ __global__ __launch_bounds__(kBlock) void k_elem_stiffness(int64_t n_elem, const
     for (int c = 0; c < NC; ++c) dst[(a * NC + c) * ND] = ke[a][c];
 }
 
-// Diagonal of the element stiffness, one thread per element.
-template <int KIND, class Dens, class T>
-__global__ __launch_bounds__(kBlock) void k_elem_diag(int64_t n_elem, const double *__restrict__ coords,
-                                                      const int32_t *__restrict__ conn, QuadRule quad,
-                                                      Dens dens, const T *__restrict__ u,
-                                                      T *__restrict__ out) {
-  using El = Elem<KIND>;
-  constexpr int D = El::D, NNE = El::NNE, NC = Dens::NC;
-  const int64_t e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e >= n_elem) return;
-  int32_t nodes[NNE];
-  double X[NNE][D];
-  load_element<KIND>(e, conn, coords, nodes, X);
-  T ue[NNE][NC], de[NNE][NC];
-#pragma unroll
-  for (int a = 0; a < NNE; ++a)
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      ue[a][c] = u[(int64_t)nodes[a] * NC + c];
-      de[a][c] = T(0);
-    }
-  for (int q = 0; q < quad.nq; ++q) {
-    double Bd[D][NNE];
-    const double det = qp_geometry<KIND>(X, quad.xi[q], Bd);
-    const T w = T(det * quad.w[q]);
-    T B[D][NNE];
-#pragma unroll
-    for (int j = 0; j < D; ++j)
-#pragma unroll
-      for (int a = 0; a < NNE; ++a) B[j][a] = T(Bd[j][a]);
-    T G[NC][D];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        T s = T(0);
-#pragma unroll
-        for (int a = 0; a < NNE; ++a) s += B[j][a] * ue[a][c];
-        G[c][j] = s;
-      }
-    const auto L = dens.linearize(G);
-#pragma unroll
-    for (int a = 0; a < NNE; ++a)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        T dG[NC][D], dP[NC][D];
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc)
-#pragma unroll
-          for (int j = 0; j < D; ++j) dG[cc][j] = (cc == c) ? B[j][a] : T(0);
-        dens.dP(L, dG, dP);
-        T s = T(0);
-#pragma unroll
-        for (int j = 0; j < D; ++j) s += dP[c][j] * B[j][a];
-        de[a][c] += w * s;
-      }
-  }
-  T *dst = out + e * NNE * NC;
-#pragma unroll
-  for (int a = 0; a < NNE; ++a)
-#pragma unroll
-    for (int c = 0; c < NC; ++c) dst[a * NC + c] = de[a][c];
-}
-
 // Colored Hessian decompression, one wavefront per matrix row.
 // The row's compressed Jacobian J[row, 0..n_colors) lives in LDS:
 //   J[row, colors[s]] = sum over elements e containing the row's node (ascending
@@ -487,6 +399,152 @@ __global__ __launch_bounds__(kBlock) void k_colored_rows(
 }  // namespace tatva
 
 using namespace tatva;
+
+// ------------------------------------------------------------ tiling -------
+
+namespace {
+
+struct HostTiles {
+  int32_t n_tiles = 0, max_local = 0, max_colors = 0;
+  std::vector<int32_t> elems;       // n_tiles * kTile
+  std::vector<uint16_t> lnodes;     // n_tiles * kTile * nne
+  std::vector<int16_t> color_off;   // n_tiles * (max_colors + 1)
+  std::vector<int32_t> node_ptr;    // n_tiles + 1
+  std::vector<int32_t> slot_ptr;    // n_nodes + 1
+  std::vector<int32_t> slot_list;   // slots, ascending tile per node
+};
+
+uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+// Z-order of element centroids, tiles of kTile consecutive elements in that
+// order, local node numbering per tile, greedy element coloring inside each
+// tile (elements of a color share no node), node -> slot lists.
+int build_tiles(const double *coords, int dim, int64_t N, const int32_t *elements, int64_t E,
+                int nne, HostTiles &T) {
+  std::vector<double> cen((size_t)E * 3, 0.0);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t e = 0; e < E; ++e)
+    for (int j = 0; j < dim; ++j) {
+      double s = 0.0;
+      for (int a = 0; a < nne; ++a) s += coords[(int64_t)elements[e * nne + a] * dim + j];
+      s /= nne;
+      cen[(size_t)e * 3 + j] = s;
+      lo[j] = std::min(lo[j], s);
+      hi[j] = std::max(hi[j], s);
+    }
+  std::vector<std::pair<uint64_t, int32_t>> key((size_t)E);
+  for (int64_t e = 0; e < E; ++e) {
+    uint64_t code = 0;
+    for (int j = 0; j < dim; ++j) {
+      const double span = hi[j] - lo[j];
+      const double t = span > 0 ? (cen[(size_t)e * 3 + j] - lo[j]) / span : 0.0;
+      const uint64_t qv = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+      code |= spread3(qv) << j;
+    }
+    key[(size_t)e] = {code, (int32_t)e};
+  }
+  std::sort(key.begin(), key.end());
+  const int64_t n_tiles = (E + kTile - 1) / kTile;
+  if (n_tiles >= (int64_t(1) << 31)) {
+    set_error("too many tiles");
+    return TATVA_EINVAL;
+  }
+  T.n_tiles = (int32_t)n_tiles;
+  T.elems.assign((size_t)n_tiles * kTile, -1);
+  T.lnodes.assign((size_t)n_tiles * kTile * nne, 0);
+  T.node_ptr.assign((size_t)n_tiles + 1, 0);
+  std::vector<std::vector<int16_t>> offs((size_t)n_tiles);
+  std::vector<int32_t> tile_nodes;  // global node id of every slot
+  std::vector<int32_t> loc;         // scratch
+  std::vector<uint64_t> used;       // per local node: color bitmask (128 colors)
+  for (int64_t t = 0; t < n_tiles; ++t) {
+    const int64_t b = t * kTile, cnt = std::min<int64_t>(kTile, E - b);
+    loc.clear();
+    for (int64_t k = 0; k < cnt; ++k)
+      for (int a = 0; a < nne; ++a) loc.push_back(elements[(int64_t)key[(size_t)(b + k)].second * nne + a]);
+    std::sort(loc.begin(), loc.end());
+    loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+    if (loc.size() > 65535) {
+      set_error("tile has too many local nodes");
+      return TATVA_EINVAL;
+    }
+    T.max_local = std::max<int32_t>(T.max_local, (int32_t)loc.size());
+    used.assign(loc.size() * 2, 0);
+    std::vector<int> color((size_t)cnt);
+    std::vector<std::array<uint16_t, 8>> ln((size_t)cnt);
+    int ncol = 0;
+    for (int64_t k = 0; k < cnt; ++k) {
+      const int32_t e = key[(size_t)(b + k)].second;
+      uint64_t m0 = 0, m1 = 0;
+      for (int a = 0; a < nne; ++a) {
+        const int32_t nd = elements[(int64_t)e * nne + a];
+        const uint16_t l = (uint16_t)(std::lower_bound(loc.begin(), loc.end(), nd) - loc.begin());
+        ln[(size_t)k][a] = l;
+        m0 |= used[2 * l];
+        m1 |= used[2 * l + 1];
+      }
+      int c = -1;
+      if (~m0) c = __builtin_ctzll(~m0);
+      else if (~m1) c = 64 + __builtin_ctzll(~m1);
+      if (c < 0) {
+        set_error("more than 128 element colors in a tile");
+        return TATVA_EINVAL;
+      }
+      color[(size_t)k] = c;
+      ncol = std::max(ncol, c + 1);
+      for (int a = 0; a < nne; ++a) {
+        const uint16_t l = ln[(size_t)k][a];
+        used[2 * l + (c >> 6)] |= uint64_t(1) << (c & 63);
+      }
+    }
+    T.max_colors = std::max(T.max_colors, ncol);
+    // order the tile's elements by color (stable)
+    std::vector<int64_t> ord((size_t)cnt);
+    for (int64_t k = 0; k < cnt; ++k) ord[(size_t)k] = k;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int64_t x, int64_t y) { return color[(size_t)x] < color[(size_t)y]; });
+    std::vector<int16_t> off((size_t)ncol + 1, 0);
+    for (int64_t k = 0; k < cnt; ++k) off[(size_t)color[(size_t)k] + 1]++;
+    for (int c = 0; c < ncol; ++c) off[(size_t)c + 1] += off[(size_t)c];
+    offs[(size_t)t] = off;
+    for (int64_t k = 0; k < cnt; ++k) {
+      const int64_t src = ord[(size_t)k];
+      T.elems[(size_t)(b + k)] = key[(size_t)(b + src)].second;
+      for (int a = 0; a < nne; ++a) T.lnodes[(size_t)((b + k) * nne + a)] = ln[(size_t)src][a];
+    }
+    T.node_ptr[(size_t)t + 1] = T.node_ptr[(size_t)t] + (int32_t)loc.size();
+    tile_nodes.insert(tile_nodes.end(), loc.begin(), loc.end());
+    if ((int64_t)tile_nodes.size() >= (int64_t(1) << 31)) {
+      set_error("too many tile slots");
+      return TATVA_EINVAL;
+    }
+  }
+  T.color_off.assign((size_t)n_tiles * (T.max_colors + 1), 0);
+  for (int64_t t = 0; t < n_tiles; ++t) {
+    const auto &off = offs[(size_t)t];
+    const int16_t last = off.back();
+    for (int c = 0; c <= T.max_colors; ++c)
+      T.color_off[(size_t)t * (T.max_colors + 1) + c] = c < (int)off.size() ? off[(size_t)c] : last;
+  }
+  // node -> slots (tiles visited in order, so each list ascends by tile)
+  T.slot_ptr.assign((size_t)N + 1, 0);
+  for (int32_t nd : tile_nodes) T.slot_ptr[(size_t)nd + 1]++;
+  for (int64_t i = 0; i < N; ++i) T.slot_ptr[(size_t)i + 1] += T.slot_ptr[(size_t)i];
+  T.slot_list.assign(tile_nodes.size(), 0);
+  std::vector<int32_t> fill(T.slot_ptr.begin(), T.slot_ptr.end() - 1);
+  for (size_t sl = 0; sl < tile_nodes.size(); ++sl) T.slot_list[(size_t)fill[(size_t)tile_nodes[sl]]++] = (int32_t)sl;
+  return TATVA_OK;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------- C ABI ----
 
@@ -560,9 +618,34 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     tatva_mesh_destroy(m);
     return TATVA_EHIP;
   };
+  HostTiles T;
+  {
+    const int rc = build_tiles(coords, dim, n_nodes, elements, n_elements, nne, T);
+    if (rc) {
+      delete m;
+      return rc;
+    }
+  }
+  m->n_slots = T.node_ptr.back();
   hipError_t err = hipSetDevice(device);
   if (err != hipSuccess) return fail(err, "hipSetDevice");
-  m->scratch_bytes = n_elements * nne * dim * (int64_t)sizeof(double);
+  // tile partial sums: n_slots x ncomp (ncomp <= dim) values of at most 8 bytes
+  m->scratch_bytes = std::max<int64_t>(1, m->n_slots) * dim * (int64_t)sizeof(double);
+  auto upload = [&](auto **dst, const auto &vec, const char *what) -> hipError_t {
+    using V = typename std::remove_reference<decltype(vec)>::type::value_type;
+    hipError_t e2 = hipMalloc((void **)dst, std::max<size_t>(1, vec.size()) * sizeof(V));
+    if (e2 != hipSuccess) return e2;
+    (void)what;
+    return hipMemcpy(*dst, vec.data(), vec.size() * sizeof(V), hipMemcpyHostToDevice);
+  };
+  if ((err = upload(&m->t_elems, T.elems, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if ((err = upload(&m->t_lnodes, T.lnodes, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if ((err = upload(&m->t_color_off, T.color_off, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if ((err = upload(&m->t_node_ptr, T.node_ptr, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if ((err = upload(&m->t_slot_ptr, T.slot_ptr, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if ((err = upload(&m->t_slot_list, T.slot_list, "tiles")) != hipSuccess) return fail(err, "tiles");
+  m->tiles = TileSet{T.n_tiles, T.max_local, T.max_colors, 0, m->t_elems, m->t_lnodes,
+                     m->t_color_off, m->t_node_ptr, m->t_slot_ptr, m->t_slot_list};
   if ((err = hipMalloc(&m->coords, n_nodes * dim * sizeof(double))) != hipSuccess)
     return fail(err, "hipMalloc coords");
   if ((err = hipMalloc(&m->conn, total * sizeof(int32_t))) != hipSuccess)
@@ -591,12 +674,16 @@ extern "C" int tatva_mesh_destroy(tatva_mesh *m) {
   if (!m) return TATVA_OK;
   (void)hipSetDevice(m->device);
   (void)hipDeviceSynchronize();
+  forget_mesh_graphs(m);
   if (m->coords) (void)hipFree(m->coords);
   if (m->conn) (void)hipFree(m->conn);
   if (m->inv_ptr) (void)hipFree(m->inv_ptr);
   if (m->inv_list) (void)hipFree(m->inv_list);
   if (m->scratch) (void)hipFree(m->scratch);
   if (m->partials) (void)hipFree(m->partials);
+  for (void *p : {(void *)m->t_elems, (void *)m->t_lnodes, (void *)m->t_color_off,
+                  (void *)m->t_node_ptr, (void *)m->t_slot_ptr, (void *)m->t_slot_list})
+    if (p) (void)hipFree(p);
   delete m;
   return TATVA_OK;
 }
@@ -651,29 +738,27 @@ extern "C" int tatva_integrate(tatva_mesh *m, int nc, int nodal, const double *v
   MESH_PRELUDE(m);
   TATVA_REQUIRE(values && nc >= 1, "invalid arguments");
   hipStream_t s = (hipStream_t)stream;
-  double *pe = per_element;
-  if (!pe) {
-    TATVA_REQUIRE(m->n_elements * nc * (int64_t)sizeof(double) <= m->scratch_bytes,
-                  "too many components for the mesh scratch buffer");
-    pe = (double *)m->scratch;
-  }
-  int rc = with_kind(m->kind, [&](auto kc) -> int {
+  return with_kind(m->kind, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
-    k_integrate_elem<K><<<grid_for(m->n_elements), kBlock, 0, s>>>(
-        m->n_elements, m->coords, m->conn, m->quad, nc, nodal, values, pe);
+    const int g = reduce_grid(m->n_elements);
+    if (per_element) {
+      k_integrate_elem<K><<<grid_for(m->n_elements), kBlock, 0, s>>>(
+          m->n_elements, m->coords, m->conn, m->quad, nc, nodal, values, per_element);
+      if (total)
+        for (int c = 0; c < nc; ++c) {
+          k_strided_sum<<<g, kBlock, 0, s>>>(m->n_elements, per_element, nc, c, m->partials);
+          k_finalize_sum<double><<<1, kBlock, 0, s>>>(m->partials, g, total + c, 0);
+        }
+    } else if (total) {
+      for (int c = 0; c < nc; ++c) {
+        k_integrate_comp<K><<<g, kBlock, 0, s>>>(m->n_elements, m->coords, m->conn, m->quad, nc,
+                                                  c, nodal, values, m->partials);
+        k_finalize_sum<double><<<1, kBlock, 0, s>>>(m->partials, g, total + c, 0);
+      }
+    }
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   });
-  if (rc) return rc;
-  if (total) {
-    const int g = reduce_grid(m->n_elements);
-    for (int c = 0; c < nc; ++c) {
-      k_strided_sum<<<g, kBlock, 0, s>>>(m->n_elements, pe, nc, c, m->partials);
-      k_finalize_sum<double><<<1, kBlock, 0, s>>>(m->partials, g, total + c, 0);
-      TATVA_CHECK_LAUNCH();
-    }
-  }
-  return TATVA_OK;
 }
 
 extern "C" int tatva_density_eval(int dim, int64_t n_points, const tatva_density *d,
@@ -725,79 +810,50 @@ extern "C" int tatva_energy(tatva_mesh *m, const tatva_density *d, const double 
   return TATVA_OK;
 }
 
-template <int NNE, int NC, class T>
-static int launch_gather(tatva_mesh *m, const T *elem, const double *f_ext, const int32_t *dof_map,
-                         T *out, hipStream_t s, const int *skip = nullptr) {
-  k_node_gather<NNE, NC, T><<<grid_for(m->n_nodes), kBlock, 0, s>>>(
-      m->n_nodes, m->inv_ptr, m->inv_list, elem, f_ext, dof_map, out, skip);
-  TATVA_CHECK_LAUNCH();
-  return TATVA_OK;
-}
-
-extern "C" int tatva_residual(tatva_mesh *m, const tatva_density *d, const double *u,
-                              const double *f_ext, const int32_t *dof_map, double *r,
-                              void *stream) {
-  MESH_PRELUDE(m);
-  TATVA_REQUIRE(u && r, "invalid arguments");
-  hipStream_t s = (hipStream_t)stream;
-  double *elem = (double *)m->scratch;
-  return with_kind(m->kind, [&](auto kc) -> int {
-    constexpr int K = decltype(kc)::value;
-    return with_density<Elem<K>::D, double>(d, [&](auto dens) -> int {
-      using Dens = decltype(dens);
-      k_elem_residual<K, Dens><<<grid_for(m->n_elements), kBlock, 0, s>>>(
-          m->n_elements, m->coords, m->conn, m->quad, dens, u, elem);
-      TATVA_CHECK_LAUNCH();
-      return launch_gather<Elem<K>::NNE, Dens::NC, double>(m, elem, f_ext, dof_map, r, s);
-    });
-  });
-}
-
 namespace tatva {
 
-template <class T>
-static int launch_hvp_t(tatva_mesh *m, const tatva_density *d, const T *u, const T *v,
-                        const int32_t *dof_map, T *y, hipStream_t s, const int *skip) {
-  T *elem = (T *)m->scratch;
+// Tile assembly + slot gather for one mode; out is reduced through dof_map.
+template <class T, int MODE>
+static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, const T *v,
+                           const int32_t *dof_map, const double *f_ext, T *out, hipStream_t s,
+                           const int *skip) {
+  T *partial = (T *)m->scratch;
   return with_kind(m->kind, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
     return with_density<Elem<K>::D, T>(d, [&](auto dens) -> int {
       using Dens = decltype(dens);
-      k_elem_hvp<K, Dens, T><<<grid_for(m->n_elements), kBlock, 0, s>>>(
-          m->n_elements, m->coords, m->conn, m->quad, dens, u, v, dof_map, elem, skip);
+      constexpr int NC = Dens::NC;
+      const size_t lds = (size_t)m->tiles.max_local * NC * sizeof(T);
+      k_tile_assemble<K, Dens, T, MODE><<<m->tiles.n_tiles, kTile, lds, s>>>(
+          m->coords, m->conn, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
+      k_slot_gather<NC, T><<<grid_for(m->n_nodes), kBlock, 0, s>>>(
+          m->n_nodes, m->t_slot_ptr, m->t_slot_list, partial, f_ext, dof_map, out, skip);
       TATVA_CHECK_LAUNCH();
-      return launch_gather<Elem<K>::NNE, Dens::NC, T>(m, elem, nullptr, dof_map, y, s, skip);
+      return TATVA_OK;
     });
   });
 }
 
 int launch_hvp_f64(tatva_mesh *m, const tatva_density *d, const double *u, const double *v,
                    const int32_t *dof_map, double *y, hipStream_t s, const int *skip) {
-  return launch_hvp_t<double>(m, d, u, v, dof_map, y, s, skip);
+  return launch_assembly<double, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip);
 }
 
 int launch_hvp_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                    const int32_t *dof_map, float *y, hipStream_t s, const int *skip) {
-  return launch_hvp_t<float>(m, d, u, v, dof_map, y, s, skip);
-}
-
-template <class T>
-static int launch_diag_t(tatva_mesh *m, const tatva_density *d, const T *u,
-                         const int32_t *dof_map, T *diag, hipStream_t s) {
-  T *elem = (T *)m->scratch;
-  return with_kind(m->kind, [&](auto kc) -> int {
-    constexpr int K = decltype(kc)::value;
-    return with_density<Elem<K>::D, T>(d, [&](auto dens) -> int {
-      using Dens = decltype(dens);
-      k_elem_diag<K, Dens, T><<<grid_for(m->n_elements), kBlock, 0, s>>>(
-          m->n_elements, m->coords, m->conn, m->quad, dens, u, elem);
-      TATVA_CHECK_LAUNCH();
-      return launch_gather<Elem<K>::NNE, Dens::NC, T>(m, elem, nullptr, dof_map, diag, s);
-    });
-  });
+  return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip);
 }
 
 }  // namespace tatva
+
+extern "C" int tatva_residual(tatva_mesh *m, const tatva_density *d, const double *u,
+                              const double *f_ext, const int32_t *dof_map, double *r,
+                              void *stream) {
+  MESH_PRELUDE(m);
+  TATVA_REQUIRE(u && r, "invalid arguments");
+  return launch_assembly<double, kModeResidual>(m, d, u, nullptr, dof_map, f_ext, r,
+                                                (hipStream_t)stream, nullptr);
+}
 
 extern "C" int tatva_hvp(tatva_mesh *m, const tatva_density *d, const double *u, const double *v,
                          const int32_t *dof_map, double *y, void *stream) {
@@ -817,14 +873,16 @@ extern "C" int tatva_hessian_diag(tatva_mesh *m, const tatva_density *d, const d
                                   const int32_t *dof_map, double *diag, void *stream) {
   MESH_PRELUDE(m);
   TATVA_REQUIRE(u && diag, "invalid arguments");
-  return launch_diag_t<double>(m, d, u, dof_map, diag, (hipStream_t)stream);
+  return launch_assembly<double, kModeDiag>(m, d, u, nullptr, dof_map, nullptr, diag,
+                                             (hipStream_t)stream, nullptr);
 }
 
 extern "C" int tatva_hessian_diag_f32(tatva_mesh *m, const tatva_density *d, const float *u,
                                       const int32_t *dof_map, float *diag, void *stream) {
   MESH_PRELUDE(m);
   TATVA_REQUIRE(u && diag, "invalid arguments");
-  return launch_diag_t<float>(m, d, u, dof_map, diag, (hipStream_t)stream);
+  return launch_assembly<float, kModeDiag>(m, d, u, nullptr, dof_map, nullptr, diag,
+                                            (hipStream_t)stream, nullptr);
 }
 
 extern "C" int64_t tatva_colored_hessian_workspace(tatva_mesh *m, int nc) {

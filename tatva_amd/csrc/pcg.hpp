@@ -1,0 +1,46 @@
+// Jacobi-PCG state shared by the single-solve and batched solvers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+#include "mesh.hpp"
+
+namespace tatva {
+
+// Device-side record of one solve; block 0 of the last kernel of an
+// iteration writes it, the host reads it between launches.
+struct PcgState {
+  double rr;
+  double bb;
+  double tol2;
+  int iter;
+  int done;
+  int status;
+  int maxiter;
+};
+
+// Vector kernels use at most this many blocks (4 per CU): the partials each
+// block re-sums stay a few KB.
+constexpr int kVecBlocks = 1024;
+
+inline int vec_grid(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > kVecBlocks) b = kVecBlocks;
+  return (int)b;
+}
+
+// Matrix-free FP32 PCG set-up on stream s: st reset, dinv = 1/diag,
+// r = b - H(u) x, p = D^-1 r, the shares of r.z (into part_rz1), r.r
+// (part_rr1) and b.b over gv blocks, and the state (tol, maxiter, done when
+// b == 0 or r already small).
+int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
+                      const int32_t *dof_map, int64_t n, const float *diag, const float *b,
+                      const float *x, float *r, float *p, float *Ap, float *dinv,
+                      double *part_rz1, double *part_rr1, double *part_bb, int gv, double tol,
+                      int maxiter, PcgState *st, hipStream_t s);
+
+}  // namespace tatva

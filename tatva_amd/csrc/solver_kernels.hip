@@ -24,18 +24,9 @@
 
 #include "common.hpp"
 #include "mesh.hpp"
+#include "pcg.hpp"
 
 namespace tatva {
-
-struct PcgState {
-  double rr;
-  double bb;
-  double tol2;
-  int iter;
-  int done;
-  int status;
-  int maxiter;
-};
 
 // Partial-sum buffers of one solve: pap, rz[2], rr[2], bb (kMaxReduceBlocks each).
 struct PcgParts {
@@ -46,17 +37,6 @@ struct PcgParts {
   double *rz(int q) const { return q ? rz1 : rz0; }
   double *rr(int q) const { return q ? rr1 : rr0; }
 };
-
-// Vector kernels use at most this many blocks (4 per CU): the partials each
-// block re-sums stay a few KB.
-constexpr int kVecBlocks = 1024;
-
-inline int vec_grid(int64_t n) {
-  int64_t b = (n + kBlock - 1) / kBlock;
-  if (b < 1) b = 1;
-  if (b > kVecBlocks) b = kVecBlocks;
-  return (int)b;
-}
 
 // ----------------------------------------------------------------- SpMV ----
 
@@ -454,6 +434,33 @@ __global__ void k_state_reset(PcgState *st) {
 
 // ----------------------------------------------------- device resources ----
 
+// An instantiated chunk graph and the launch parameters it was captured with.
+// Solver kernels read and write only workspace buffers and the operator's
+// arrays, so a graph is reused whenever these pointers and sizes repeat
+// (Newton iterations, load steps, benchmark steps).
+struct GraphKey {
+  const void *ws;
+  const void *a, *b, *c, *d, *e;  // operator arrays (CSR / mesh / state)
+  int64_t n, m;
+  int i0, i1, i2, i3;
+  double f0, f1, f2;  // density parameters baked into captured kernel arguments
+  bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+struct CachedGraph {
+  GraphKey key{};
+  bool valid = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    valid = false;
+  }
+};
+
 struct DeviceRes {
   std::mutex mu;
   bool init = false;
@@ -461,10 +468,21 @@ struct DeviceRes {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipEvent_t prof_a = nullptr, prof_b = nullptr;  // timing events around a sampled operator
   PcgState *host_state = nullptr;
+  CachedGraph csr, matfree;
 };
 
 static DeviceRes g_res[64];
 static std::atomic<int> g_profiling{0};
+
+// Drops cached solver graphs that captured this mesh's arrays (called before
+// the mesh's device memory is freed).
+void forget_mesh_graphs(const tatva_mesh *m) {
+  for (auto &r : g_res) {
+    if (!r.init) continue;
+    std::lock_guard<std::mutex> lock(r.mu);
+    if (r.matfree.valid && r.matfree.key.a == (const void *)m) r.matfree.reset();
+  }
+}
 
 static int get_res(int device, DeviceRes **out) {
   if (device < 0 || device >= 64) {
@@ -513,7 +531,7 @@ static int spmv_grid(int64_t n, int lpr) {
 }
 
 struct WsLayout {
-  size_t vec[5];
+  size_t vec[8];
   size_t parts;
   size_t state;
   size_t total;
@@ -587,19 +605,33 @@ static int run_chunks(DeviceRes *res, hipGraphExec_t exec, PcgState *st, int max
   return TATVA_OK;
 }
 
-// Captures one chunk (even length) of `step(q)` calls, q = iteration parity.
-static int capture_chunk(hipStream_t s, int chunk, const std::function<int(int, bool)> &step,
-                         bool profiling, hipGraph_t *graph, hipGraphExec_t *exec) {
+// Returns the chunk graph (even length) of `step(q)` calls, q = iteration
+// parity, capturing it unless the cache already holds one for `key`.
+static int chunk_graph(hipStream_t s, int chunk, const std::function<int(int, bool)> &step,
+                       bool profiling, const GraphKey &key, CachedGraph &cache) {
+  if (cache.valid && cache.key == key) return TATVA_OK;
+  cache.reset();
   TATVA_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = TATVA_OK;
   for (int i = 0; i < chunk && rc == TATVA_OK; ++i) rc = step(i & 1, !(i == 0 && profiling));
-  hipError_t ce = hipStreamEndCapture(s, graph);
-  if (rc) return rc;
+  hipError_t ce = hipStreamEndCapture(s, &cache.graph);
+  if (rc) {
+    cache.reset();
+    return rc;
+  }
   if (ce != hipSuccess) {
+    cache.reset();
     set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
     return TATVA_EHIP;
   }
-  TATVA_TRY_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+  hipError_t ie = hipGraphInstantiate(&cache.exec, cache.graph, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    cache.reset();
+    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    return TATVA_EHIP;
+  }
+  cache.key = key;
+  cache.valid = true;
   return TATVA_OK;
 }
 
@@ -677,21 +709,22 @@ extern "C" int tatva_spmv_bcsr(int64_t n, const int32_t *indptr, const int32_t *
   return TATVA_OK;
 }
 
-extern "C" int64_t tatva_pcg_csr_workspace(int64_t n) { return (int64_t)ws_layout(n, 4, sizeof(double)).total; }
+// workspace vectors: p, r, Ap, D^-1, x, b
+extern "C" int64_t tatva_pcg_csr_workspace(int64_t n) { return (int64_t)ws_layout(n, 6, sizeof(double)).total; }
 
 extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
                              const int32_t *indices, const double *data,
-                             const int32_t *block_cols, int32_t block_size, const double *b,
-                             double *x, double tol, int32_t maxiter, int32_t chunk,
+                             const int32_t *block_cols, int32_t block_size, const double *b_in,
+                             double *x_io, double tol, int32_t maxiter, int32_t chunk,
                              void *workspace, int64_t ws_bytes, tatva_pcg_stats *stats,
                              void *stream) {
-  TATVA_REQUIRE(n > 0 && nnz > 0 && indptr && indices && data && b && x, "invalid arguments");
+  TATVA_REQUIRE(n > 0 && nnz > 0 && indptr && indices && data && b_in && x_io, "invalid arguments");
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
   if (block_cols)
     TATVA_REQUIRE((block_size == 2 || block_size == 3) && n % block_size == 0,
                   "block view needs block_size 2 or 3 dividing n");
   TATVA_REQUIRE(tol >= 0.0, "tol must be non-negative");
-  const WsLayout L = ws_layout(n, 4, sizeof(double));
+  const WsLayout L = ws_layout(n, 6, sizeof(double));
   TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
   chunk += chunk & 1;  // parity ping-pong needs an even chunk
   int device = 0;
@@ -705,12 +738,16 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   double *r = (double *)(w + L.vec[1]);
   double *Ap = (double *)(w + L.vec[2]);
   double *dinv = (double *)(w + L.vec[3]);
+  double *x = (double *)(w + L.vec[4]);
+  double *b = (double *)(w + L.vec[5]);
   const PcgParts P = parts_at(w, L);
   PcgState *st = (PcgState *)(w + L.state);
   hipStream_t caller = (hipStream_t)stream;
   hipStream_t s = res->cap;
   TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
   TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
+  TATVA_TRY_HIP(hipMemcpyAsync(x, x_io, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  TATVA_TRY_HIP(hipMemcpyAsync(b, b_in, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   const int lpr = choose_lpr(n, nnz);
   const int gv = vec_grid(n);
   const bool profiling = g_profiling.load() != 0;
@@ -762,31 +799,69 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
     k_pcg_pupdate<double><<<gv, kBlock, 0, s>>>(n, p, r, dinv, P.rz(q ^ 1), P.rz(q), P.rr(q), st);
     return TATVA_OK;
   };
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  rc = capture_chunk(s, chunk, step, profiling, &graph, &exec);
-  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, apply_op_dot, stats);
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  GraphKey key{};
+  key.ws = workspace;
+  key.a = indptr;
+  key.b = indices;
+  key.c = data;
+  key.d = block_cols;
+  key.n = n;
+  key.m = nnz;
+  key.i0 = nblk;
+  key.i1 = chunk;
+  key.i2 = profiling ? 1 : 0;
+  key.i3 = lpr;
+  rc = chunk_graph(s, chunk, step, profiling, key, res->csr);
+  if (rc == TATVA_OK)
+    rc = run_chunks(res, res->csr.exec, st, maxiter, chunk, profiling, apply_op_dot, stats);
+  if (rc == TATVA_OK)
+    TATVA_TRY_HIP(hipMemcpyAsync(x_io, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
   TATVA_TRY_HIP(hipStreamWaitEvent(caller, res->ev_out, 0));
   return rc;
 }
 
-extern "C" int64_t tatva_pcg_matfree_workspace(tatva_mesh *m, int64_t n) {
-  (void)m;
-  return (int64_t)ws_layout(n, 5, sizeof(float)).total;
+// workspace: reduced vectors p, r, Ap, D^-1, x, b, D (FP32), then the full FP32 state u
+namespace tatva {
+
+int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
+                      const int32_t *dof_map, int64_t n, const float *diag, const float *b,
+                      const float *x, float *r, float *p, float *Ap, float *dinv,
+                      double *part_rz1, double *part_rr1, double *part_bb, int gv, double tol,
+                      int maxiter, PcgState *st, hipStream_t s) {
+  k_state_reset<<<1, 64, 0, s>>>(st);
+  k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
+  // r = b - H x
+  const int rc = launch_hvp_f32(m, d, u, x, dof_map, Ap, s);
+  if (rc) return rc;
+  k_sub<float><<<grid_for(n), kBlock, 0, s>>>(n, b, Ap, r);
+  k_pcg_init<float><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, part_rz1, part_rr1, part_bb);
+  k_pcg_init_state<<<1, kBlock, 0, s>>>(part_rr1, part_bb, gv, tol, maxiter, st);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
 }
 
-extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, const float *u,
-                                     const int32_t *dof_map, int64_t n, const float *diag,
-                                     const float *b, float *x, double tol, int32_t maxiter,
+}  // namespace tatva
+
+static size_t matfree_u_offset(int64_t n) { return ws_layout(n, 7, sizeof(float)).total; }
+
+extern "C" int64_t tatva_pcg_matfree_workspace(tatva_mesh *m, int64_t n) {
+  if (!m || n <= 0) return -1;
+  const int64_t n_full = m->n_nodes * m->dim;
+  return (int64_t)(matfree_u_offset(n) + align256((size_t)n_full * sizeof(float)));
+}
+
+extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, const float *u_in,
+                                     const int32_t *dof_map, int64_t n, const float *diag_in,
+                                     const float *b_in, float *x_io, double tol, int32_t maxiter,
                                      int32_t chunk, void *workspace, int64_t ws_bytes,
                                      tatva_pcg_stats *stats, void *stream) {
-  TATVA_REQUIRE(m && d && u && diag && b && x && n > 0, "invalid arguments");
+  TATVA_REQUIRE(m && d && u_in && diag_in && b_in && x_io && n > 0, "invalid arguments");
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
-  const WsLayout L = ws_layout(n, 5, sizeof(float));
-  TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
+  const WsLayout L = ws_layout(n, 7, sizeof(float));
+  TATVA_REQUIRE(workspace && ws_bytes >= tatva_pcg_matfree_workspace(m, n),
+                "PCG workspace too small");
+  const int64_t n_full = m->n_nodes * density_ncomp(m, d);
   chunk += chunk & 1;
   TATVA_TRY_HIP(hipSetDevice(m->device));
   DeviceRes *res = nullptr;
@@ -798,23 +873,25 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   float *r = (float *)(w + L.vec[1]);
   float *Ap = (float *)(w + L.vec[2]);
   float *dinv = (float *)(w + L.vec[3]);
+  float *x = (float *)(w + L.vec[4]);
+  float *b = (float *)(w + L.vec[5]);
+  float *diag = (float *)(w + L.vec[6]);
+  float *u = (float *)(w + matfree_u_offset(n));
   const PcgParts P = parts_at(w, L);
   PcgState *st = (PcgState *)(w + L.state);
   hipStream_t caller = (hipStream_t)stream;
   hipStream_t s = res->cap;
   TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
   TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
+  TATVA_TRY_HIP(hipMemcpyAsync(x, x_io, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  TATVA_TRY_HIP(hipMemcpyAsync(b, b_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  TATVA_TRY_HIP(hipMemcpyAsync(diag, diag_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  TATVA_TRY_HIP(hipMemcpyAsync(u, u_in, n_full * sizeof(float), hipMemcpyDeviceToDevice, s));
   const int gv = vec_grid(n);
   const bool profiling = g_profiling.load() != 0;
-  k_state_reset<<<1, 64, 0, s>>>(st);
-  k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
-  // r = b - H x
-  rc = launch_hvp_f32(m, d, u, x, dof_map, Ap, s);
+  rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, r, p, Ap, dinv, P.rz(1), P.rr(1), P.bb,
+                         gv, tol, maxiter, st, s);
   if (rc) return rc;
-  k_sub<float><<<grid_for(n), kBlock, 0, s>>>(n, b, Ap, r);
-  k_pcg_init<float><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, P.rz(1), P.rr(1), P.bb);
-  k_pcg_init_state<<<1, kBlock, 0, s>>>(P.rr(1), P.bb, gv, tol, maxiter, st);
-  TATVA_CHECK_LAUNCH();
   const std::function<int()> apply_op = [&]() -> int {
     return launch_hvp_f32(m, d, u, p, dof_map, Ap, s, &st->done);
   };
@@ -829,12 +906,23 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
     k_pcg_pupdate<float><<<gv, kBlock, 0, s>>>(n, p, r, dinv, P.rz(q ^ 1), P.rz(q), P.rr(q), st);
     return TATVA_OK;
   };
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  rc = capture_chunk(s, chunk, step, profiling, &graph, &exec);
-  if (rc == TATVA_OK) rc = run_chunks(res, exec, st, maxiter, chunk, profiling, apply_op, stats);
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  GraphKey key{};
+  key.ws = workspace;
+  key.a = m;
+  key.b = dof_map;
+  key.n = n;
+  key.m = n_full;
+  key.i0 = d->kind;
+  key.i1 = chunk;
+  key.i2 = profiling ? 1 : 0;
+  key.f0 = d->p0;
+  key.f1 = d->p1;
+  key.f2 = d->p2;
+  rc = chunk_graph(s, chunk, step, profiling, key, res->matfree);
+  if (rc == TATVA_OK)
+    rc = run_chunks(res, res->matfree.exec, st, maxiter, chunk, profiling, apply_op, stats);
+  if (rc == TATVA_OK)
+    TATVA_TRY_HIP(hipMemcpyAsync(x_io, x, n * sizeof(float), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
   TATVA_TRY_HIP(hipStreamWaitEvent(caller, res->ev_out, 0));
   return rc;

@@ -1,0 +1,446 @@
+"""Serving loop: continuous batching of mixed-size nonlinear solve requests.
+
+Requests (a cube mesh, a material, a load) arrive over time.  The server
+admits them while batch slots and the HBM budget allow, keeps each admitted
+request's mesh, constraints and Newton state resident on the GPU, and runs
+the inner FP32 Jacobi-PCG solves of all admitted requests together through a
+``tatva_batch``: every PCG iteration of every request in flight shares the
+same few kernel launches, whatever the request sizes.  A request that
+finishes its linear solve takes its FP64 Newton step (residual, update,
+convergence test, next load step) between batch launches and re-enters the
+batch with its next linear system; a request arriving meanwhile joins at the
+next launch.  The arithmetic of a request does not depend on what else is in
+the batch, so its result is the same served alone or in any mix.
+
+There is no counterpart in the reference; the request workload follows the
+mixed Hex8 linear-elastic / Neo-Hookean cube stream: n = round(exp(U(ln 8,
+ln 48))) cells per side, E ~ LogNormal(0, 0.5), nu ~ U(0.2, 0.45), materials
+50/50, Poisson arrivals at a fixed rate.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as nat
+from .element import Hexahedron8
+from .energies import LinearElasticity, NeoHookean
+from .lifter import Fixed, Lifter, RuntimeValue
+from .mesh import Mesh
+from .operator import Operator
+
+
+def traction_loads(coords, n, t=(1e-3, 0.0, 0.0)):
+    """Consistent nodal loads of a uniform traction on the x = 1 face of a
+    structured n^3 unit-cube Hex8 mesh: t h^2 {1/4 corner, 1/2 edge, 1 interior}."""
+    h = 1.0 / n
+    f = np.zeros(coords.shape[0] * 3)
+    face = np.where(np.isclose(coords[:, 0], 1.0))[0]
+    j = np.rint(coords[face, 1] / h).astype(int)
+    k = np.rint(coords[face, 2] / h).astype(int)
+    w = np.where((j == 0) | (j == n), 0.5, 1.0) * np.where((k == 0) | (k == n), 0.5, 1.0)
+    for c in range(3):
+        f[face * 3 + c] = t[c] * h * h * w
+    return f
+
+
+def lame(E: float, nu: float):
+    """(mu, lambda) from Young's modulus and Poisson's ratio."""
+    return E / (2.0 * (1.0 + nu)), E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+
+
+@dataclass
+class SolveRequest:
+    """One Hex8 cube solve: x = 0 clamped.  "linear_elastic": traction
+    ``traction`` on x = 1 (one step).  "neo_hookean": u_x prescribed on x = 1,
+    raised by ``stretch`` per load step over ``load_steps`` steps."""
+
+    rid: int
+    n: int
+    material: str
+    E: float
+    nu: float
+    arrival: float = 0.0
+    load_steps: int = 1
+    stretch: float = 0.03
+    traction: float = 1e-3
+
+    def __post_init__(self):
+        if self.material not in ("linear_elastic", "neo_hookean"):
+            raise ValueError("material must be 'linear_elastic' or 'neo_hookean'")
+        if self.n < 1 or self.load_steps < 1:
+            raise ValueError("n and load_steps must be >= 1")
+
+    @property
+    def n_elements(self) -> int:
+        return self.n ** 3
+
+    @property
+    def n_dofs(self) -> int:
+        return 3 * (self.n + 1) ** 3
+
+    def hbm_bytes(self) -> int:
+        """Upper bound of the device memory the request holds while admitted:
+        mesh (coords, connectivity, incidence, tiles, tile partials) and the
+        Newton/PCG vectors."""
+        E, N, d = self.n_elements, (self.n + 1) ** 3, self.n_dofs
+        mesh = 24 * N + 32 * E + 4 * (N + 1) + 32 * E + 4 * E + 16 * E + 8 * (N + 1) \
+            + 2 * 8 * E * 4 + 2 * 8 * E * 3 * 8
+        vectors = d * (8 + 8 + 4 + 4 + 8 + 4 + 4) + d * 4 * 8
+        return int(mesh + vectors + (1 << 20))
+
+
+def make_stream(count: int, qps: float, seed: int = 0, n_range=(8, 48),
+                nh_load_steps: int = 2, nh_stretch: float = 0.03):
+    """The mixed request stream: Poisson arrivals at rate qps, n log-uniform
+    in n_range, E ~ LogNormal(0, 0.5), nu ~ U(0.2, 0.45), materials 50/50."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    lo, hi = math.log(n_range[0]), math.log(n_range[1])
+    for i in range(count):
+        t += rng.exponential(1.0 / qps) if qps > 0 else 0.0
+        n = int(round(math.exp(rng.uniform(lo, hi))))
+        E = float(rng.lognormal(0.0, 0.5))
+        nu = float(rng.uniform(0.2, 0.45))
+        nh = bool(rng.uniform() < 0.5)
+        out.append(SolveRequest(rid=i, n=n, material="neo_hookean" if nh else "linear_elastic",
+                                E=E, nu=nu, arrival=t,
+                                load_steps=nh_load_steps if nh else 1, stretch=nh_stretch))
+    return out
+
+
+@dataclass
+class RequestResult:
+    rid: int
+    n: int
+    material: str
+    n_free: int
+    arrival: float
+    admitted: float
+    finished: float
+    newton_iterations: int = 0
+    cg_iterations: int = 0
+    load_steps: int = 0
+    converged: bool = False
+    rel_residual: float = float("nan")
+    u: object = None  # full FP64 solution on the device (kept if requested)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+    @property
+    def queue_wait(self) -> float:
+        return self.admitted - self.arrival
+
+
+class CubeProblem:
+    """The problem of a request: Hex8 unit cube of req.n cells per side, x = 0
+    clamped; linear elastic with the x = 1 traction as external force, or
+    Neo-Hookean with u_x on x = 1 prescribed per load step."""
+
+    def __init__(self, req: SolveRequest):
+        t = nat.torch()
+        self.req = req
+        n = req.n
+        mesh = Mesh.unit_cube(n)
+        self.mesh = mesh
+        self.op = Operator(mesh, Hexahedron8())
+        mu, lam = lame(req.E, req.nu)
+        x0 = np.where(np.isclose(mesh.coords[:, 0], 0.0))[0]
+        clamp = (x0[:, None] * 3 + np.arange(3)).ravel()
+        self.size = mesh.n_nodes * 3
+        self.ramp = None
+        if req.material == "linear_elastic":
+            f = traction_loads(mesh.coords, n, (req.traction, 0.0, 0.0))
+            self.energy = self.op.energy(LinearElasticity(mu, lam), external_force=f)
+            self.lifter0 = Lifter(self.size, Fixed(clamp, 0.0))
+            self._x1 = None
+        else:
+            self.energy = self.op.energy(NeoHookean(mu, lam))
+            x1 = np.where(np.isclose(mesh.coords[:, 0], 1.0))[0]
+            self._x1 = x1
+            self._n_clamp = clamp.size
+            self.lifter0 = Lifter(self.size, Fixed(np.concatenate([clamp, x1 * 3]),
+                                                   RuntimeValue("load")))
+            self.ramp = t.zeros(self.size, dtype=t.float64, device=nat.device())
+            self.ramp[0::3] = t.as_tensor(mesh.coords[:, 0], device=nat.device())
+
+    def lifter(self, step: int) -> Lifter:
+        """Constraints of load step ``step`` (1-based)."""
+        if self._x1 is None:
+            return self.lifter0
+        vals = np.concatenate([np.zeros(self._n_clamp),
+                               np.full(self._x1.size, self.req.stretch * step)])
+        return self.lifter0.at["load"].set(vals)
+
+    def predict(self, u) -> None:
+        """Start of a load step: homogeneous-stretch increment u_x += stretch * x."""
+        if self.ramp is not None:
+            u.add_(self.ramp, alpha=self.req.stretch)
+
+
+class _Job:
+    """A request resident on the GPU: mesh, constraints, FP64 state, the FP32
+    linear system of its current Newton step and its batch slot."""
+
+    def __init__(self, req: SolveRequest, slot: int, server: "Server", admitted: float):
+        t = nat.torch()
+        self.req = req
+        self.slot = slot
+        self.server = server
+        self.prob = CubeProblem(req)
+        self.op, self.energy = self.prob.op, self.prob.energy
+        self.lifter = self.prob.lifter(1)
+        self.red = self.energy.reduced(self.lifter)
+        self.n = self.red.size
+        dev = nat.device()
+        size = self.prob.size
+        self.u = t.zeros(size, dtype=t.float64, device=dev)
+        self.u32 = t.empty(size, dtype=t.float32, device=dev)
+        self.b32 = t.empty(self.n, dtype=t.float32, device=dev)
+        self.x32 = t.empty(self.n, dtype=t.float32, device=dev)
+        L = nat.lib()
+        nbytes = L.tatva_batch_slot_workspace(self.op.handle, 3, self.n)
+        if nbytes < 0:
+            nat.check(nat.EINVAL)
+        self.ws = t.empty(int(nbytes), dtype=t.uint8, device=dev)
+        self.result = RequestResult(rid=req.rid, n=req.n, material=req.material, n_free=self.n,
+                                    arrival=req.arrival, admitted=admitted, finished=float("nan"))
+        self.step = 0
+        self.r0 = None
+        self.newton_in_step = 0
+        self.done = False
+
+    # ------------------------------------------------------------ phases --
+
+    def _set_load_step(self, s: int):
+        self.step = s
+        self.r0 = None
+        self.newton_in_step = 0
+        self.prob.predict(self.u)
+        self.lifter = self.prob.lifter(s)
+        self.red.lifter = self.lifter
+        self.lifter.apply_constraints(self.u)
+
+    def begin(self):
+        self._set_load_step(1)
+        return self._newton()
+
+    def _newton(self) -> bool:
+        """FP64 residual and convergence test; starts the next linear solve in
+        the batch.  Returns True while the job still has work in the batch."""
+        srv = self.server
+        L = nat.lib()
+        while True:
+            r = self.red.residual_full_state(self.u)
+            rn = _norm(r)
+            if self.r0 is None:
+                self.r0 = rn
+            rel = rn / self.r0 if self.r0 > 0 else 0.0
+            self.result.rel_residual = rel
+            if not math.isfinite(rn) or self.newton_in_step >= srv.max_newton:
+                return self._finish(False)
+            if rn <= srv.tol * self.r0 or rn == 0.0:
+                if self.step < self.req.load_steps:
+                    self._set_load_step(self.step + 1)
+                    continue
+                return self._finish(True)
+            break
+        self.newton_in_step += 1
+        self.result.newton_iterations += 1
+        s = nat.stream()
+        nat.check(L.tatva_cast_f64_f32(self.u.numel(), 1.0, nat.dptr(self.u), nat.dptr(self.u32), s))
+        nat.check(L.tatva_cast_f64_f32(self.n, -1.0, nat.dptr(r), nat.dptr(self.b32), s))
+        dmap = self.lifter.dof_map_device()
+        diag = self.energy.hessian_diag(self.u32, dof_map=dmap, n_out=self.n,
+                                        dtype=nat.torch().float32)
+        self._diag = diag
+        self.x32.zero_()
+        nat.check(L.tatva_batch_start(srv._batch, self.slot, self.op.handle,
+                                      ctypes.byref(self.energy._desc), nat.dptr(self.u32),
+                                      nat.dptr(dmap), self.n, nat.dptr(diag), nat.dptr(self.b32),
+                                      nat.dptr(self.x32), float(srv.inner_tol),
+                                      int(srv.maxiter_factor * self.n), nat.dptr(self.ws),
+                                      self.ws.numel(), s))
+        return True
+
+    def on_linear_done(self, st) -> bool:
+        """Newton update u_free += x after the batch reports the solve done."""
+        self.result.cg_iterations += int(st.iterations)
+        if st.status == nat.EBREAKDOWN:
+            return self._finish(False)
+        L = nat.lib()
+        nat.check(L.tatva_axpy_free_f32(self.n, nat.dptr(self.lifter.free_device()), 1.0,
+                                        nat.dptr(self.x32), nat.dptr(self.u), nat.stream()))
+        return self._newton()
+
+    def _finish(self, ok: bool) -> bool:
+        self.done = True
+        self.result.converged = ok
+        self.result.load_steps = self.step
+        return False
+
+    def release(self):
+        if self.server.keep_solutions:
+            self.result.u = self.u
+        self.op.close()
+        self.ws = self.u32 = self.b32 = self.x32 = self._diag = None
+
+
+def _norm(x) -> float:
+    t = nat.torch()
+    out = t.empty((), dtype=t.float64, device=x.device)
+    nat.check(nat.lib().tatva_dot(x.numel(), nat.dptr(x), nat.dptr(x), nat.dptr(out), nat.stream()))
+    return math.sqrt(max(float(out.item()), 0.0))
+
+
+@dataclass
+class ServeReport:
+    results: list
+    seconds: float
+    batch_launches: int
+    peak_admitted: int
+    peak_hbm_bytes: int
+    dof_iterations: int = 0
+    extra: dict = field(default_factory=dict)
+
+    def latencies(self):
+        return np.array([r.latency for r in self.results])
+
+    def summary(self) -> dict:
+        lat = self.latencies()
+        done = [r for r in self.results if r.converged]
+        return {
+            "requests": len(self.results),
+            "converged": len(done),
+            "seconds": self.seconds,
+            "throughput_req_per_s": len(self.results) / self.seconds if self.seconds > 0 else 0.0,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else float("nan"),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else float("nan"),
+            "mean_queue_ms": float(np.mean([r.queue_wait for r in self.results]) * 1e3)
+            if self.results else float("nan"),
+            "dof_iters_per_s": self.dof_iterations / self.seconds if self.seconds > 0 else 0.0,
+            "batch_launches": self.batch_launches,
+            "peak_admitted": self.peak_admitted,
+            "peak_hbm_gb": self.peak_hbm_bytes / 1e9,
+        }
+
+
+class Server:
+    """Continuous-batching solve server on one GPU.
+
+    ``run(requests)`` replays the requests at their arrival times (seconds
+    from the start of the run) and returns a ServeReport.  ``max_slots``
+    bounds the requests in flight, ``hbm_budget`` their device memory
+    (SolveRequest.hbm_bytes); ``chunk`` PCG iterations run between polls.
+    """
+
+    def __init__(self, *, max_slots: int = 64, hbm_budget: float = 200e9, chunk: int = 8,
+                 tol: float = 1e-8, inner_tol: float = 1e-3, max_newton: int = 30,
+                 maxiter_factor: int = 10, keep_solutions: bool = False):
+        if max_slots < 1 or chunk < 1:
+            raise ValueError("max_slots and chunk must be >= 1")
+        self.max_slots = int(max_slots)
+        self.hbm_budget = float(hbm_budget)
+        self.chunk = int(chunk)
+        self.tol, self.inner_tol = float(tol), float(inner_tol)
+        self.max_newton = int(max_newton)
+        self.maxiter_factor = int(maxiter_factor)
+        self.keep_solutions = keep_solutions
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().tatva_batch_create(nat.device().index or 0, self.max_slots,
+                                               ctypes.byref(h)))
+        self._batch = h.value
+        self._stats = (nat.PcgStats * self.max_slots)()
+
+    def close(self):
+        if getattr(self, "_batch", None):
+            nat.lib().tatva_batch_destroy(self._batch)
+            self._batch = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def run(self, requests, *, realtime: bool = True) -> ServeReport:
+        t = nat.torch()
+        L = nat.lib()
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        for r in pending:
+            if r.hbm_bytes() > self.hbm_budget:
+                raise ValueError(f"request {r.rid} needs more than the HBM budget")
+        waiting: list[SolveRequest] = []
+        jobs: dict[int, _Job] = {}
+        free_slots = list(range(self.max_slots))[::-1]
+        used_bytes = 0
+        results = []
+        launches = 0
+        peak_adm = 0
+        peak_bytes = 0
+        dof_iters = 0
+        t.cuda.synchronize()
+        t0 = time.perf_counter()
+        clock = (lambda: time.perf_counter() - t0) if realtime else None
+        vclock = 0.0  # non-realtime: every request is present from the start
+        while pending or waiting or jobs:
+            now = clock() if realtime else vclock
+            while pending and (pending[0].arrival <= now or not realtime):
+                waiting.append(pending.pop(0))
+            # admission: FIFO, slots and HBM budget
+            while waiting and free_slots and \
+                    used_bytes + waiting[0].hbm_bytes() <= self.hbm_budget:
+                req = waiting.pop(0)
+                slot = free_slots.pop()
+                job = _Job(req, slot, self, time.perf_counter() - t0)
+                used_bytes += req.hbm_bytes()
+                jobs[slot] = job
+                if not job.begin():
+                    self._retire(job, jobs, free_slots, results, t0)
+                    used_bytes -= req.hbm_bytes()
+            peak_adm = max(peak_adm, len(jobs))
+            peak_bytes = max(peak_bytes, used_bytes)
+            if not jobs:
+                if pending and realtime:
+                    time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
+                continue
+            nat.check(L.tatva_batch_iterate(self._batch, self.chunk, nat.stream()))
+            launches += 1
+            running = ctypes.c_int32(0)
+            nat.check(L.tatva_batch_poll(self._batch, self._stats, ctypes.byref(running),
+                                         nat.stream()))
+            for slot in list(jobs):
+                st = self._stats[slot]
+                if not st.reserved:  # still iterating
+                    continue
+                job = jobs[slot]
+                dof_iters += job.n * int(st.iterations)
+                if not job.on_linear_done(st):
+                    used_bytes -= job.req.hbm_bytes()
+                    self._retire(job, jobs, free_slots, results, t0)
+        t.cuda.synchronize()
+        seconds = time.perf_counter() - t0
+        results.sort(key=lambda r: r.rid)
+        return ServeReport(results=results, seconds=seconds, batch_launches=launches,
+                           peak_admitted=peak_adm, peak_hbm_bytes=peak_bytes,
+                           dof_iterations=dof_iters)
+
+    def _retire(self, job, jobs, free_slots, results, t0):
+        nat.check(nat.lib().tatva_batch_release(self._batch, job.slot))
+        job.result.finished = time.perf_counter() - t0
+        job.release()
+        results.append(job.result)
+        del jobs[job.slot]
+        free_slots.append(job.slot)
+
+
+__all__ = ["SolveRequest", "RequestResult", "ServeReport", "Server", "CubeProblem", "make_stream",
+           "traction_loads", "lame"]

@@ -187,6 +187,34 @@ def test_energy_residual_hvp_parity(kind, density):
     assert w.shape == tuple(psi.shape[:2])
 
 
+@pytest.mark.parametrize("kind,n", [("tri3", 16), ("tet4", 8), ("hex8", 9)])
+def test_assembly_over_many_tiles(kind, n):
+    """Residual, HVP and diagonal on meshes of several 256-element tiles (nodes
+    shared across tiles are summed by the slot gather) against the oracle."""
+    density = energies.NeoHookean(1.0, 10.0)
+    op, E, prob = pair(kind, density, n=n, f_ext=True)
+    assert op.mesh.elements.shape[0] > 2 * 256
+    rng = np.random.default_rng(21)
+    # Neo-Hookean needs det F > 0: the perturbation scales with the cell size
+    u = 0.04 / n * rng.standard_normal(prob.n)
+    v = rng.standard_normal(prob.n)
+    assert np.isfinite(prob.residual(u)).all()
+    assert rel_err(to_np(E.grad(u)), prob.residual(u)) < RTOL
+    assert rel_err(to_np(E.hvp(u, v)), prob.hvp(u, v)) < RTOL
+    assert rel_err(to_np(E.hessian_diag(u)), prob.hessian_diag(u)) < RTOL
+    t = __import__("torch")
+    u32 = t.tensor(u, dtype=t.float32, device="cuda")
+    v32 = t.tensor(v, dtype=t.float32, device="cuda")
+    assert rel_err(to_np(E.hvp(u32, v32)), prob.hvp(u, v)) < 2e-5
+    # integrals: per element and total only (no per-element buffer)
+    vals = rng.standard_normal((op.mesh.coords.shape[0], 5))
+    pe = to_np(op.integrate_per_element(vals))
+    tot = to_np(op.integrate(vals))
+    ref = fem.integrate_per_element(prob.elem, prob.coords, prob.elements, vals)
+    assert rel_err(pe, ref) < RTOL
+    assert rel_err(tot, ref.sum(axis=0)) < RTOL
+
+
 def test_residual_and_hessian_are_deterministic():
     op, E, prob = pair("hex8", energies.NeoHookean(1.0, 10.0), n=6)
     u = 0.01 * np.random.default_rng(4).standard_normal(prob.n)

@@ -1,0 +1,133 @@
+"""Serving loop: request stream (CPU) and continuous batching on the GPU.
+
+GPU checks: every served result against an independent solve of the same
+request by the single-request mixed Newton (FP32 matrix-free PCG through
+tatva_pcg_matfree_f32, graph-captured) and the FP64 colored-CSR Newton, and
+the same request served alone and in a mix (bit-identical: a slot's
+arithmetic does not depend on the other slots).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from tatva_amd import serve
+
+
+def test_stream_is_seeded_and_follows_the_distributions():
+    a = serve.make_stream(400, qps=50.0, seed=0)
+    b = serve.make_stream(400, qps=50.0, seed=0)
+    assert [(r.n, r.material, r.E, r.nu, r.arrival) for r in a] == \
+        [(r.n, r.material, r.E, r.nu, r.arrival) for r in b]
+    ns = np.array([r.n for r in a])
+    assert ns.min() >= 8 and ns.max() <= 48
+    # log-uniform: the median of n is near sqrt(8 * 48) ~ 19.6
+    assert 15 <= np.median(ns) <= 25
+    nus = np.array([r.nu for r in a])
+    assert nus.min() >= 0.2 and nus.max() <= 0.45
+    logE = np.log([r.E for r in a])
+    assert abs(logE.mean()) < 0.1 and abs(logE.std() - 0.5) < 0.06
+    frac_nh = np.mean([r.material == "neo_hookean" for r in a])
+    assert 0.4 < frac_nh < 0.6
+    arr = np.array([r.arrival for r in a])
+    assert np.all(np.diff(arr) > 0)
+    # exponential gaps with mean 1/qps
+    assert abs(np.diff(arr).mean() - 1 / 50.0) < 0.005
+    assert all(r.load_steps == (2 if r.material == "neo_hookean" else 1) for r in a)
+
+
+def test_request_validation_and_memory_estimate():
+    with pytest.raises(ValueError):
+        serve.SolveRequest(rid=0, n=4, material="steel", E=1.0, nu=0.3)
+    with pytest.raises(ValueError):
+        serve.SolveRequest(rid=0, n=0, material="linear_elastic", E=1.0, nu=0.3)
+    small = serve.SolveRequest(rid=0, n=8, material="linear_elastic", E=1.0, nu=0.3)
+    big = serve.SolveRequest(rid=1, n=48, material="neo_hookean", E=1.0, nu=0.3)
+    assert small.n_dofs == 3 * 9 ** 3 and big.n_elements == 48 ** 3
+    assert 0 < small.hbm_bytes() < big.hbm_bytes() < 1e9
+    mu, lam = serve.lame(1.0, 0.3)
+    assert math.isclose(mu, 1 / 2.6) and math.isclose(lam, 0.3 / (1.3 * 0.4))
+
+
+def test_traction_loads_sum_to_force():
+    import tatva_amd as ta
+    m = ta.Mesh.unit_cube(5)
+    f = serve.traction_loads(m.coords, 5, (2e-3, 0.0, 0.0))
+    assert math.isclose(f[0::3].sum(), 2e-3)       # t * area(face) = t
+    assert np.all(f[1::3] == 0) and np.all(f[2::3] == 0)
+
+
+# ----------------------------------------------------------------- GPU ----
+
+def _mix(seed=1):
+    reqs = serve.make_stream(6, qps=0.0, seed=seed, n_range=(3, 7))
+    # one of each material at least
+    reqs[0].material, reqs[0].load_steps = "linear_elastic", 1
+    reqs[1].material, reqs[1].load_steps = "neo_hookean", 2
+    return reqs
+
+
+def _reference_solve(req, mode):
+    """The same request through the single-request Newton solver."""
+    import torch
+
+    from tatva_amd import solvers
+
+    P = serve.CubeProblem(req)
+    kw = dict(lin_tol=1e-10) if mode == "csr" else dict(inner_tol=1e-3)
+    solver = solvers.Newton(P.energy, P.lifter(1), mode=mode, tol=1e-8, chunk=8, **kw)
+    u = torch.zeros(P.size, dtype=torch.float64, device="cuda")
+    stats = []
+    for s in range(1, req.load_steps + 1):
+        P.predict(u)
+        stats.append(solver.solve(u, P.lifter(s)))
+    return u, stats
+
+
+@pytest.mark.gpu
+def test_served_results_match_single_request_solvers():
+    reqs = _mix()
+    srv = serve.Server(max_slots=4, chunk=4, keep_solutions=True)
+    rep = srv.run(reqs, realtime=False)
+    srv.close()
+    assert len(rep.results) == len(reqs)
+    assert all(r.converged for r in rep.results), [(r.rid, r.rel_residual) for r in rep.results]
+    assert rep.peak_admitted == 4     # slots were reused as requests finished
+    for r, req in zip(rep.results, reqs):
+        assert r.rel_residual <= 1e-8
+        u_csr, st = _reference_solve(req, "csr")
+        assert all(s.converged for s in st)
+        a, b = r.u.cpu().numpy(), u_csr.cpu().numpy()
+        # both Newton solves stop at ||r|| <= 1e-8 ||r0||; the states agree to
+        # the conditioning of the problem times that (tolerance 1e-6 relative)
+        assert np.max(np.abs(a - b)) <= 1e-6 * np.max(np.abs(b)), r.rid
+
+
+@pytest.mark.gpu
+def test_result_is_identical_alone_and_in_the_mix():
+    reqs = _mix(seed=2)
+    srv = serve.Server(max_slots=8, chunk=3, keep_solutions=True)
+    mixed = srv.run(reqs, realtime=False)
+    for req in reqs[:3]:
+        alone = srv.run([req], realtime=False)
+        a = alone.results[0]
+        m = mixed.results[req.rid]
+        assert a.converged and m.converged
+        assert a.newton_iterations == m.newton_iterations
+        assert a.cg_iterations == m.cg_iterations
+        # bit-identical (the stated tolerance is exact equality)
+        assert np.array_equal(a.u.cpu().numpy(), m.u.cpu().numpy()), req.rid
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_realtime_stream_reports_latency():
+    reqs = serve.make_stream(8, qps=40.0, seed=3, n_range=(4, 10))
+    srv = serve.Server(max_slots=4, chunk=8)
+    rep = srv.run(reqs, realtime=True)
+    srv.close()
+    s = rep.summary()
+    assert s["requests"] == 8 and s["converged"] == 8
+    assert 0 < s["p50_ms"] <= s["p99_ms"]
+    assert all(r.finished >= r.admitted >= r.arrival - 1e-3 for r in rep.results)
