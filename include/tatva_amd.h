@@ -158,6 +158,19 @@ int tatva_colored_hessian(tatva_mesh *mesh, const tatva_density *density, const 
                           const int32_t *indices, int64_t nnz, double *data, void *workspace,
                           int64_t workspace_bytes, void *stream);
 
+/* The same for a general lifter (Fixed and Periodic, lifter/base.py:201-251,
+ * constraints.py:184-220): the reduced Hessian P^T H P with P the linear part of
+ * the lift.  read_map (full dofs) gives each full dof's reduced column (or -1);
+ * reduced row r sums the full rows row_list[row_ptr[r] .. row_ptr[r+1]).  With
+ * flags & 1 every element's columns are accumulated one at a time (needed when an
+ * element holds two dofs of one reduced column, e.g. a slave and its master). */
+int tatva_colored_hessian_mapped(tatva_mesh *mesh, const tatva_density *density,
+                                 const double *u, const int32_t *read_map,
+                                 const int32_t *row_ptr, const int32_t *row_list, int64_t n_rows,
+                                 const int32_t *colors, int32_t n_colors, const int32_t *indptr,
+                                 const int32_t *indices, int64_t nnz, int32_t flags, double *data,
+                                 void *workspace, int64_t workspace_bytes, void *stream);
+
 /* rows[k] = row of entry k and col_colors[k] = colors[indices[k]] for a CSR
  * pattern (compute_rows_cols, tatva/sparse/base.py:108-136). */
 int tatva_rows_cols(int64_t n, const int32_t *indptr, const int32_t *indices,
@@ -245,6 +258,18 @@ int tatva_set_dofs(int64_t n, const int32_t *dofs, const double *values, double 
 /* out_red[i] = in_full[free[i]] (Lifter.reduce, lifter/base.py:231-233). */
 int tatva_gather_free(int64_t n_red, const int32_t *free_dofs, const double *in_full,
                       double *out_red, void *stream);
+/* Linear part of a lift through a read map: out_full[d] = in_red[map[d]], or 0
+ * where map[d] < 0.  With Fixed constraints only map is the free-dof index map;
+ * a Periodic slave maps to its master's reduced index (Lifter.lift with the
+ * constraint values at zero, lifter/base.py:201-221, constraints.py:214-216). */
+int tatva_lift_map(int64_t n_full, const int32_t *map, const double *in_red, double *out_full,
+                   void *stream);
+/* out_red[j] = sum of in_full[list[k]], k in [ptr[j], ptr[j+1]) in ascending
+ * order: the transpose of the lift (Lifter.reduce_adjoint, lifter/base.py:235-251,
+ * Periodic.apply_transpose constraints.py:218-220); list holds every full dof
+ * whose read map is j. */
+int tatva_reduce_rows(int64_t n_red, const int32_t *ptr, const int32_t *list,
+                      const double *in_full, double *out_red, void *stream);
 /* u_full[free[i]] += alpha * du[i] with du FP32 or FP64. */
 int tatva_axpy_free_f32(int64_t n_red, const int32_t *free_dofs, double alpha, const float *du,
                         double *u_full, void *stream);
@@ -267,6 +292,13 @@ typedef struct tatva_csr_host tatva_csr_host;
 int tatva_pattern_from_mesh(const int32_t *elements, int64_t n_elements, int nne,
                             int64_t n_nodes, int ndpn, const int32_t *dof_map, int64_t n_rows,
                             tatva_csr_host **out);
+/* CSR pattern of general element dof tables (the core of pattern_from_compound,
+ * tatva/sparse/_extraction.py:125-245): every pair of valid (>= 0) dofs of one
+ * element row of element_dofs (E x k, -1 = no dof), plus (d, d) for each
+ * diag_dofs entry; deduplicated, columns sorted. */
+int tatva_pattern_from_element_dofs(const int32_t *element_dofs, int64_t n_elements, int k,
+                                    int64_t size, const int32_t *diag_dofs, int64_t n_diag,
+                                    tatva_csr_host **out);
 /* Row/column submatrix S[keep][:, keep] of a CSR pattern; keep_map: old index ->
  * new index or -1, monotone on kept indices (Lifter.reduce_sparsity,
  * lifter/base.py:322-331). */

@@ -498,6 +498,18 @@ class Lifter:
     def lift_from_zeros(self, u_red):
         return self.lift(u_red, np.zeros(self.size))
 
+    def lift_linear(self, v_red):
+        """Derivative of lift w.r.t. u_red applied to v_red (Fixed values held
+        constant, so they contribute zero)."""
+        v = np.zeros(self.size)
+        v[self.free] = v_red
+        for c in self.constraints:
+            if isinstance(c, Fixed):
+                v[c.dofs] = 0.0
+            else:
+                v = c.lift(v)
+        return v
+
     def reduce(self, u):
         return np.asarray(u)[self.free]
 
@@ -516,6 +528,30 @@ class Lifter:
 # --------------------------------------------------------------------------
 # Distance-2 coloring (sparse/_coloring.py)
 # --------------------------------------------------------------------------
+
+
+def pattern_from_element_dofs(element_dofs, size, diagonal_dofs=None):
+    """pattern_from_compound's builder (sparse/_extraction.py:198-238): element
+    dof pairs (repeat/tile), -1 entries filtered, diagonal dofs added,
+    deduplicated through linearised indices."""
+    ed = np.asarray(element_dofs, dtype=np.int64)
+    rows_l, cols_l = [], []
+    if ed.size:
+        k = ed.shape[1]
+        r = np.repeat(ed, k, axis=1).ravel()
+        c = np.tile(ed, (1, k)).ravel()
+        ok = (r >= 0) & (c >= 0)
+        rows_l.append(r[ok])
+        cols_l.append(c[ok])
+    if diagonal_dofs is not None and np.size(diagonal_dofs):
+        d = np.asarray(diagonal_dofs, dtype=np.int64).ravel()
+        rows_l.append(d)
+        cols_l.append(d)
+    if not rows_l:
+        return sps.csr_matrix((size, size), dtype=np.int8)
+    lin = np.unique(np.concatenate(rows_l) * size + np.concatenate(cols_l))
+    return sps.csr_matrix((np.ones(lin.size, dtype=np.int8), (lin // size, lin % size)),
+                          shape=(size, size))
 
 
 def distance2_adjacency(n, indptr, indices):
@@ -640,9 +676,7 @@ def pcg(A, b, x0=None, tol=1e-10, maxiter=100000, diag=None):
 def hessian_csr(prob: Problem, lifter: Lifter, u_full, pattern, colors):
     """Colored sparse Hessian of E_free(u) = E(lift(u)) on the reduced pattern."""
     def jvp(seed):
-        v = np.zeros(prob.n)
-        v[lifter.free] = seed
-        return lifter.reduce_adjoint(prob.hvp(u_full, v))
+        return lifter.reduce_adjoint(prob.hvp(u_full, lifter.lift_linear(seed)))
     data = colored_jacfwd(jvp, pattern.indptr, pattern.indices, colors)
     return sps.csr_matrix((data, pattern.indices, pattern.indptr), shape=pattern.shape)
 
@@ -668,7 +702,7 @@ def newton(prob: Problem, lifter: Lifter, u_full, pattern, colors, tol=1e-10,
         K = hessian_csr(prob, lifter, u, pattern, colors)
         du, k, _ = pcg(K, -r, tol=lin_tol, maxiter=maxiter)
         cg_total += k
-        u[lifter.free] += du
+        u = lifter.lift(lifter.reduce(u) + du, u)
     r = lifter.reduce_adjoint(prob.residual(u))
     hist.append(np.linalg.norm(r))
     return u, max_newton, cg_total, hist

@@ -63,6 +63,9 @@ _SIGS = {
     "tatva_colored_hessian": (c_int, [c_void, ctypes.POINTER(Density), c_void, c_void, c_void, c_i64,
                                       c_void, c_i32, c_void, c_void, c_i64, c_void, c_void, c_i64,
                                       c_void]),
+    "tatva_colored_hessian_mapped": (c_int, [c_void, ctypes.POINTER(Density), c_void, c_void, c_void,
+                                             c_void, c_i64, c_void, c_i32, c_void, c_void, c_i64,
+                                             c_i32, c_void, c_void, c_i64, c_void]),
     "tatva_rows_cols": (c_int, [c_i64, c_void, c_void, c_void, c_void, c_void, c_void]),
     "tatva_spmv_csr": (c_int, [c_i64, c_i64, c_void, c_void, c_void, c_void, c_void, c_void]),
     "tatva_pcg_csr_workspace": (c_i64, [c_i64]),
@@ -88,6 +91,8 @@ _SIGS = {
     "tatva_scatter_free": (c_int, [c_i64, c_void, c_void, c_void, c_void]),
     "tatva_set_dofs": (c_int, [c_i64, c_void, c_void, c_dbl, c_void, c_void]),
     "tatva_gather_free": (c_int, [c_i64, c_void, c_void, c_void, c_void]),
+    "tatva_lift_map": (c_int, [c_i64, c_void, c_void, c_void, c_void]),
+    "tatva_reduce_rows": (c_int, [c_i64, c_void, c_void, c_void, c_void, c_void]),
     "tatva_axpy_free_f32": (c_int, [c_i64, c_void, c_dbl, c_void, c_void, c_void]),
     "tatva_axpy_free": (c_int, [c_i64, c_void, c_dbl, c_void, c_void, c_void]),
     "tatva_cast_f64_f32": (c_int, [c_i64, c_dbl, c_void, c_void, c_void]),
@@ -95,6 +100,8 @@ _SIGS = {
     "tatva_dot": (c_int, [c_i64, c_void, c_void, c_void, c_void]),
     "tatva_pattern_from_mesh": (c_int, [c_void, c_i64, c_int, c_i64, c_int, c_void, c_i64,
                                         ctypes.POINTER(c_void)]),
+    "tatva_pattern_from_element_dofs": (c_int, [c_void, c_i64, c_int, c_i64, c_void, c_i64,
+                                                ctypes.POINTER(c_void)]),
     "tatva_csr_submatrix": (c_int, [c_i64, c_void, c_void, c_void, c_i64, ctypes.POINTER(c_void)]),
     "tatva_csr_host_rows": (c_i64, [c_void]),
     "tatva_csr_host_nnz": (c_i64, [c_void]),

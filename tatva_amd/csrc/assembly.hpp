@@ -184,15 +184,221 @@ __device__ __forceinline__ void elem_diag(const double (&X)[Elem<KIND>::NNE][Ele
   }
 }
 
-// ------------------------------------------------------ tile assembly ------
-
 enum { kModeResidual = 0, kModeHvp = 1, kModeDiag = 2 };
+
+// ------------------------------------------------- Hex8 in modal form -----
+//
+// A trilinear Hex8 field f = sum_a N_a f_a equals (1/8) sum_m M_m psi_m(xi)
+// with psi = {1, xi, eta, xi eta, zeta, xi zeta, eta zeta, xi eta zeta}
+// (mode index b = i + 2j + 4k over the node sign pattern) and M = H f, H the
+// 8x8 matrix of node sign products H[m][a] = prod_{d in m} s_d(a)
+// (element/base.py:478-491,535); node values from modes use H^T.
+// A natural derivative at a point then costs 3 FMAs instead of 8, and the
+// element output sum_q B^T S_q is one butterfly of 7 accumulated modes.  An
+// element whose bilinear coordinate modes are all zero (a parallelepiped,
+// e.g. every cell of an axis-aligned structured grid; the butterfly along x
+// first makes them exactly zero there) has a constant Jacobian, inverted once
+// per element.  The results equal the nodal formulas up to rounding.
+
+// Hex8 local node -> sign index i + 2j + 4k (sx, sy, sz > 0 -> i, j, k = 1).
+__host__ __device__ constexpr int hex_bin(int a) {
+  return a == 2 ? 3 : a == 3 ? 2 : a == 6 ? 7 : a == 7 ? 6 : a;
+}
+
+// In-place sign-product transform v <- H v (unnormalised), stages x, y, z.
+template <class S>
+__device__ __forceinline__ void hadamard8(S (&v)[8]) {
+#pragma unroll
+  for (int h = 1; h < 8; h <<= 1)
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      if (!(b & h)) {
+        const S lo = v[b], hi = v[b | h];
+        v[b] = hi + lo;
+        v[b | h] = hi - lo;
+      }
+}
+
+// Transpose of hadamard8 (node value from modes: lo - hi at the minus node,
+// lo + hi at the plus node, per direction).
+template <class S>
+__device__ __forceinline__ void hadamard8_t(S (&v)[8]) {
+#pragma unroll
+  for (int h = 1; h < 8; h <<= 1)
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      if (!(b & h)) {
+        const S lo = v[b], hi = v[b | h];
+        v[b] = lo - hi;
+        v[b | h] = lo + hi;
+      }
+}
+
+// M[c][b] = (1/8) (H f_c)[b]
+template <class S, int NC>
+__device__ __forceinline__ void hex_modes(const S (&f)[8][NC], S (&M)[NC][8]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    S v[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) v[hex_bin(a)] = f[a][c];
+    hadamard8(v);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) M[c][b] = v[b] * S(0.125);
+  }
+}
+
+// d[k] = d f / d xi_k at (x, y, z) for a modal field M (scaled by 1/8)
+template <class S>
+__device__ __forceinline__ void hex_dmodes(const S (&M)[8], S x, S y, S z, S (&d)[3]) {
+  d[0] = M[1] + M[3] * y + M[5] * z + M[7] * (y * z);
+  d[1] = M[2] + M[3] * x + M[6] * z + M[7] * (x * z);
+  d[2] = M[4] + M[5] * x + M[6] * y + M[7] * (x * y);
+}
+
+// Hex8 residual (MODE = kModeResidual: S = P(G), plus the Poisson source) or
+// Hessian-vector product (S = dP(G(u); G(v))): ye = sum_q w_q B_q^T S_q.
+// Geometry in FP64, physics in T.
+template <class Dens, class T, int MODE, bool AFFINE>
+__device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&ue)[8][Dens::NC],
+                                             const T (&ve)[8][Dens::NC], const QuadRule &quad,
+                                             const Dens &dens, T (&ye)[8][Dens::NC]) {
+  constexpr int NC = Dens::NC;
+  double XM[3][8];
+  hex_modes<double, 3>(X, XM);
+  T UM[NC][8], VM[NC][8];
+  hex_modes<T, NC>(ue, UM);
+  hex_modes<T, NC>(ve, VM);
+  T A[NC][8];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) A[c][b] = T(0);
+  double inv0[3][3] = {}, det0 = 0.0;
+  if constexpr (AFFINE) {
+    double J[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      J[0][j] = XM[j][1];
+      J[1][j] = XM[j][2];
+      J[2][j] = XM[j][4];
+    }
+    det0 = det_inv(J, inv0);
+  }
+  for (int q = 0; q < quad.nq; ++q) {
+    const double x = quad.xi[q][0], y = quad.xi[q][1], z = quad.xi[q][2];
+    double inv[3][3], det;
+    if constexpr (AFFINE) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) inv[i][j] = inv0[i][j];
+      det = det0;
+    } else {
+      double J[3][3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double d[3];
+        hex_dmodes<double>(XM[j], x, y, z, d);
+        J[0][j] = d[0];
+        J[1][j] = d[1];
+        J[2][j] = d[2];
+      }
+      det = det_inv(J, inv);
+    }
+    const T w = T(det * quad.w[q]);
+    T iv[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) iv[i][j] = T(inv[i][j]);
+    const T xt = T(x), yt = T(y), zt = T(z);
+    // G[c][j] = sum_k H[c][k] inv[j][k], H = natural derivatives
+    T G[NC][3], S[NC][3];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      T h[3];
+      hex_dmodes<T>(UM[c], xt, yt, zt, h);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) G[c][j] = h[0] * iv[j][0] + h[1] * iv[j][1] + h[2] * iv[j][2];
+    }
+    if constexpr (MODE == kModeResidual) {
+      dens.P(G, S);
+    } else {
+      T dG[NC][3];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        T h[3];
+        hex_dmodes<T>(VM[c], xt, yt, zt, h);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dG[c][j] = h[0] * iv[j][0] + h[1] * iv[j][1] + h[2] * iv[j][2];
+      }
+      const auto L = dens.linearize(G);
+      dens.dP(L, dG, S);
+    }
+    // Q[c][k] = w sum_j S[c][j] inv[j][k], accumulated into the derivative modes
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      T Q[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) Q[k] = w * (S[c][0] * iv[0][k] + S[c][1] * iv[1][k] + S[c][2] * iv[2][k]);
+      A[c][1] += Q[0];
+      A[c][2] += Q[1];
+      A[c][4] += Q[2];
+      A[c][3] += Q[0] * yt + Q[1] * xt;
+      A[c][5] += Q[0] * zt + Q[2] * xt;
+      A[c][6] += Q[1] * zt + Q[2] * yt;
+      A[c][7] += Q[0] * (yt * zt) + Q[1] * (xt * zt) + Q[2] * (xt * yt);
+    }
+    if constexpr (MODE == kModeResidual && Dens::kSource) {
+      // - w f N_a with N_a = (1/8) sum_m phi_m(a) psi_m
+      const T s = w * dens.f;
+      A[0][0] -= s;
+      A[0][1] -= s * xt;
+      A[0][2] -= s * yt;
+      A[0][3] -= s * (xt * yt);
+      A[0][4] -= s * zt;
+      A[0][5] -= s * (xt * zt);
+      A[0][6] -= s * (yt * zt);
+      A[0][7] -= s * (xt * yt * zt);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    T v[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v[b] = A[c][b];
+    hadamard8_t(v);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) ye[a][c] = v[hex_bin(a)] * T(0.125);
+  }
+}
+
+// True when the element's bilinear coordinate modes are all exactly zero
+// (computed with the same butterfly as hex_modes).
+inline bool hex_is_affine_host(const double (&X)[8][3]) {
+  for (int j = 0; j < 3; ++j) {
+    double v[8];
+    for (int a = 0; a < 8; ++a) v[hex_bin(a)] = X[a][j];
+    for (int h = 1; h < 8; h <<= 1)
+      for (int b = 0; b < 8; ++b)
+        if (!(b & h)) {
+          const double lo = v[b], hi = v[b | h];
+          v[b] = hi + lo;
+          v[b | h] = hi - lo;
+        }
+    if (v[3] != 0.0 || v[5] != 0.0 || v[6] != 0.0 || v[7] != 0.0) return false;
+  }
+  return true;
+}
+
+// ------------------------------------------------------ tile assembly ------
 
 // Body of one tile (one workgroup of kTile threads; acc = LDS of
 // max_local * NC values).  u: full state; v (HVP): tangent read through
 // dof_map (reduced, zero on constrained dofs).  Writes the tile's node sums to
 // partial[(node_ptr[tile] + j) * NC + c].
-template <int KIND, class Dens, class T, int MODE>
+template <int KIND, class Dens, class T, int MODE, bool AFFINE>
 __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict__ coords,
                                               const int32_t *__restrict__ conn,
                                               const QuadRule &quad, const Dens &dens,
@@ -221,7 +427,9 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     for (int a = 0; a < NNE; ++a)
 #pragma unroll
       for (int c = 0; c < NC; ++c) ue[a][c] = u[(int64_t)nodes[a] * NC + c];
-    if constexpr (MODE == kModeResidual) {
+    if constexpr (KIND == TATVA_HEX8 && MODE == kModeResidual) {
+      hex8_element<Dens, T, kModeResidual, AFFINE>(X, ue, ue, quad, dens, ye);
+    } else if constexpr (MODE == kModeResidual) {
       elem_residual<KIND, Dens>(X, ue, quad, dens, ye);
     } else if constexpr (MODE == kModeHvp) {
       T ve[NNE][NC];
@@ -237,7 +445,8 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
             ve[a][c] = v[dof];
           }
         }
-      elem_hvp<KIND, Dens, T>(X, ue, ve, quad, dens, ye);
+      if constexpr (KIND == TATVA_HEX8) hex8_element<Dens, T, kModeHvp, AFFINE>(X, ue, ve, quad, dens, ye);
+      else elem_hvp<KIND, Dens, T>(X, ue, ve, quad, dens, ye);
     } else {
       elem_diag<KIND, Dens, T>(X, ue, quad, dens, ye);
     }
@@ -262,7 +471,7 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
   for (int i = threadIdx.x; i < nloc * NC; i += kTile) out[i] = acc[i];
 }
 
-template <int KIND, class Dens, class T, int MODE>
+template <int KIND, class Dens, class T, int MODE, bool AFFINE>
 __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restrict__ coords,
                                                          const int32_t *__restrict__ conn,
                                                          QuadRule quad, Dens dens,
@@ -273,7 +482,7 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
                                                          const int *__restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ unsigned char lds_raw[];
-  tile_assemble<KIND, Dens, T, MODE>((int)xcd_block(blockIdx.x, gridDim.x), coords, conn, quad,
+  tile_assemble<KIND, Dens, T, MODE, AFFINE>((int)xcd_block(blockIdx.x, gridDim.x), coords, conn, quad,
                                      dens, u, v, dof_map, ts, partial,
                                      reinterpret_cast<T *>(lds_raw));
 }
@@ -319,6 +528,38 @@ __global__ __launch_bounds__(kBlock) void k_slot_gather(int64_t n_nodes, const i
       out[dof] = val;
     }
   }
+}
+
+// As k_slot_gather without f_ext, grid-stride over nodes, plus the block's
+// share of w . out (w and out reduced through dof_map): the PCG operator step
+// y = H p with p.Hp in one pass.
+template <int NC, class T>
+__global__ __launch_bounds__(kBlock) void k_slot_gather_dot(int64_t n_nodes, const int32_t *__restrict__ slot_ptr,
+                                                            const int32_t *__restrict__ slot_list,
+                                                            const T *__restrict__ partial,
+                                                            const int32_t *__restrict__ dof_map,
+                                                            const T *__restrict__ w,
+                                                            T *__restrict__ out,
+                                                            double *__restrict__ part_dot,
+                                                            const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  double dot = 0.0;
+  for (int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n_nodes;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T acc[NC];
+    slot_sum<NC, T>(i, slot_ptr, slot_list, partial, acc);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int64_t dof = i * NC + c;
+      const int64_t r = dof_map ? (int64_t)dof_map[dof] : dof;
+      if (r >= 0) {
+        out[r] = acc[c];
+        dot += (double)acc[c] * (double)w[r];
+      }
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0) part_dot[blockIdx.x] = dot;
 }
 
 }  // namespace tatva

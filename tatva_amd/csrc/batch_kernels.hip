@@ -64,7 +64,7 @@ __device__ __forceinline__ Dens make_density(const tatva_density &d) {
   else return Dens{float(d.p0), float(d.p1)};
 }
 
-template <int KIND, class Dens>
+template <int KIND, class Dens, bool AFFINE>
 __global__ __launch_bounds__(kTile) void k_batch_assemble(const BatchSlot *__restrict__ slots,
                                                           const int32_t *__restrict__ bmap,
                                                           QuadRule quad) {
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kTile) void k_batch_assemble(const BatchSlot *__res
   if (S.st->done) return;
   extern __shared__ unsigned char lds_raw[];
   const Dens dens = make_density<Dens>(S.dens);
-  tile_assemble<KIND, Dens, float, kModeHvp>(b - S.a_blk0, S.coords, S.conn, quad, dens, S.u, S.p,
+  tile_assemble<KIND, Dens, float, kModeHvp, AFFINE>(b - S.a_blk0, S.coords, S.conn, quad, dens, S.u, S.p,
                                              S.dof_map, S.tiles, S.partial,
                                              reinterpret_cast<float *>(lds_raw));
 }
@@ -222,13 +222,13 @@ using namespace tatva;
 struct tatva_batch {
   struct Meta {
     bool active = false;
-    int kind = 0, dkind = 0, ncomp = 0;
+    int kind = 0, dkind = 0, ncomp = 0, affine = 0;
     QuadRule quad{};
     size_t lds = 0;
     tatva_mesh *mesh = nullptr;
   };
   struct Launch {
-    int kind, dkind, ncomp;
+    int kind, dkind, ncomp, affine;
     QuadRule quad;
     size_t lds;
     int a_off, a_n, g_off, g_n;
@@ -282,16 +282,19 @@ int batch_upload(tatva_batch *B, hipStream_t s) {
     if (!mi.active) continue;
     bool found = false;
     for (auto &L : B->launches)
-      if (L.kind == mi.kind && L.dkind == mi.dkind && same_quad(L.quad, mi.quad)) found = true;
+      if (L.kind == mi.kind && L.dkind == mi.dkind && L.affine == mi.affine &&
+          same_quad(L.quad, mi.quad))
+        found = true;
     if (found) continue;
-    tatva_batch::Launch L{mi.kind, mi.dkind, mi.ncomp, mi.quad, 0, 0, 0, 0, 0};
+    tatva_batch::Launch L{mi.kind, mi.dkind, mi.ncomp, mi.affine, mi.quad, 0, 0, 0, 0, 0};
     B->launches.push_back(L);
   }
   for (auto &L : B->launches) {
     L.a_off = (int)bmap.size();
     for (int i = 0; i < B->capacity; ++i) {
       const auto &mi = B->meta[i];
-      if (!mi.active || mi.kind != L.kind || mi.dkind != L.dkind || !same_quad(mi.quad, L.quad))
+      if (!mi.active || mi.kind != L.kind || mi.dkind != L.dkind || mi.affine != L.affine ||
+          !same_quad(mi.quad, L.quad))
         continue;
       B->slots[i].a_blk0 = (int32_t)bmap.size() - L.a_off;
       bmap.insert(bmap.end(), B->slots[i].tiles.n_tiles, i);
@@ -301,7 +304,8 @@ int batch_upload(tatva_batch *B, hipStream_t s) {
     L.g_off = (int)bmap.size();
     for (int i = 0; i < B->capacity; ++i) {
       const auto &mi = B->meta[i];
-      if (!mi.active || mi.kind != L.kind || mi.dkind != L.dkind || !same_quad(mi.quad, L.quad))
+      if (!mi.active || mi.kind != L.kind || mi.dkind != L.dkind || mi.affine != L.affine ||
+          !same_quad(mi.quad, L.quad))
         continue;
       B->slots[i].g_blk0 = (int32_t)bmap.size() - L.g_off;
       bmap.insert(bmap.end(), B->slots[i].g_blocks, i);
@@ -476,6 +480,7 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   mi.dkind = d->kind;
   mi.ncomp = ncomp;
   mi.quad = m->quad;
+  mi.affine = m->hex_affine;
   mi.lds = (size_t)m->tiles.max_local * ncomp * sizeof(float);
   mi.mesh = m;
   B->dirty = true;
@@ -509,8 +514,12 @@ extern "C" int tatva_batch_iterate(tatva_batch *B, int32_t iters, void *stream) 
       rc = with_class(L.kind, L.dkind, [&](auto kc, auto dens_tag) -> int {
         constexpr int K = decltype(kc)::value;
         using Dens = decltype(dens_tag);
-        k_batch_assemble<K, Dens><<<L.a_n, kTile, L.lds, s>>>(B->d_slots, B->d_bmap + L.a_off,
-                                                               L.quad);
+        if (K == TATVA_HEX8 && L.affine)
+          k_batch_assemble<K, Dens, true><<<L.a_n, kTile, L.lds, s>>>(B->d_slots,
+                                                                      B->d_bmap + L.a_off, L.quad);
+        else
+          k_batch_assemble<K, Dens, false><<<L.a_n, kTile, L.lds, s>>>(B->d_slots,
+                                                                       B->d_bmap + L.a_off, L.quad);
         k_batch_gather<Dens::NC><<<L.g_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + L.g_off);
         return TATVA_OK;
       });

@@ -396,6 +396,57 @@ __global__ __launch_bounds__(kBlock) void k_colored_rows(
   for (int32_t k = k0 + lane; k < k1; k += kWave) data[k] = J[colors[indices[k]]];
 }
 
+// Reduced row r = sum of full rows row_list[row_ptr[r]..row_ptr[r+1]); columns
+// through read_map.  serial: one lane accumulates an element's columns in turn
+// (two element dofs may share a reduced column).
+template <int NNE, int NC>
+__global__ __launch_bounds__(kBlock) void k_colored_rows_mapped(
+    int64_t n_rows, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ row_list,
+    const int32_t *__restrict__ read_map, const int32_t *__restrict__ colors, int n_colors,
+    const int32_t *__restrict__ indptr, const int32_t *__restrict__ indices,
+    const int32_t *__restrict__ inv_ptr, const int32_t *__restrict__ inv_list,
+    const int32_t *__restrict__ conn, const double *__restrict__ Ke, double *__restrict__ data,
+    int serial) {
+  constexpr int ND = NNE * NC;
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x % kWave;
+  const int wid = threadIdx.x / kWave;
+  const int64_t row = xcd_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
+  if (row >= n_rows) return;
+  double *J = lds + (int64_t)wid * n_colors;
+  for (int c = lane; c < n_colors; c += kWave) J[c] = 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int32_t t = row_ptr[row]; t < row_ptr[row + 1]; ++t) {
+    const int64_t R = row_list[t];
+    const int64_t node = R / NC;
+    const int ci = (int)(R - node * NC);
+    const int32_t b0 = inv_ptr[node], b1 = inv_ptr[node + 1];
+    for (int32_t p = b0; p < b1; ++p) {
+      const int32_t packed = inv_list[p];
+      const int64_t e = packed >> 3;
+      const int a = packed & 7;
+      const double *ke = Ke + (e * ND + a * NC + ci) * ND;
+      if (serial) {
+        if (lane == 0)
+          for (int l = 0; l < ND; ++l) {
+            const int32_t s = read_map[(int64_t)conn[e * NNE + l / NC] * NC + l % NC];
+            if (s >= 0) J[colors[s]] += ke[l];
+          }
+      } else if (lane < ND) {
+        const int32_t s = read_map[(int64_t)conn[e * NNE + lane / NC] * NC + lane % NC];
+        if (s >= 0) J[colors[s]] += ke[lane];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  const int32_t k0 = indptr[row], k1 = indptr[row + 1];
+  for (int32_t k = k0 + lane; k < k1; k += kWave) data[k] = J[colors[indices[k]]];
+}
+
 }  // namespace tatva
 
 using namespace tatva;
@@ -627,6 +678,16 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     }
   }
   m->n_slots = T.node_ptr.back();
+  if (elem_kind == TATVA_HEX8) {
+    bool affine = true;
+    for (int64_t e = 0; e < n_elements && affine; ++e) {
+      double X[8][3];
+      for (int a = 0; a < 8; ++a)
+        for (int j = 0; j < 3; ++j) X[a][j] = coords[(int64_t)elements[e * 8 + a] * 3 + j];
+      affine = hex_is_affine_host(X);
+    }
+    m->hex_affine = affine ? 1 : 0;
+  }
   hipError_t err = hipSetDevice(device);
   if (err != hipSuccess) return fail(err, "hipSetDevice");
   // tile partial sums: n_slots x ncomp (ncomp <= dim) values of at most 8 bytes
@@ -816,7 +877,7 @@ namespace tatva {
 template <class T, int MODE>
 static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, const T *v,
                            const int32_t *dof_map, const double *f_ext, T *out, hipStream_t s,
-                           const int *skip) {
+                           const int *skip, double *part_dot = nullptr, int gather_blocks = 0) {
   T *partial = (T *)m->scratch;
   return with_kind(m->kind, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
@@ -824,10 +885,18 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
       using Dens = decltype(dens);
       constexpr int NC = Dens::NC;
       const size_t lds = (size_t)m->tiles.max_local * NC * sizeof(T);
-      k_tile_assemble<K, Dens, T, MODE><<<m->tiles.n_tiles, kTile, lds, s>>>(
-          m->coords, m->conn, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
-      k_slot_gather<NC, T><<<grid_for(m->n_nodes), kBlock, 0, s>>>(
-          m->n_nodes, m->t_slot_ptr, m->t_slot_list, partial, f_ext, dof_map, out, skip);
+      if (K == TATVA_HEX8 && m->hex_affine)
+        k_tile_assemble<K, Dens, T, MODE, true><<<m->tiles.n_tiles, kTile, lds, s>>>(
+            m->coords, m->conn, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
+      else
+        k_tile_assemble<K, Dens, T, MODE, false><<<m->tiles.n_tiles, kTile, lds, s>>>(
+            m->coords, m->conn, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
+      if (part_dot)
+        k_slot_gather_dot<NC, T><<<gather_blocks, kBlock, 0, s>>>(
+            m->n_nodes, m->t_slot_ptr, m->t_slot_list, partial, dof_map, v, out, part_dot, skip);
+      else
+        k_slot_gather<NC, T><<<grid_for(m->n_nodes), kBlock, 0, s>>>(
+            m->n_nodes, m->t_slot_ptr, m->t_slot_list, partial, f_ext, dof_map, out, skip);
       TATVA_CHECK_LAUNCH();
       return TATVA_OK;
     });
@@ -842,6 +911,18 @@ int launch_hvp_f64(tatva_mesh *m, const tatva_density *d, const double *u, const
 int launch_hvp_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                    const int32_t *dof_map, float *y, hipStream_t s, const int *skip) {
   return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip);
+}
+
+int hvp_gather_blocks(const tatva_mesh *m) {
+  int64_t b = (m->n_nodes + kBlock - 1) / kBlock;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, kMaxReduceBlocks));
+}
+
+int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
+                       const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
+                       const int *skip) {
+  return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip, part_dot,
+                                          hvp_gather_blocks(m));
 }
 
 }  // namespace tatva
@@ -927,6 +1008,46 @@ extern "C" int tatva_colored_hessian(tatva_mesh *m, const tatva_density *d, cons
       k_colored_rows<NNE, NC><<<(int)nblk, kBlock, lds, s>>>(
           n_rows, red_to_full, dof_map, colors, n_colors, indptr, indices, m->inv_ptr,
           m->inv_list, m->conn, Ke, data);
+      TATVA_CHECK_LAUNCH();
+      return TATVA_OK;
+    });
+  });
+}
+
+extern "C" int tatva_colored_hessian_mapped(tatva_mesh *m, const tatva_density *d, const double *u,
+                                            const int32_t *read_map, const int32_t *row_ptr,
+                                            const int32_t *row_list, int64_t n_rows,
+                                            const int32_t *colors, int32_t n_colors,
+                                            const int32_t *indptr, const int32_t *indices,
+                                            int64_t nnz, int32_t flags, double *data,
+                                            void *workspace, int64_t ws_bytes, void *stream) {
+  MESH_PRELUDE(m);
+  TATVA_REQUIRE(u && read_map && row_ptr && row_list && colors && indptr && indices && data,
+                "invalid arguments");
+  TATVA_REQUIRE(n_colors >= 1 && n_colors <= kMaxColors, "n_colors must be in [1, 2048]");
+  const int nc = density_ncomp(m, d);
+  const int64_t need = tatva_colored_hessian_workspace(m, nc);
+  TATVA_REQUIRE(workspace && ws_bytes >= need, "colored Hessian workspace too small");
+  (void)nnz;
+  hipStream_t s = (hipStream_t)stream;
+  double *Ke = (double *)workspace;
+  const int rows_per_block = kBlock / kWave;
+  const size_t lds = (size_t)rows_per_block * n_colors * sizeof(double);
+  return with_kind(m->kind, [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    return with_density<Elem<K>::D, double>(d, [&](auto dens) -> int {
+      using Dens = decltype(dens);
+      constexpr int NNE = Elem<K>::NNE, NC = Dens::NC;
+      const int64_t nthreads = m->n_elements * NNE * NC;
+      k_elem_stiffness<K, Dens><<<grid_for(nthreads), kBlock, 0, s>>>(
+          m->n_elements, m->coords, m->conn, m->quad, dens, u, Ke);
+      TATVA_CHECK_LAUNCH();
+      if (n_rows == 0) return TATVA_OK;
+      const int64_t nblk = (n_rows + rows_per_block - 1) / rows_per_block;
+      TATVA_REQUIRE(nblk < (int64_t(1) << 31), "too many rows");
+      k_colored_rows_mapped<NNE, NC><<<(int)nblk, kBlock, lds, s>>>(
+          n_rows, row_ptr, row_list, read_map, colors, n_colors, indptr, indices, m->inv_ptr,
+          m->inv_list, m->conn, Ke, data, flags & 1);
       TATVA_CHECK_LAUNCH();
       return TATVA_OK;
     });

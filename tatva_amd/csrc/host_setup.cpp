@@ -108,6 +108,84 @@ static int node_graph(const int32_t *elements, int64_t E, int nne, int64_t N,
   return TATVA_OK;
 }
 
+extern "C" int tatva_pattern_from_element_dofs(const int32_t *element_dofs, int64_t E, int k,
+                                               int64_t size, const int32_t *diag_dofs,
+                                               int64_t n_diag, tatva_csr_host **out) {
+  if (!out) {
+    set_error("out is NULL");
+    return TATVA_EINVAL;
+  }
+  *out = nullptr;
+  if (E < 0 || k < 0 || size < 0 || n_diag < 0 || (E * k > 0 && !element_dofs) ||
+      (n_diag > 0 && !diag_dofs) || size >= (int64_t(1) << 31)) {
+    set_error("invalid arguments");
+    return TATVA_EINVAL;
+  }
+  for (int64_t i = 0; i < E * k; ++i)
+    if (element_dofs[i] < -1 || element_dofs[i] >= size) {
+      set_error("element dofs outside [-1, size)");
+      return TATVA_EINVAL;
+    }
+  std::vector<char> diag((size_t)size, 0);
+  for (int64_t i = 0; i < n_diag; ++i) {
+    if (diag_dofs[i] < 0 || diag_dofs[i] >= size) {
+      set_error("diagonal dofs outside [0, size)");
+      return TATVA_EINVAL;
+    }
+    diag[(size_t)diag_dofs[i]] = 1;
+  }
+  // dof -> incident elements (ascending), -1 entries skipped
+  std::vector<int64_t> iptr((size_t)size + 1, 0);
+  for (int64_t i = 0; i < E * k; ++i)
+    if (element_dofs[i] >= 0) iptr[(size_t)element_dofs[i] + 1]++;
+  for (int64_t i = 0; i < size; ++i) iptr[(size_t)i + 1] += iptr[(size_t)i];
+  std::vector<int64_t> ilist((size_t)iptr[(size_t)size]);
+  {
+    std::vector<int64_t> fill(iptr.begin(), iptr.end() - 1);
+    for (int64_t e = 0; e < E; ++e)
+      for (int a = 0; a < k; ++a) {
+        const int32_t d = element_dofs[e * k + a];
+        if (d >= 0) ilist[(size_t)fill[(size_t)d]++] = e;
+      }
+  }
+  auto *h = new tatva_csr_host();
+  h->n_rows = size;
+  std::vector<int64_t> deg((size_t)size, 0), ptr;
+  for (int pass = 0; pass < 2; ++pass) {
+    parallel_for(size, [&](int64_t b, int64_t e, int) {
+      std::vector<int32_t> cand;
+      for (int64_t i = b; i < e; ++i) {
+        cand.clear();
+        for (int64_t p = iptr[(size_t)i]; p < iptr[(size_t)i + 1]; ++p) {
+          const int64_t el = ilist[(size_t)p];
+          for (int a = 0; a < k; ++a) {
+            const int32_t d = element_dofs[el * k + a];
+            if (d >= 0) cand.push_back(d);
+          }
+        }
+        if (diag[(size_t)i]) cand.push_back((int32_t)i);
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        if (pass == 0) deg[(size_t)i] = (int64_t)cand.size();
+        else std::copy(cand.begin(), cand.end(), h->indices.begin() + ptr[(size_t)i]);
+      }
+    });
+    if (pass == 0) {
+      ptr.assign((size_t)size + 1, 0);
+      for (int64_t i = 0; i < size; ++i) ptr[(size_t)i + 1] = ptr[(size_t)i] + deg[(size_t)i];
+      if (ptr[(size_t)size] >= (int64_t(1) << 31)) {
+        delete h;
+        set_error("pattern has 2^31 or more entries");
+        return TATVA_EINVAL;
+      }
+      h->indices.assign((size_t)ptr[(size_t)size], 0);
+    }
+  }
+  h->indptr.assign(ptr.begin(), ptr.end());
+  *out = h;
+  return TATVA_OK;
+}
+
 extern "C" int tatva_pattern_from_mesh(const int32_t *elements, int64_t E, int nne, int64_t N,
                                        int ndpn, const int32_t *dof_map, int64_t n_rows,
                                        tatva_csr_host **out) {

@@ -27,6 +27,7 @@ struct tatva_mesh {
   int64_t scratch_bytes;
   double *partials;      // reduction partials, kMaxReduceBlocks * 4
   int max_incident;      // largest number of elements sharing a node
+  int hex_affine;        // Hex8 mesh of parallelepipeds only (constant Jacobian per element)
   // LDS tiling (assembly.hpp)
   tatva::TileSet tiles;
   int64_t n_slots;
@@ -47,6 +48,12 @@ int launch_hvp_f64(tatva_mesh *m, const tatva_density *d, const double *u, const
                    const int32_t *dof_map, double *y, hipStream_t s, const int *skip = nullptr);
 int launch_hvp_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                    const int32_t *dof_map, float *y, hipStream_t s, const int *skip = nullptr);
+// y = H(u) v (FP32) with the block shares of v.y in part_dot, over
+// hvp_gather_blocks(m) blocks (v, y reduced through dof_map).
+int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
+                       const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
+                       const int *skip = nullptr);
+int hvp_gather_blocks(const tatva_mesh *m);
 int density_ncomp(const tatva_mesh *m, const tatva_density *d);
 // Drops cached solver graphs that captured this mesh's arrays.
 void forget_mesh_graphs(const tatva_mesh *m);

@@ -892,16 +892,16 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, r, p, Ap, dinv, P.rz(1), P.rr(1), P.bb,
                          gv, tol, maxiter, st, s);
   if (rc) return rc;
+  const int g_op = hvp_gather_blocks(m);
   const std::function<int()> apply_op = [&]() -> int {
-    return launch_hvp_f32(m, d, u, p, dof_map, Ap, s, &st->done);
+    return launch_hvp_dot_f32(m, d, u, p, dof_map, Ap, P.pap, s, &st->done);
   };
   const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
     if (with_op) {
       const int e = apply_op();
       if (e) return e;
     }
-    k_pcg_dot<float><<<gv, kBlock, 0, s>>>(n, p, Ap, P.pap, st);
-    k_pcg_update<float><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, P.pap, gv, P.rz(q ^ 1),
+    k_pcg_update<float><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, P.pap, g_op, P.rz(q ^ 1),
                                               P.rz(q), P.rr(q), st);
     k_pcg_pupdate<float><<<gv, kBlock, 0, s>>>(n, p, r, dinv, P.rz(q ^ 1), P.rz(q), P.rr(q), st);
     return TATVA_OK;
@@ -948,6 +948,27 @@ __global__ void k_set_dofs(int64_t n, const int32_t *__restrict__ idx, const dou
                            double value, double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[idx[i]] = vals ? vals[i] : value;
+}
+
+// out[d] = map[d] >= 0 ? in[map[d]] : 0 (linear part of a lift)
+__global__ void k_lift_map(int64_t n, const int32_t *__restrict__ map, const double *__restrict__ in,
+                           double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int32_t r = map[i];
+    out[i] = r >= 0 ? in[r] : 0.0;
+  }
+}
+
+// out[j] = sum of in[list[k]] over k in [ptr[j], ptr[j+1]), ascending k
+__global__ void k_reduce_rows(int64_t n, const int32_t *__restrict__ ptr,
+                              const int32_t *__restrict__ list, const double *__restrict__ in,
+                              double *__restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int32_t k = ptr[j]; k < ptr[j + 1]; ++k) s += in[list[k]];
+  out[j] = s;
 }
 
 template <class T>
@@ -1015,6 +1036,26 @@ extern "C" int tatva_gather_free(int64_t n, const int32_t *idx, const double *in
   TATVA_REQUIRE(n >= 0 && (n == 0 || (idx && in && out)), "invalid arguments");
   if (n == 0) return TATVA_OK;
   k_gather_free<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, idx, in, out);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_lift_map(int64_t n_full, const int32_t *map, const double *in_red,
+                              double *out_full, void *stream) {
+  TATVA_REQUIRE(n_full >= 0 && (n_full == 0 || (map && in_red && out_full)), "invalid arguments");
+  if (n_full == 0) return TATVA_OK;
+  k_lift_map<<<grid_for(n_full), kBlock, 0, (hipStream_t)stream>>>(n_full, map, in_red, out_full);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+extern "C" int tatva_reduce_rows(int64_t n_red, const int32_t *ptr, const int32_t *list,
+                                 const double *in_full, double *out_red, void *stream) {
+  TATVA_REQUIRE(n_red >= 0 && (n_red == 0 || (ptr && list && in_full && out_red)),
+                "invalid arguments");
+  if (n_red == 0) return TATVA_OK;
+  k_reduce_rows<<<grid_for(n_red), kBlock, 0, (hipStream_t)stream>>>(n_red, ptr, list, in_full,
+                                                                     out_red);
   TATVA_CHECK_LAUNCH();
   return TATVA_OK;
 }

@@ -248,23 +248,41 @@ class ReducedEnergy:
         return self.energy(self.lift(u_red))
 
     def residual(self, u_red):
-        return self.energy.residual(self.lift(u_red), dof_map=self.lifter.dof_map_device(),
-                                    n_out=self.size)
+        return self.residual_full_state(self.lift(u_red))
 
     def residual_full_state(self, u_full):
-        """Reduced residual at an already lifted state."""
-        return self.energy.residual(u_full, dof_map=self.lifter.dof_map_device(), n_out=self.size)
+        """Reduced residual at an already lifted state: the kernels write the free
+        rows directly; with Periodic constraints the full residual goes through
+        reduce_adjoint (a slave's row is added to its master's)."""
+        lf = self.lifter
+        if lf.has_periodic:
+            return lf.reduce_adjoint(self.energy.residual(u_full))
+        return self.energy.residual(u_full, dof_map=lf.dof_map_device(), n_out=self.size)
 
     def hvp(self, u_red, v_red):
-        return self.energy.hvp(self.lift(u_red), v_red, dof_map=self.lifter.dof_map_device(),
-                               n_out=self.size)
+        return self.hvp_full_state(self.lift(u_red), v_red)
 
     def hvp_full_state(self, u_full, v_red):
-        return self.energy.hvp(u_full, v_red, dof_map=self.lifter.dof_map_device(), n_out=self.size)
+        lf = self.lifter
+        if lf.has_periodic:
+            t = nat.torch()
+            if isinstance(v_red, t.Tensor) and v_red.dtype == t.float32:
+                raise NotImplementedError("FP32 HVP with Periodic constraints is not supported")
+            y = self.energy.hvp(u_full, lf.lift_linear(v_red))
+            return lf.reduce_adjoint(y)
+        return self.energy.hvp(u_full, v_red, dof_map=lf.dof_map_device(), n_out=self.size)
 
     def hessian_diag(self, u_red, dtype=None):
-        return self.energy.hessian_diag(self.lift(u_red), dof_map=self.lifter.dof_map_device(),
-                                        n_out=self.size, dtype=dtype)
+        """Jacobi diagonal.  With Periodic constraints it sums the full diagonal
+        entries of the dofs reading each reduced index (exact unless a slave and
+        its master share an element)."""
+        lf = self.lifter
+        uf = self.lift(u_red)
+        if lf.has_periodic:
+            d = lf.reduce_adjoint(self.energy.hessian_diag(uf))
+            return d if dtype is None else d.to(dtype)
+        return self.energy.hessian_diag(uf, dof_map=lf.dof_map_device(), n_out=self.size,
+                                        dtype=dtype)
 
     def linearize(self, u_red):
         uf = self.lift(u_red)

@@ -1,8 +1,9 @@
-"""Lifter: reduced <-> full dof vectors under Dirichlet constraints.
+"""Lifter: reduced <-> full dof vectors under Dirichlet and periodic constraints.
 
-Mirrors ``tatva.lifter`` (lifter/base.py:55-331, lifter/constraints.py:290-318,
-lifter/common.py:43-65) for ``Fixed`` constraints with constant, array or
-``RuntimeValue`` values (load stepping via ``lifter.at[key].set(v)``).  The
+Mirrors ``tatva.lifter`` (lifter/base.py:55-331, lifter/constraints.py:184-220,
+290-318, lifter/common.py:43-65) for ``Fixed`` constraints with constant, array or
+``RuntimeValue`` values (load stepping via ``lifter.at[key].set(v)``) and
+``Periodic`` constraints.  The
 free/constrained index sets are host setup; lift/reduce run as HIP kernels on
 GPU tensors.  The reduced sparsity is built natively (S[free][:, free]).
 """
@@ -113,8 +114,62 @@ class Fixed(Constraint):
         return self._dev_cache
 
 
+class Periodic(Constraint):
+    """``dofs`` take the values of ``master_dofs`` (lifter/constraints.py:184-220)."""
+
+    def __init__(self, dofs, master_dofs):
+        super().__init__(dofs)
+        self.master_dofs = _as_index(master_dofs)
+        if self.master_dofs.size != self.dofs.size:
+            raise LifterError("Periodic needs one master per constrained dof")
+        self._dev_cache = None
+
+    def augment_sparsity(self, sparsity):
+        """M^T S M with M = I + (slave -> master) (constraints.py:194-212); the
+        product runs in int32 so that no entry count can wrap."""
+        S = sps.csr_matrix(sparsity)
+        n = S.shape[0]
+        rows = np.concatenate([np.arange(n), self.dofs])
+        cols = np.concatenate([np.arange(n), self.master_dofs])
+        M = sps.csr_matrix((np.ones(rows.size, dtype=np.int32), (rows, cols)), shape=(n, n))
+        S1 = sps.csr_matrix((np.ones(S.nnz, dtype=np.int32), S.indices, S.indptr), shape=S.shape)
+        out = (M.T @ (S1 @ M)).tocsr()
+        out.sum_duplicates()
+        out.sort_indices()
+        return sps.csr_matrix((np.ones(out.nnz, dtype=np.int8), out.indices, out.indptr),
+                              shape=out.shape)
+
+    def apply_lift(self, u_full, runtime_values) -> None:
+        """u_full[dofs] = u_full[master_dofs] (constraints.py:214-216), in place:
+        all masters are read before any slave is written."""
+        if self.dofs.size == 0:
+            return
+        t = nat.torch()
+        L = nat.lib()
+        slaves, masters = self._dev()
+        tmp = t.empty(self.dofs.size, dtype=t.float64, device=u_full.device)
+        nat.check(L.tatva_gather_free(self.dofs.size, nat.dptr(masters), nat.dptr(u_full),
+                                      nat.dptr(tmp), nat.stream()))
+        nat.check(L.tatva_scatter_free(self.dofs.size, nat.dptr(slaves), nat.dptr(tmp),
+                                       nat.dptr(u_full), nat.stream()))
+
+    def _dev(self):
+        if self._dev_cache is None:
+            i32 = nat.torch().int32
+            self._dev_cache = (nat.to_device(self.dofs, i32), nat.to_device(self.master_dofs, i32))
+        return self._dev_cache
+
+
 class _Layout:
-    """Free/constrained index sets shared by lifters that differ in runtime values."""
+    """Free/constrained index sets shared by lifters that differ in runtime values.
+
+    ``dof_map``: full dof -> reduced index of a free dof, else -1.
+    ``read_map``: the linear part of the lift, full dof -> reduced index whose
+    value the dof takes (a free dof its own, a Periodic slave its master's,
+    resolved in constraint order as lifter/base.py:201-221 applies them), else -1.
+    ``red_ptr``/``red_list``: for each reduced index the full dofs reading it,
+    ascending; reduce_adjoint sums them (lifter/base.py:235-251).
+    """
 
     def __init__(self, size: int, constraints):
         self.size = size
@@ -124,14 +179,33 @@ class _Layout:
             con = np.zeros(0, dtype=np.int64)
         if con.size and (con[0] < 0 or con[-1] >= size):
             raise LifterError("constrained dofs outside [0, size)")
+        for c in constraints:
+            if isinstance(c, Periodic) and c.master_dofs.size and \
+                    (c.master_dofs.min() < 0 or c.master_dofs.max() >= size):
+                raise LifterError("periodic master dofs outside [0, size)")
         self.constrained = con
         mask = np.ones(size, dtype=bool)
         mask[con] = False
         self.free = np.nonzero(mask)[0].astype(np.int64)
         self.dof_map = np.full(size, -1, dtype=np.int32)
         self.dof_map[self.free] = np.arange(self.free.size, dtype=np.int32)
+        self.has_periodic = any(isinstance(c, Periodic) for c in constraints)
+        rm = self.dof_map.copy()
+        for c in constraints:
+            if isinstance(c, Periodic):
+                rm[c.dofs] = rm[c.master_dofs].copy()
+            else:
+                rm[c.dofs] = -1
+        self.read_map = rm
+        order = np.argsort(rm, kind="stable")
+        order = order[rm[order] >= 0]
+        counts = np.bincount(rm[order], minlength=self.free.size)
+        self.red_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        self.red_list = order.astype(np.int32)
         self._free_dev = None
         self._map_dev = None
+        self._read_dev = None
+        self._rows_dev = None
 
     def free_dev(self):
         if self._free_dev is None:
@@ -142,6 +216,17 @@ class _Layout:
         if self._map_dev is None:
             self._map_dev = nat.to_device(self.dof_map, nat.torch().int32)
         return self._map_dev
+
+    def read_dev(self):
+        if self._read_dev is None:
+            self._read_dev = nat.to_device(self.read_map, nat.torch().int32)
+        return self._read_dev
+
+    def rows_dev(self):
+        if self._rows_dev is None:
+            i32 = nat.torch().int32
+            self._rows_dev = (nat.to_device(self.red_ptr, i32), nat.to_device(self.red_list, i32))
+        return self._rows_dev
 
 
 class Lifter:
@@ -180,11 +265,27 @@ class Lifter:
     def free_device(self):
         return self._layout.free_dev()
 
+    @property
+    def has_periodic(self) -> bool:
+        return self._layout.has_periodic
+
+    @property
+    def read_map(self) -> np.ndarray:
+        """full dof -> reduced index it reads in a lift (-1: constraint value)."""
+        return self._layout.read_map
+
+    def read_map_device(self):
+        return self._layout.read_dev()
+
+    def reduce_rows_device(self):
+        """(ptr, list): the full dofs summed into each reduced index by reduce_adjoint."""
+        return self._layout.rows_dev()
+
     def _require_device_kernels(self):
         for c in self.constraints:
-            if not isinstance(c, Fixed):
+            if not isinstance(c, (Fixed, Periodic)):
                 raise NotImplementedError(f"{type(c).__name__} constraints are not supported on "
-                                          "the GPU path (Fixed only)")
+                                          "the GPU path (Fixed and Periodic)")
 
     def __hash__(self) -> int:
         return hash((self.size, self.constraints))
@@ -264,10 +365,33 @@ class Lifter:
         return out
 
     def reduce_adjoint(self, r_full):
-        """Reduced dual vector (base.py:235-251); for Fixed the constrained rows are
-        zeroed and the free rows kept, i.e. r_full[free_dofs]."""
+        """Reduced dual vector (base.py:235-251): the transposes of the constraints
+        in reverse order, then the free rows.  With Fixed only this is
+        r_full[free_dofs]; a Periodic slave's entry is added to its master's."""
         self._require_device_kernels()
-        return self.reduce(r_full)
+        if not self.has_periodic:
+            return self.reduce(r_full)
+        t = nat.torch()
+        rf = nat.to_device(r_full).reshape(-1).contiguous()
+        if rf.shape[0] != self.size:
+            raise LifterError(f"r_full must have {self.size} entries")
+        out = t.empty(self.size_reduced, dtype=t.float64, device=rf.device)
+        ptr, lst = self.reduce_rows_device()
+        nat.check(nat.lib().tatva_reduce_rows(self.size_reduced, nat.dptr(ptr), nat.dptr(lst),
+                                              nat.dptr(rf), nat.dptr(out), nat.stream()))
+        return out
+
+    def lift_linear(self, v_reduced):
+        """Linear part of the lift (constraint values at zero): the tangent of
+        lift_from_zeros, v_full[d] = v_reduced[read_map[d]] or 0."""
+        t = nat.torch()
+        vr = nat.to_device(v_reduced).reshape(-1).contiguous()
+        if vr.shape[0] != self.size_reduced:
+            raise LifterError(f"v_reduced must have {self.size_reduced} entries")
+        out = t.empty(self.size, dtype=t.float64, device=vr.device)
+        nat.check(nat.lib().tatva_lift_map(self.size, nat.dptr(self.read_map_device()),
+                                           nat.dptr(vr), nat.dptr(out), nat.stream()))
+        return out
 
     # ------------------------------------------------------------ sparsity --
 
@@ -369,4 +493,4 @@ def lifted(fn=None, *, argnums=0, output=None):
     return lifted_fn
 
 
-__all__ = ["Lifter", "Fixed", "Constraint", "RuntimeValue", "LifterError", "lifted"]
+__all__ = ["Lifter", "Fixed", "Periodic", "Constraint", "RuntimeValue", "LifterError", "lifted"]

@@ -192,6 +192,8 @@ class Newton:
         if lifter is not None:
             self.red.lifter = lifter
         lf = self.red.lifter
+        if self.mode == "mixed" and lf.has_periodic:
+            raise NotImplementedError("mode='mixed' supports Fixed constraints only")
         lf.apply_constraints(u_full)
         dof_map = lf.dof_map_device()
         free = lf.free_device()
@@ -218,7 +220,7 @@ class Newton:
             stats.newton_iterations += 1
             if self.mode == "csr":
                 ta = time.perf_counter()
-                data = self._asm.assemble_full_state(u_full, dof_map, free, data)
+                data = self._asm.assemble_full_state(u_full, lf, data)
                 K = replace(self._cm, data=data)
                 t.cuda.synchronize()
                 tb = time.perf_counter()
@@ -234,6 +236,8 @@ class Newton:
                     break
                 nat.check(L.tatva_axpy_free(n, nat.dptr(free), -1.0, nat.dptr(du),
                                             nat.dptr(u_full), nat.stream()))
+                if lf.has_periodic:
+                    lf.apply_constraints(u_full)  # slaves follow their masters
             else:
                 u32 = t.empty(u_full.numel(), dtype=t.float32, device=u_full.device)
                 nat.check(L.tatva_cast_f64_f32(u_full.numel(), 1.0, nat.dptr(u_full), nat.dptr(u32),

@@ -45,6 +45,61 @@ def pattern_from_mesh(mesh, n_dofs_per_node: int):
     return sps.csr_matrix((np.ones(nnz, dtype=np.int8), ix, ip), shape=(n, n))
 
 
+def _host_csr(L, h, n):
+    try:
+        nnz = L.tatva_csr_host_nnz(h)
+        ip = np.empty(n + 1, dtype=np.int32)
+        ix = np.empty(nnz, dtype=np.int32)
+        nat.check(L.tatva_csr_host_copy(h, ip.ctypes.data, ix.ctypes.data))
+    finally:
+        L.tatva_csr_host_free(h)
+    return sps.csr_matrix((np.ones(nnz, dtype=np.int8), ix, ip), shape=(n, n))
+
+
+def pattern_from_element_dofs(element_dofs, size: int, diagonal_dofs=None):
+    """CSR sparsity of a general element dof table (E, k) with -1 for "no dof":
+    all pairs of valid dofs of each element, plus (d, d) for ``diagonal_dofs``.
+    The pattern builder behind pattern_from_compound (sparse/_extraction.py:198-238)."""
+    ed = np.ascontiguousarray(element_dofs, dtype=np.int32)
+    if ed.ndim != 2:
+        raise ValueError("element_dofs must be a 2-D (n_elements, dofs_per_element) array")
+    dg = np.ascontiguousarray(np.zeros(0) if diagonal_dofs is None else diagonal_dofs,
+                              dtype=np.int32).ravel()
+    L = nat.lib()
+    h = ctypes.c_void_p()
+    nat.check(L.tatva_pattern_from_element_dofs(ed.ctypes.data if ed.size else None, ed.shape[0],
+                                                ed.shape[1], int(size),
+                                                dg.ctypes.data if dg.size else None, dg.size,
+                                                ctypes.byref(h)))
+    return _host_csr(L, h, int(size))
+
+
+def pattern_from_fields(mesh, size: int, nodal=(), diagonal=(), block_slices=None):
+    """``pattern_from_compound`` (sparse/_extraction.py:125-245) for a state laid
+    out as fields, without the Compound class: ``nodal`` is a sequence of
+    (n_nodes, dofs_per_node) int tables (global dof of each node's components,
+    -1 where a node has none), coupled through the mesh elements; ``diagonal``
+    is a sequence of dof arrays that only couple to themselves.  With
+    ``block_slices`` (a list of slices, one per block) the blocks
+    ``S[s1, s2]`` are returned as a list of lists, as with block_wise=True."""
+    el = np.asarray(mesh.elements, dtype=np.int64)
+    tables = []
+    for nd in nodal:
+        nd = np.asarray(nd, dtype=np.int64)
+        if nd.ndim == 1:
+            nd = nd[:, None]
+        if nd.shape[0] != mesh.coords.shape[0]:
+            raise ValueError("a nodal field table needs one row per mesh node")
+        tables.append(nd[el].reshape(el.shape[0], -1))
+    ed = np.concatenate(tables, axis=1) if tables else np.zeros((0, 0), dtype=np.int64)
+    dg = np.concatenate([np.asarray(d, dtype=np.int64).ravel() for d in diagonal]) \
+        if len(diagonal) else None
+    S = pattern_from_element_dofs(ed, size, dg)
+    if block_slices is None:
+        return S
+    return [[S[s1, s2] for s2 in block_slices] for s1 in block_slices]
+
+
 def distance2_colors(indptr, indices, n_dofs: int) -> np.ndarray:
     """Natural-order greedy distance-2 coloring (sparse/_coloring.py:270-283)."""
     ip = np.ascontiguousarray(indptr, dtype=np.int32)
@@ -198,6 +253,7 @@ class _HessianAssembler:
         if cm.n_colors > 2048:
             raise ValueError("more than 2048 colors are not supported")
         self._ws = None
+        self._serial = {}
 
     def workspace(self):
         if self._ws is None:
@@ -207,25 +263,46 @@ class _HessianAssembler:
         return self._ws
 
     def __call__(self, u, data=None):
-        t = nat.torch()
         en = self.res.energy
         if self.reduced:
-            u_full = en.lift(u)
-            dof_map = en.lifter.dof_map_device()
-            r2f = en.lifter.free_device()
-        else:
-            u_full = self.full._vec(u)
-            dof_map = r2f = None
-        return self.assemble_full_state(u_full, dof_map, r2f, data)
+            return self.assemble_full_state(en.lift(u), en.lifter, data)
+        return self.assemble_full_state(self.full._vec(u), None, data)
 
-    def assemble_full_state(self, u_full, dof_map, r2f, data=None):
+    def _serial_flag(self, lifter) -> int:
+        """1 when some element holds two dofs that read the same reduced index
+        (a periodic slave and its master in one element)."""
+        key = id(lifter._layout)
+        if self._serial.get(key) is None:
+            nc = self.full.ncomp
+            el = self.full.op.mesh.elements.astype(np.int64)
+            dofs = (el[:, :, None] * nc + np.arange(nc)).reshape(el.shape[0], -1)
+            rm = lifter.read_map[dofs]
+            rs = np.sort(np.where(rm >= 0, rm, -1 - np.arange(rm.shape[1])[None, :]), axis=1)
+            self._serial[key] = int(np.any(rs[:, 1:] == rs[:, :-1]))
+        return self._serial[key]
+
+    def assemble_full_state(self, u_full, lifter=None, data=None):
+        """Hessian data on the pattern at a lifted state: reduced by ``lifter``
+        (None: the full dofs)."""
         t = nat.torch()
         cm = self.cm
         if data is None:
             data = t.empty(cm.nnz, dtype=t.float64, device=u_full.device)
         ws = self.workspace()
         desc = self.full._desc
-        nat.check(nat.lib().tatva_colored_hessian(
+        L = nat.lib()
+        if lifter is not None and lifter.has_periodic:
+            ptr, lst = lifter.reduce_rows_device()
+            nat.check(L.tatva_colored_hessian_mapped(
+                self.full.op.handle, ctypes.byref(desc), nat.dptr(u_full),
+                nat.dptr(lifter.read_map_device()), nat.dptr(ptr), nat.dptr(lst), cm.shape[0],
+                nat.dptr(cm.colors), cm.n_colors, nat.dptr(cm.indptr), nat.dptr(cm.indices),
+                cm.nnz, self._serial_flag(lifter), nat.dptr(data), nat.dptr(ws), ws.numel(),
+                nat.stream()))
+            return data
+        dof_map = lifter.dof_map_device() if lifter is not None else None
+        r2f = lifter.free_device() if lifter is not None else None
+        nat.check(L.tatva_colored_hessian(
             self.full.op.handle, ctypes.byref(desc), nat.dptr(u_full), nat.dptr(dof_map),
             nat.dptr(r2f), cm.shape[0], nat.dptr(cm.colors), cm.n_colors, nat.dptr(cm.indptr),
             nat.dptr(cm.indices), cm.nnz, nat.dptr(data), nat.dptr(ws), ws.numel(), nat.stream()))
@@ -273,4 +350,5 @@ def __getattr__(name):
 
 
 __all__ = ["ColoredMatrix", "compute_rows_cols", "distance2_colors", "jacfwd",
-           "linearized_jacfwd", "pattern_from_mesh"]
+           "linearized_jacfwd", "pattern_from_mesh", "pattern_from_element_dofs",
+           "pattern_from_fields"]

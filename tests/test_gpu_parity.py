@@ -59,7 +59,8 @@ def oracle_density(d):
 
 
 def pair(kind, density, n=4, f_ext=False):
-    m, el = make_mesh(kind, n)
+    # "hex8-affine": un-jittered cube, every element a parallelepiped
+    m, el = make_mesh("hex8", n, jitter=0.0) if kind == "hex8-affine" else make_mesh(kind, n)
     op = ta.Operator(m, el)
     prob = fem.Problem(ORACLE_EL[type(el)](), m.coords, m.elements, oracle_density(density))
     fe = None
@@ -153,7 +154,7 @@ def test_eval_grad_weights_integrate(kind):
 
 DENSITIES = [energies.Poisson(1.0), energies.LinearElasticity(0.38, 0.58),
              energies.NeoHookean(1.0, 10.0)]
-CASES = [(k, d) for k in ("tri3", "tet4", "hex8") for d in DENSITIES]
+CASES = [(k, d) for k in ("tri3", "tet4", "hex8", "hex8-affine") for d in DENSITIES]
 
 
 @pytest.mark.parametrize("kind,density", CASES, ids=[f"{k}-{type(d).__name__}" for k, d in CASES])
@@ -187,7 +188,7 @@ def test_energy_residual_hvp_parity(kind, density):
     assert w.shape == tuple(psi.shape[:2])
 
 
-@pytest.mark.parametrize("kind,n", [("tri3", 16), ("tet4", 8), ("hex8", 9)])
+@pytest.mark.parametrize("kind,n", [("tri3", 16), ("tet4", 8), ("hex8", 9), ("hex8-affine", 9)])
 def test_assembly_over_many_tiles(kind, n):
     """Residual, HVP and diagonal on meshes of several 256-element tiles (nodes
     shared across tiles are summed by the slot gather) against the oracle."""
@@ -374,3 +375,104 @@ def test_newton_mixed_precision_neohookean():
         assert st_m.converged and st_c.converged
         assert st_m.residual_history[-1] <= 1e-8 * st_m.residual_history[0]
     assert rel_err(to_np(um), to_np(uc)) < 1e-6
+
+
+# ------------------------------------------------------------- periodic ----
+
+def _periodic_case(nx, ny, density):
+    """Tri3 square, Fixed bottom (u = 0) and top (u_y = 0.01), left edge periodic
+    with the right edge (tests/test_sparse_tracer.py:162-216 layout)."""
+    m = ta.Mesh.unit_square(nx, ny)
+    X = m.coords
+    bot = np.where(X[:, 1] == 0.0)[0]
+    top = np.where(X[:, 1] == 1.0)[0]
+    left = np.where(X[:, 0] == 0.0)[0]
+    left = left[np.argsort(X[left, 1])]
+    right = np.where(X[:, 0] == 1.0)[0]
+    right = right[np.argsort(X[right, 1])]
+    c = np.arange(2)
+    fixed = (bot[:, None] * 2 + c).ravel()
+    slaves, masters = (left[:, None] * 2 + c).ravel(), (right[:, None] * 2 + c).ravel()
+    L = lifter.Lifter(m.n_nodes * 2, lifter.Fixed(fixed, 0.0), lifter.Fixed(top * 2 + 1, 0.01),
+                      lifter.Periodic(slaves, masters))
+    Lo = fem.Lifter(m.n_nodes * 2, fem.Fixed(fixed, 0.0), fem.Fixed(top * 2 + 1, 0.01),
+                    fem.Periodic(slaves, masters))
+    op = ta.Operator(m, ta.Tri3())
+    E = op.energy(density)
+    prob = fem.Problem(fem.Tri3(), X, m.elements, oracle_density(density))
+    return m, L, Lo, E, prob, slaves, masters
+
+
+def test_periodic_lifter_kats_on_gpu():
+    """tests/test_lifter.py:26-41, tests/test_lifter_residual.py:7-78."""
+    k = GOLDEN["lifter_fixed_periodic"]
+    L = lifter.Lifter(k["size"], lifter.Fixed(k["fixed"]),
+                      lifter.Periodic(k["periodic_dofs"], k["periodic_master"]))
+    u = L.lift_from_zeros(np.array(k["u_reduced"]))
+    np.testing.assert_array_equal(to_np(u), k["lift_expected"])
+    np.testing.assert_array_equal(to_np(L.reduce(u)), k["u_reduced"])
+    for name in ("reduce_adjoint_periodic", "reduce_adjoint_periodic_fixed"):
+        k = GOLDEN[name]
+        cons = [lifter.Periodic(k["periodic_dofs"], k["periodic_master"])]
+        if "fixed" in k:
+            cons.append(lifter.Fixed(k["fixed"]))
+        L = lifter.Lifter(k["size"], *cons)
+        np.testing.assert_array_equal(to_np(L.reduce_adjoint(np.array(k["r_full"]))),
+                                      k["expected"])
+    # tests/test_lifter_residual.py:39-56: lifted dual output
+    L = lifter.Lifter(4, lifter.Periodic([2], [1]))
+    f = lifter.lifted(lambda u: u * 2.0, output="dual")
+    np.testing.assert_array_equal(to_np(f(L, np.array([1.0, 2.0, 3.0]))), [2.0, 8.0, 6.0])
+
+
+@pytest.mark.parametrize("density", [energies.LinearElasticity(0.4, 0.6),
+                                     energies.NeoHookean(1.0, 1.0)],
+                         ids=["LinearElasticity", "NeoHookean"])
+def test_periodic_residual_hvp_hessian_newton(density):
+    m, L, Lo, E, prob, slaves, masters = _periodic_case(6, 5, density)
+    red = E.reduced(L)
+    rng = np.random.default_rng(7)
+    ur = 0.01 * rng.standard_normal(L.size_reduced)
+    vr = rng.standard_normal(L.size_reduced)
+    uf = Lo.lift_from_zeros(ur)
+    np.testing.assert_allclose(to_np(red.lift(ur)), uf, rtol=0, atol=1e-15)
+    r_or = Lo.reduce_adjoint(prob.residual(uf))
+    assert rel_err(to_np(red.grad(ur)), r_or) < RTOL
+    y_or = Lo.reduce_adjoint(prob.hvp(uf, Lo.lift_linear(vr)))
+    assert rel_err(to_np(red.hvp(ur, vr)), y_or) < RTOL
+    # colored reduced Hessian on the augmented + reduced pattern
+    S = L.adapt_sparsity(sparse.pattern_from_mesh(m, 2))
+    cm = sparse.ColoredMatrix.from_csr(S)
+    K = sparse.jacfwd(red.grad, cm)(ur)
+    So = Lo.adapt_sparsity(fem.pattern_from_mesh(m.elements, m.n_nodes, 2))
+    np.testing.assert_array_equal(to_np(cm.indices), So.indices)
+    Ko = fem.hessian_csr(prob, Lo, uf, So, to_np(cm.colors))
+    assert rel_err(to_np(K.data), Ko.data) < RTOL
+    # Newton (colored CSR + PCG) against the oracle's Newton
+    u = t_zeros(m.n_nodes * 2)
+    st = solvers.Newton(E, L, mode="csr", tol=1e-10, lin_tol=1e-12).solve(u)
+    assert st.converged
+    uo, *_ = fem.newton(prob, Lo, Lo.lift_from_zeros(np.zeros(Lo.size_reduced)), So,
+                        to_np(cm.colors), tol=1e-10, lin_tol=1e-12)
+    un = to_np(u)
+    np.testing.assert_array_equal(un[slaves], un[masters])
+    assert np.abs(un - uo).max() <= 1e-8 * np.abs(uo).max()
+
+
+def test_periodic_hessian_when_slave_and_master_share_elements():
+    """One cell across: every element holds a slave and its master (the serial
+    accumulation path of the mapped colored Hessian)."""
+    m, L, Lo, E, prob, slaves, masters = _periodic_case(1, 4, energies.LinearElasticity(0.4, 0.6))
+    red = E.reduced(L)
+    S = L.adapt_sparsity(sparse.pattern_from_mesh(m, 2))
+    cm = sparse.ColoredMatrix.from_csr(S)
+    ur = np.zeros(L.size_reduced)
+    K = sparse.jacfwd(red.grad, cm)(ur)
+    So = Lo.adapt_sparsity(fem.pattern_from_mesh(m.elements, m.n_nodes, 2))
+    Ko = fem.hessian_csr(prob, Lo, Lo.lift_from_zeros(ur), So, to_np(cm.colors))
+    assert rel_err(to_np(K.data), Ko.data) < RTOL
+
+
+def t_zeros(n):
+    import torch
+    return torch.zeros(n, dtype=torch.float64, device="cuda")

@@ -189,3 +189,69 @@ def test_jacfwd_rejects_callables():
         sparse._HessianAssembler(lambda u: u, cm_cls(None, None, None, S.shape, None, 3))
     with pytest.raises(NotImplementedError):
         ta.grad(lambda u: u)
+
+
+def test_periodic_layout_and_sparsity_match_oracle():
+    """Read map / reduce rows of Fixed + Periodic, and augment + reduce of the
+    reference's periodic sparsity test layout (tests/test_sparse_tracer.py:162-216)."""
+    L = lifter.Lifter(6, lifter.Fixed([0, 5]), lifter.Periodic([2], [1]))
+    np.testing.assert_array_equal(L.free_dofs, [1, 3, 4])
+    np.testing.assert_array_equal(L.read_map, [-1, 0, 0, 1, 2, -1])
+    np.testing.assert_array_equal(L._layout.red_ptr, [0, 2, 3, 4])
+    np.testing.assert_array_equal(L._layout.red_list, [1, 2, 3, 4])
+    # order matters: a periodic slave copying a master fixed later reads nothing
+    L2 = lifter.Lifter(4, lifter.Periodic([2], [1]), lifter.Fixed([1]))
+    np.testing.assert_array_equal(L2.read_map, [0, -1, -1, 1])
+    m = ta.Mesh.unit_square(4, 4)
+    X = m.coords
+    bot = np.where(X[:, 1] == 0.0)[0]
+    left = np.where(X[:, 0] == 0.0)[0]
+    left = left[np.argsort(X[left, 1])]
+    right = np.where(X[:, 0] == 1.0)[0]
+    right = right[np.argsort(X[right, 1])]
+    c = np.arange(2)
+    cons = [((bot[:, None] * 2 + c).ravel(),),
+            ((left[:, None] * 2 + c).ravel(), (right[:, None] * 2 + c).ravel())]
+    L = lifter.Lifter(m.n_nodes * 2, lifter.Fixed(*cons[0]), lifter.Periodic(*cons[1]))
+    Lo = fem.Lifter(m.n_nodes * 2, fem.Fixed(*cons[0]), fem.Periodic(*cons[1]))
+    S = sparse.pattern_from_mesh(m, 2)
+    So = fem.pattern_from_mesh(m.elements, m.n_nodes, 2)
+    aug, aug_o = L.augment_sparsity(S), fem.Periodic(*cons[1]).augment(So)
+    assert (aug != aug_o).nnz == 0
+    R, Ro = L.adapt_sparsity(S), Lo.adapt_sparsity(So)
+    np.testing.assert_array_equal(R.indptr, Ro.indptr)
+    np.testing.assert_array_equal(R.indices, Ro.indices)
+    # read map = linear part of the oracle lift, column by column
+    for j in range(0, L.size_reduced, 7):
+        e = np.zeros(L.size_reduced)
+        e[j] = 1.0
+        np.testing.assert_array_equal(np.nonzero(Lo.lift_linear(e))[0],
+                                      np.nonzero(L.read_map == j)[0])
+
+
+def test_field_patterns_match_oracle():
+    """Displacement (2 dofs, all nodes), a scalar on a node subset and two
+    global dofs (pattern_from_compound's nodal + diagonal cases)."""
+    m = ta.Mesh.unit_square(5, 4)
+    N = m.n_nodes
+    u = np.arange(2 * N).reshape(N, 2)
+    sub = np.where(m.coords[:, 0] <= 0.4)[0]
+    p = np.full((N, 1), -1)
+    p[sub, 0] = 2 * N + np.arange(sub.size)
+    glob = np.array([2 * N + sub.size, 2 * N + sub.size + 1])
+    size = int(glob[-1]) + 1
+    S = sparse.pattern_from_fields(m, size, nodal=[u, p], diagonal=[glob])
+    ed = np.concatenate([u[m.elements].reshape(m.n_elements, -1),
+                         p[m.elements].reshape(m.n_elements, -1)], axis=1)
+    So = fem.pattern_from_element_dofs(ed, size, glob)
+    np.testing.assert_array_equal(S.indptr, So.indptr)
+    np.testing.assert_array_equal(S.indices, So.indices)
+    blocks = sparse.pattern_from_fields(m, size, nodal=[u, p], diagonal=[glob],
+                                        block_slices=[slice(0, 2 * N), slice(2 * N, size)])
+    assert blocks[0][1].shape == (2 * N, size - 2 * N)
+    assert (blocks[1][0] != So[2 * N:, :2 * N]).nnz == 0
+    # empty element table: diagonal only
+    D = sparse.pattern_from_element_dofs(np.zeros((0, 3), dtype=np.int32), 4, [1, 2])
+    np.testing.assert_array_equal(D.indptr, [0, 0, 1, 2, 2])
+    with pytest.raises(ValueError):
+        sparse.pattern_from_element_dofs(np.array([[0, 7]]), 5)

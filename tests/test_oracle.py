@@ -345,3 +345,85 @@ def test_tet4_generator_matches_reference_layout():
     np.testing.assert_allclose(w.sum(), 1.0, rtol=1e-14)
     Xj, _ = fem.box_tet4(4, 4, 4, jitter=0.2, seed=0)
     assert np.all(fem.integration_weights(fem.Tet4(), Xj, fem.box_tet4(4, 4, 4)[1]) > 0)
+
+
+# ------------------------------------------------------------- periodic ----
+
+def _periodic_square(n=5, nc=2):
+    """tests/test_sparse_tracer.py:162-216 layout: Fixed bottom, Periodic left <- right."""
+    X, E = fem.unit_square_tri3(n, n)
+    bot = np.where(X[:, 1] == 0.0)[0]
+    left = np.where(X[:, 0] == 0.0)[0]
+    left = left[np.argsort(X[left, 1])]
+    right = np.where(X[:, 0] == 1.0)[0]
+    right = right[np.argsort(X[right, 1])]
+    comp = np.arange(nc)
+    fixed = (bot[:, None] * nc + comp).ravel()
+    slaves = (left[:, None] * nc + comp).ravel()
+    masters = (right[:, None] * nc + comp).ravel()
+    return X, E, fixed, slaves, masters
+
+
+def test_lift_linear_is_the_derivative_of_lift():
+    X, E, fixed, slaves, masters = _periodic_square()
+    n = X.shape[0] * 2
+    L = fem.Lifter(n, fem.Fixed(fixed, 0.3), fem.Periodic(slaves, masters))
+    rng = np.random.default_rng(0)
+    u, v = rng.standard_normal(L.size_reduced), rng.standard_normal(L.size_reduced)
+    d = L.lift_from_zeros(u + v) - L.lift_from_zeros(u)
+    np.testing.assert_allclose(d, L.lift_linear(v), atol=1e-14)
+    # reduce_adjoint is the transpose of lift_linear
+    w = rng.standard_normal(n)
+    assert abs(L.lift_linear(v) @ w - v @ L.reduce_adjoint(w)) < 1e-12
+
+
+def test_periodic_augment_equals_pair_closure():
+    """M^T S M == {(a, b): (i, j) in S, a in {i, m(i)}, b in {j, m(j)}}."""
+    X, E, fixed, slaves, masters = _periodic_square(4)
+    S = fem.pattern_from_mesh(E, X.shape[0], 2)
+    aug = fem.Periodic(slaves, masters).augment(S).tocoo()
+    m = dict(zip(slaves.tolist(), masters.tolist()))
+    Sc = S.tocoo()
+    want = set()
+    for i, j in zip(Sc.row.tolist(), Sc.col.tolist()):
+        for a in {i, m.get(i, i)}:
+            for b in {j, m.get(j, j)}:
+                want.add((a, b))
+    assert set(zip(aug.row.tolist(), aug.col.tolist())) == want
+
+
+def test_periodic_newton_equals_direct_solve():
+    """Linear elasticity with a periodic pair of edges: the oracle's Newton on
+    the colored reduced Hessian equals spsolve of P^T K P u = -P^T r(g)."""
+    X, E, fixed, slaves, masters = _periodic_square(5)
+    top = np.where(X[:, 1] == 1.0)[0] * 2 + 1
+    prob = fem.Problem(fem.Tri3(), X, E, fem.LinearElasticity(0.4, 0.6))
+    L = fem.Lifter(X.shape[0] * 2, fem.Fixed(fixed, 0.0), fem.Fixed(top, 0.01),
+                   fem.Periodic(slaves, masters))
+    S = L.adapt_sparsity(fem.pattern_from_mesh(E, X.shape[0], 2))
+    colors = fem.distance2_colors_py(S.indptr, S.indices, S.shape[0])
+    u, its, cg, hist = fem.newton(prob, L, L.lift_from_zeros(np.zeros(L.size_reduced)), S, colors)
+    assert hist[-1] <= 1e-10 * hist[0]
+    P = np.stack([L.lift_linear(e) for e in np.eye(L.size_reduced)], axis=1)
+    K = _dense_hessian(prob, np.zeros(prob.n))
+    g = L.lift_from_zeros(np.zeros(L.size_reduced))
+    ur = np.linalg.solve(P.T @ K @ P, -P.T @ prob.residual(g))
+    np.testing.assert_allclose(u, L.lift_from_zeros(ur), atol=1e-12 * np.abs(u).max() + 1e-15)
+    # the periodic edges carry equal displacements
+    np.testing.assert_array_equal(u[slaves], u[masters])
+
+
+def test_element_dof_pattern_oracle():
+    """Pinned by brute force, and equal to pattern_from_mesh for a plain nodal field."""
+    X, E = fem.unit_square_tri3(3, 2)
+    nd = np.arange(X.shape[0] * 2).reshape(-1, 2)
+    S = fem.pattern_from_element_dofs(nd[E].reshape(E.shape[0], -1), X.shape[0] * 2)
+    Sm = fem.pattern_from_mesh(E, X.shape[0], 2)
+    assert (S != Sm).nnz == 0
+    rng = np.random.default_rng(1)
+    ed = rng.integers(-1, 9, size=(5, 4))
+    S = fem.pattern_from_element_dofs(ed, 11, diagonal_dofs=[10, 3])
+    want = {(a, b) for row in ed for a in row for b in row if a >= 0 and b >= 0}
+    want |= {(10, 10), (3, 3)}
+    Sc = S.tocoo()
+    assert set(zip(Sc.row.tolist(), Sc.col.tolist())) == want
