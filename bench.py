@@ -30,6 +30,12 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the node-block SpMV on the 64^3 line, from the two
+# rocprofv3 PMC passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
+# plain 512 MiB copy) recorded in profiles/k_pcg_bspmv_hex8_le_64cubed.md.
+SPMV_TRAFFIC_64 = {"fetch_kib": 276631.5, "fetch_factor": 2.000, "write_kib": 7453.9,
+                   "write_factor": 1.000}
+
 
 def build_problem(n):
     import numpy as np
@@ -292,8 +298,10 @@ def main(argv=None):
     ap.add_argument("--nh-n", type=int, default=48, help="cells per side, Neo-Hookean comparison")
     ap.add_argument("--nh-steps", type=int, default=2, help="load steps, Neo-Hookean comparison")
     ap.add_argument("--nh-stretch", type=float, default=0.1, help="final u_x on x = 1")
-    ap.add_argument("--workload", choices=("c2", "c3", "c5"), default="c2",
-                    help="c2: the default line; c3/c5: large mixed-vs-FP64 Neo-Hookean runs")
+    ap.add_argument("--workload", choices=("c2", "c2mixed", "c3", "c5", "nh48"), default="c2",
+                    help="c2: the default line; c2mixed / nh48: only the mixed-vs-FP64 "
+                         "comparison on the 64^3 elasticity / 48^3 Neo-Hookean cube; "
+                         "c3/c5: large mixed-vs-FP64 Neo-Hookean runs")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-qps", type=float, default=10.0)
     args = ap.parse_args(argv)
@@ -308,8 +316,26 @@ def main(argv=None):
     from tatva_amd import solvers
 
     torch.cuda.set_device(0)
-    if args.workload != "c2":
+    if args.workload in ("c3", "c5"):
         r = large_mixed(args.workload, oracle_check=not args.no_cpu_baseline)
+        print(json.dumps({"workload": args.workload, **r}), flush=True)
+        return 0
+    if args.workload == "c2mixed":
+        P = build_problem(args.n)
+        nd = P["mesh"].n_nodes * 3
+
+        def run_c2(sv):
+            uu = torch.zeros(nd, dtype=torch.float64, device="cuda")
+            return uu, [sv.solve(uu)]
+
+        r = mixed_vs_fp64(P["E"], P["L"], run_c2, colored_matrix=P["cm"])
+        print(json.dumps({"workload": args.workload, **r}), flush=True)
+        return 0
+    if args.workload == "nh48":
+        NH = stretch_problem(args.nh_n)
+        L0 = NH["L"].at["load"].set(NH["values"](0.0))
+        r = mixed_vs_fp64(NH["E"], L0,
+                          lambda sv: solve_load_steps(NH, sv, args.nh_steps, args.nh_stretch))
         print(json.dumps({"workload": args.workload, **r}), flush=True)
         return 0
     t_setup = time.perf_counter()
@@ -357,6 +383,10 @@ def main(argv=None):
         kernel = "k_pcg_spmv<LPR> (CSR SpMV fused with p.Ap)"
         spmv_bytes = 12 * nnz + 4 * (n_free + 1) + 16 * n_free
     achieved = spmv_bytes / spmv_s / 1e9 if op_samples else None
+    spmv_traffic = None
+    if blocked and args.n == 64:
+        t = SPMV_TRAFFIC_64
+        spmv_traffic = (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024
     converged = all(s.converged for s in stats)
 
     extras = {}
@@ -434,7 +464,9 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": spmv_traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes on this line, "
+                              "profiles/k_pcg_bspmv_hex8_le_64cubed.md" if spmv_traffic else None,
             "algorithmic_bytes": spmv_bytes,
             "avg_launch_us": spmv_s * 1e6 if op_samples else None,
             "samples": op_samples,

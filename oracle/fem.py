@@ -673,6 +673,51 @@ def pcg(A, b, x0=None, tol=1e-10, maxiter=100000, diag=None):
     return x, it, rel
 
 
+def pcg_single_reduction(A, b, x0=None, tol=1e-10, maxiter=100000, diag=None):
+    """Jacobi-preconditioned CG in the single-reduction form of Chronopoulos and
+    Gear (J. Comput. Appl. Math. 25, 1989): u = D^-1 r, w = A u,
+    gamma = r.u, delta = w.u, beta = gamma / gamma_old,
+    alpha = gamma / (delta - beta gamma / alpha_old), p = u + beta p,
+    s = w + beta s, x += alpha p, r -= alpha s.  Equal to ``pcg`` in exact
+    arithmetic; the convergence test is on r before each update.
+
+    Returns (x, iterations, relative residual of the recursive residual).
+    """
+    b = np.asarray(b, dtype=np.float64)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+    dinv = 1.0 / (A.diagonal() if diag is None else diag)
+    r = b - A @ x
+    bnorm = np.linalg.norm(b)
+    if bnorm == 0.0:
+        return x, 0, 0.0
+    u = dinv * r
+    gamma = r @ u
+    it = 0
+    p = s = None
+    alpha_old = gamma_old = None
+    while True:
+        rel = np.linalg.norm(r) / bnorm
+        if rel <= tol or it >= maxiter:
+            return x, it, rel
+        w = A @ u
+        delta = w @ u
+        if it == 0:
+            beta, denom = 0.0, delta
+            p, s = u.copy(), w.copy()
+        else:
+            beta = gamma / gamma_old
+            denom = delta - beta * gamma / alpha_old
+            p = u + beta * p
+            s = w + beta * s
+        alpha = gamma / denom
+        x += alpha * p
+        r = r - alpha * s
+        u = dinv * r
+        alpha_old, gamma_old = alpha, gamma
+        gamma = r @ u
+        it += 1
+
+
 def hessian_csr(prob: Problem, lifter: Lifter, u_full, pattern, colors):
     """Colored sparse Hessian of E_free(u) = E(lift(u)) on the reduced pattern."""
     def jvp(seed):

@@ -4,17 +4,18 @@
 // share the launches instead of queueing behind each other.
 //
 // Per iteration (tick t), with one class = slots of equal (element kind,
-// density kind, quadrature rule):
-//   per class  A  tile assembly of H(u_s) p_s into the slot's tile partials
-//   per class  G  slot gather -> Ap_s (reduced) and the block shares of p.Ap
-//   all slots  U  alpha, x += alpha p, r -= alpha Ap, shares of r.z and r.r
-//   all slots  P  convergence test, beta, p = D^-1 r + beta p
-// A block finds its slot through a per-block table (one load) and works on
-// that slot's arrays only; the slot's own blocks re-sum its partials across
-// the kernel boundary exactly as the single-solve PCG does.  Every slot runs
-// the same block decomposition whatever else is in the batch, so its
-// arithmetic, and its result, do not depend on the mix.  The parity of the
-// partial buffers is (t - start tick of the slot) mod 2.
+// density kind, quadrature rule, affine Hex8 flag):
+//   per class  A  tile assembly of H(u_s) z_s into the slot's tile partials
+//   per class  G  slot gather -> w_s = H z_s (reduced) and the shares of z.w
+//   all slots  S  the single-reduction CG step of each slot (the scalars from
+//                 its partials, p, s, x, r, z updates, shares of r.z and r.r)
+// with the recurrences of the single-solve PCG (solver_kernels.hip).  A block
+// finds its slot through a per-block table (one load) and works on that
+// slot's arrays only; the slot's own blocks re-sum its partials across the
+// kernel boundary.  Every slot runs the same block decomposition whatever else
+// is in the batch, so its arithmetic, and its result, do not depend on the
+// mix.  The parity of the partial buffers and scalars is (t - start tick of the
+// slot) mod 2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,7 +52,7 @@ struct BatchSlot {
   const int32_t *dof_map;  // full -> reduced (or NULL: identity)
   const float *u;          // FP32 full state (linearisation point)
   float *partial;          // tile partials, n_slots * ncomp
-  float *x, *r, *p, *Ap, *dinv;
+  float *x, *r, *z, *p, *s, *w, *dinv;
   double *pap, *rz0, *rz1, *rr0, *rr1, *bb;
   PcgState *st;
   int32_t g_blocks, v_blocks, start_tick;
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(kTile) void k_batch_assemble(const BatchSlot *__res
   if (S.st->done) return;
   extern __shared__ unsigned char lds_raw[];
   const Dens dens = make_density<Dens>(S.dens);
-  tile_assemble<KIND, Dens, float, kModeHvp, AFFINE>(b - S.a_blk0, S.coords, S.conn, quad, dens, S.u, S.p,
+  tile_assemble<KIND, Dens, float, kModeHvp, AFFINE>(b - S.a_blk0, S.coords, S.conn, quad, dens, S.u, S.z,
                                              S.dof_map, S.tiles, S.partial,
                                              reinterpret_cast<float *>(lds_raw));
 }
@@ -86,8 +87,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather(const BatchSlot *__rest
   if (S.st->done) return;
   const int lb = b - S.g_blk0;
   const int32_t *__restrict__ dof_map = S.dof_map;
-  const float *__restrict__ p = S.p;
-  float *__restrict__ Ap = S.Ap;
+  const float *__restrict__ z = S.z;
+  float *__restrict__ w = S.w;
   double dot = 0.0;
   for (int64_t i = (int64_t)lb * kBlock + threadIdx.x; i < S.n_nodes;
        i += (int64_t)S.g_blocks * kBlock) {
@@ -98,8 +99,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather(const BatchSlot *__rest
       const int64_t dof = i * NC + c;
       const int64_t r = dof_map ? (int64_t)dof_map[dof] : dof;
       if (r >= 0) {
-        Ap[r] = acc[c];
-        dot += (double)acc[c] * (double)p[r];
+        w[r] = acc[c];
+        dot += (double)acc[c] * (double)z[r];
       }
     }
   }
@@ -107,86 +108,77 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather(const BatchSlot *__rest
   if (threadIdx.x == 0) S.pap[lb] = dot;
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_update(const BatchSlot *__restrict__ slots,
-                                                         const int32_t *__restrict__ bmap,
-                                                         int tick) {
-  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
-  const BatchSlot &S = slots[bmap[b]];
+__global__ __launch_bounds__(kBlock) void k_batch_step(const BatchSlot *__restrict__ slots,
+                                                       const int32_t *__restrict__ bmap,
+                                                       int tick) {
+  const int blk = (int)xcd_block(blockIdx.x, gridDim.x);
+  const BatchSlot &S = slots[bmap[blk]];
   PcgState *st = S.st;
   if (st->done) return;
-  const int lb = b - S.v_blk0;
+  const int lb = blk - S.v_blk0;
   const int q = (tick - S.start_tick) & 1;
-  const int iter = st->iter;
-  const double pap = sum_partials(S.pap, S.g_blocks);
-  const double rz = sum_partials(q ? S.rz0 : S.rz1, S.v_blocks);
-  if (!(pap > 0.0) || !isfinite(pap) || iter >= st->maxiter) {
+  const int iter = st->it[q ^ 1];
+  const double gamma = sum_partials(q ? S.rz0 : S.rz1, S.v_blocks);
+  const double rr = sum_partials(q ? S.rr0 : S.rr1, S.v_blocks);
+  const double delta = sum_partials(S.pap, S.g_blocks);
+  const bool first = iter == 0;
+  const double beta = first ? 0.0 : gamma / st->gamma[q ^ 1];
+  const double denom = first ? delta : delta - beta * gamma / st->alpha[q ^ 1];
+  const bool converged = rr <= st->tol2;
+  const bool spent = iter >= st->maxiter;
+  const bool bad = !(denom > 0.0) || !isfinite(denom) || !isfinite(gamma) || !isfinite(rr);
+  if (converged || spent || bad) {
     if (lb == 0 && threadIdx.x == 0) {
       st->done = 1;
-      st->status = (iter >= st->maxiter) ? TATVA_ENOTCONV : TATVA_EBREAKDOWN;
+      st->rr = rr;
+      st->iter = iter;
+      st->status = converged ? TATVA_OK : spent ? TATVA_ENOTCONV : TATVA_EBREAKDOWN;
     }
     return;
   }
-  const float alpha = float(rz / pap);
+  const double alpha = gamma / denom;
+  if (lb == 0 && threadIdx.x == 0) {
+    st->alpha[q] = alpha;
+    st->gamma[q] = gamma;
+    st->it[q] = iter + 1;
+    st->iter = iter + 1;
+    st->rr = rr;
+  }
+  const float a = float(alpha), bt = float(beta);
   float *__restrict__ x = S.x;
   float *__restrict__ r = S.r;
-  const float *__restrict__ p = S.p;
-  const float *__restrict__ Ap = S.Ap;
+  float *__restrict__ z = S.z;
+  float *__restrict__ p = S.p;
+  float *__restrict__ s = S.s;
+  const float *__restrict__ w = S.w;
   const float *__restrict__ dinv = S.dinv;
-  double srz = 0.0, srr = 0.0;
+  double sg = 0.0, srr = 0.0;
   for (int64_t i = (int64_t)lb * kBlock + threadIdx.x; i < S.n; i += (int64_t)S.v_blocks * kBlock) {
-    x[i] += alpha * p[i];
-    const float ri = r[i] - alpha * Ap[i];
+    const float pi = first ? z[i] : z[i] + bt * p[i];
+    const float si = first ? w[i] : w[i] + bt * s[i];
+    p[i] = pi;
+    s[i] = si;
+    x[i] += a * pi;
+    const float ri = r[i] - a * si;
     r[i] = ri;
-    srz += (double)ri * (double)(dinv[i] * ri);
+    const float zi = dinv[i] * ri;
+    z[i] = zi;
+    sg += (double)ri * (double)zi;
     srr += (double)ri * (double)ri;
   }
-  srz = block_sum(srz);
+  sg = block_sum(sg);
   srr = block_sum(srr);
   if (threadIdx.x == 0) {
-    (q ? S.rz1 : S.rz0)[lb] = srz;
+    (q ? S.rz1 : S.rz0)[lb] = sg;
     (q ? S.rr1 : S.rr0)[lb] = srr;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_pupdate(const BatchSlot *__restrict__ slots,
-                                                          const int32_t *__restrict__ bmap,
-                                                          int tick) {
-  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
-  const BatchSlot &S = slots[bmap[b]];
-  PcgState *st = S.st;
-  if (st->done) return;
-  const int lb = b - S.v_blk0;
-  const int q = (tick - S.start_tick) & 1;
-  const double rz_old = sum_partials(q ? S.rz0 : S.rz1, S.v_blocks);
-  const double rz_new = sum_partials(q ? S.rz1 : S.rz0, S.v_blocks);
-  const double rr = sum_partials(q ? S.rr1 : S.rr0, S.v_blocks);
-  const bool converged = rr <= st->tol2;
-  const bool bad = !isfinite(rr) || !isfinite(rz_new);
-  if (lb == 0 && threadIdx.x == 0) {
-    st->iter += 1;
-    st->rr = rr;
-    if (converged) {
-      st->done = 1;
-      st->status = TATVA_OK;
-    } else if (bad) {
-      st->done = 1;
-      st->status = TATVA_EBREAKDOWN;
-    }
-  }
-  if (converged || bad) return;
-  const float beta = float(rz_new / rz_old);
-  float *__restrict__ p = S.p;
-  const float *__restrict__ r = S.r;
-  const float *__restrict__ dinv = S.dinv;
-  for (int64_t i = (int64_t)lb * kBlock + threadIdx.x; i < S.n; i += (int64_t)S.v_blocks * kBlock)
-    p[i] = dinv[i] * r[i] + beta * p[i];
-}
-
-// Per-slot workspace: p, r, Ap, dinv (n floats each), tile partials
+// Per-slot workspace: z, p, s, w, r, dinv (n floats each), tile partials
 // (n_slots * ncomp floats), the partial sums pap (gather blocks) and
 // rz0/rz1/rr0/rr1/bb (vector blocks).
 struct SlotLayout {
-  size_t p, r, Ap, dinv, partial, pap, rz0, rz1, rr0, rr1, bb, total;
+  size_t z, p, s, w, r, dinv, partial, pap, rz0, rz1, rr0, rr1, bb, total;
 };
 
 static size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -199,9 +191,11 @@ static SlotLayout slot_layout(const tatva_mesh *m, int ncomp, int64_t n) {
     o = al256(o + bytes);
     return at;
   };
+  L.z = take(n * sizeof(float));
   L.p = take(n * sizeof(float));
+  L.s = take(n * sizeof(float));
+  L.w = take(n * sizeof(float));
   L.r = take(n * sizeof(float));
-  L.Ap = take(n * sizeof(float));
   L.dinv = take(n * sizeof(float));
   L.partial = take((size_t)std::max<int64_t>(1, m->n_slots) * ncomp * sizeof(float));
   L.pap = take(gather_grid(m->n_nodes) * sizeof(double));
@@ -456,8 +450,10 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   S.partial = (float *)(w + L.partial);
   S.x = x;
   S.r = (float *)(w + L.r);
+  S.z = (float *)(w + L.z);
   S.p = (float *)(w + L.p);
-  S.Ap = (float *)(w + L.Ap);
+  S.s = (float *)(w + L.s);
+  S.w = (float *)(w + L.w);
   S.dinv = (float *)(w + L.dinv);
   S.pap = (double *)(w + L.pap);
   S.rz0 = (double *)(w + L.rz0);
@@ -470,7 +466,8 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   S.v_blocks = vec_grid(n);
   if (B->tick >= (int64_t)1 << 30) B->tick &= 1;  // keeps tick and start_tick in int range
   S.start_tick = (int32_t)B->tick;
-  const int rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, S.r, S.p, S.Ap, S.dinv, S.rz1,
+  // r = b - H x, z = D^-1 r, shares of r.z, r.r, b.b (w is scratch here)
+  const int rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, S.r, S.z, S.w, S.dinv, S.rz1,
                                    S.rr1, S.bb, S.v_blocks, tol, maxiter, S.st, s);
   if (rc) return rc;
   B->slots[slot] = S;
@@ -525,8 +522,7 @@ extern "C" int tatva_batch_iterate(tatva_batch *B, int32_t iters, void *stream) 
       });
       if (rc) return rc;
     }
-    k_batch_update<<<B->v_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + B->v_off, tick);
-    k_batch_pupdate<<<B->v_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + B->v_off, tick);
+    k_batch_step<<<B->v_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + B->v_off, tick);
     TATVA_CHECK_LAUNCH();
     B->tick += 1;
   }

@@ -20,6 +20,11 @@ struct PcgState {
   int done;
   int status;
   int maxiter;
+  // single-reduction CG: alpha_i, gamma_i and the iteration count written by
+  // the step kernel of parity q = i mod 2 and read at parity q ^ 1
+  double alpha[2];
+  double gamma[2];
+  int it[2];
 };
 
 // Vector kernels use at most this many blocks (4 per CU): the partials each

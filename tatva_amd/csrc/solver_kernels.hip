@@ -1,19 +1,28 @@
 // CSR SpMV, Jacobi-PCG (FP64 CSR and FP32 matrix-free) with hipGraph-captured
 // iteration chunks, and the small vector kernels of the Newton loop.
 //
-// One PCG iteration i = three kernels (four with the matrix-free operator):
-//   K1  Ap = A p; each block writes its share of p.Ap            -> part_pap
-//   K2  every block re-sums part_pap and part_rz[q^1] (fixed order) to get
-//       alpha = rz_i / p.Ap; x += alpha p, r -= alpha Ap; shares of
-//       r.(D^-1 r) and r.r                                    -> part_rz[q], part_rr[q]
-//   K3  every block re-sums part_rz[q], part_rz[q^1], part_rr[q]: convergence
-//       test, beta = rz_{i+1}/rz_i, p = D^-1 r + beta p
+// The single-solve PCG runs the single-reduction (Chronopoulos-Gear) form of
+// Jacobi-preconditioned CG: with u = D^-1 r and w = A u,
+//   gamma_i = r_i.u_i, delta_i = w_i.u_i, beta_i = gamma_i / gamma_{i-1},
+//   alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1}),
+//   p_i = u_i + beta_i p_{i-1}, s_i = w_i + beta_i s_{i-1}  (s_i = A p_i),
+//   x_{i+1} = x_i + alpha_i p_i, r_{i+1} = r_i - alpha_i s_i,
+// which equals standard PCG in exact arithmetic and needs one cross-block sum
+// per iteration.  One iteration i = two kernels (three with the matrix-free
+// operator):
+//   KA  w = A u; each block writes its share of w.u            -> part_delta
+//   KB  every block re-sums part_delta and the shares of r.u, r.r of the
+//       previous KB (fixed order) to get the scalars, tests convergence on
+//       r_i, updates p, s, x, r, u and writes the shares of r.u and r.r of
+//       r_{i+1}                                                 -> part_rz[q], part_rr[q]
 // with q = i mod 2 (a kernel argument; chunks have an even length).  There is
 // no in-kernel fence or atomic: cross-block sums cross a kernel boundary, and
 // every block computes identical scalars, so the iteration is deterministic.
-// Block 0 alone writes the state record (done/status/iter/rr) that later
-// kernels test and the host reads between chunks; each kernel returns at once
-// when the state says done, so a chunk may overrun convergence harmlessly.
+// Block 0 alone writes the state record; scalars a kernel reads are written
+// by the other parity, so no block reads a value block 0 of the same kernel
+// writes.  Each kernel returns at once when the state says done, so a chunk
+// may overrun convergence harmlessly.  The batched solver (batch_kernels.hip)
+// keeps the three-kernel standard form (update, then p update).
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -314,6 +323,71 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict
     p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
+// KB of the single-reduction form (see the file comment).  u is D^-1 r.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x, T *__restrict__ r,
+                                                    T *__restrict__ u, T *__restrict__ p,
+                                                    T *__restrict__ s, const T *__restrict__ w,
+                                                    const T *__restrict__ dinv,
+                                                    const double *__restrict__ part_delta,
+                                                    int n_delta,
+                                                    const double *__restrict__ part_g_old,
+                                                    const double *__restrict__ part_rr_old,
+                                                    double *__restrict__ part_g_new,
+                                                    double *__restrict__ part_rr_new, int q,
+                                                    PcgState *__restrict__ st) {
+  if (st->done) return;
+  const int iter = st->it[q ^ 1];
+  const double gamma = sum_partials(part_g_old, gridDim.x);
+  const double rr = sum_partials(part_rr_old, gridDim.x);
+  const double delta = sum_partials(part_delta, n_delta);
+  const bool first = iter == 0;
+  const double beta = first ? 0.0 : gamma / st->gamma[q ^ 1];
+  const double denom = first ? delta : delta - beta * gamma / st->alpha[q ^ 1];
+  const bool converged = rr <= st->tol2;
+  const bool spent = iter >= st->maxiter;
+  const bool bad = !(denom > 0.0) || !isfinite(denom) || !isfinite(gamma) || !isfinite(rr);
+  if (converged || spent || bad) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->rr = rr;
+      st->iter = iter;
+      st->status = converged ? TATVA_OK : spent ? TATVA_ENOTCONV : TATVA_EBREAKDOWN;
+    }
+    return;
+  }
+  const double alpha = gamma / denom;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha[q] = alpha;
+    st->gamma[q] = gamma;
+    st->it[q] = iter + 1;
+    st->iter = iter + 1;
+    st->rr = rr;
+  }
+  const T a = T(alpha), bt = T(beta);
+  double sg = 0.0, srr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T pi = first ? u[i] : u[i] + bt * p[i];
+    const T si = first ? w[i] : w[i] + bt * s[i];
+    p[i] = pi;
+    s[i] = si;
+    x[i] += a * pi;
+    const T ri = r[i] - a * si;
+    r[i] = ri;
+    const T ui = dinv[i] * ri;
+    u[i] = ui;
+    sg += (double)ri * (double)ui;
+    srr += (double)ri * (double)ri;
+  }
+  sg = block_sum(sg);
+  srr = block_sum(srr);
+  if (threadIdx.x == 0) {
+    part_g_new[blockIdx.x] = sg;
+    part_rr_new[blockIdx.x] = srr;
+  }
+}
+
 // r given: z = D^-1 r, p = z, shares of r.z (into the parity-1 slot, read by
 // iteration 0 as "previous"), r.r and b.b.
 template <class T>
@@ -357,6 +431,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_state(const double *__restr
     st->bb = bb;
     st->tol2 = tol * tol * bb;
     st->iter = 0;
+    st->it[0] = 0;
+    st->it[1] = 0;
     st->maxiter = maxiter;
     int done = 1, status = TATVA_OK;
     if (bad != TATVA_OK) {
@@ -531,7 +607,7 @@ static int spmv_grid(int64_t n, int lpr) {
 }
 
 struct WsLayout {
-  size_t vec[8];
+  size_t vec[12];
   size_t parts;
   size_t state;
   size_t total;
@@ -710,7 +786,12 @@ extern "C" int tatva_spmv_bcsr(int64_t n, const int32_t *indptr, const int32_t *
 }
 
 // workspace vectors: p, r, Ap, D^-1, x, b
-extern "C" int64_t tatva_pcg_csr_workspace(int64_t n) { return (int64_t)ws_layout(n, 6, sizeof(double)).total; }
+// workspace: u, w, p, s, r, D^-1, x, b (FP64), partials, state
+constexpr int kCsrVecs = 8;
+
+extern "C" int64_t tatva_pcg_csr_workspace(int64_t n) {
+  return (int64_t)ws_layout(n, kCsrVecs, sizeof(double)).total;
+}
 
 extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
                              const int32_t *indices, const double *data,
@@ -724,7 +805,7 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
     TATVA_REQUIRE((block_size == 2 || block_size == 3) && n % block_size == 0,
                   "block view needs block_size 2 or 3 dividing n");
   TATVA_REQUIRE(tol >= 0.0, "tol must be non-negative");
-  const WsLayout L = ws_layout(n, 6, sizeof(double));
+  const WsLayout L = ws_layout(n, kCsrVecs, sizeof(double));
   TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
   chunk += chunk & 1;  // parity ping-pong needs an even chunk
   int device = 0;
@@ -733,14 +814,16 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   int rc = get_res(device, &res);
   if (rc) return rc;
   std::lock_guard<std::mutex> lock(res->mu);
-  char *w = (char *)workspace;
-  double *p = (double *)(w + L.vec[0]);
-  double *r = (double *)(w + L.vec[1]);
-  double *Ap = (double *)(w + L.vec[2]);
-  double *dinv = (double *)(w + L.vec[3]);
-  double *x = (double *)(w + L.vec[4]);
-  double *b = (double *)(w + L.vec[5]);
-  const PcgParts P = parts_at(w, L);
+  char *wsb = (char *)workspace;
+  double *u = (double *)(wsb + L.vec[0]);
+  double *w = (double *)(wsb + L.vec[1]);
+  double *p = (double *)(wsb + L.vec[2]);
+  double *sv = (double *)(wsb + L.vec[3]);
+  double *r = (double *)(wsb + L.vec[4]);
+  double *dinv = (double *)(wsb + L.vec[5]);
+  double *x = (double *)(wsb + L.vec[6]);
+  double *b = (double *)(wsb + L.vec[7]);
+  const PcgParts P = parts_at(wsb, L);
   PcgState *st = (PcgState *)(w + L.state);
   hipStream_t caller = (hipStream_t)stream;
   hipStream_t s = res->cap;
@@ -768,14 +851,15 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   };
+  // KA: w = A u and the shares of w.u
   auto apply_op_dot = [&]() -> int {
     if (nblk == 3)
-      k_pcg_bspmv<3><<<g_op, kBlock, 0, s>>>(n / 3, indptr, block_cols, data, p, Ap, P.pap, st);
+      k_pcg_bspmv<3><<<g_op, kBlock, 0, s>>>(n / 3, indptr, block_cols, data, u, w, P.pap, st);
     else if (nblk == 2)
-      k_pcg_bspmv<2><<<g_op, kBlock, 0, s>>>(n / 2, indptr, block_cols, data, p, Ap, P.pap, st);
+      k_pcg_bspmv<2><<<g_op, kBlock, 0, s>>>(n / 2, indptr, block_cols, data, u, w, P.pap, st);
     else
       with_lpr(lpr, [&](auto lc) -> int {
-        k_pcg_spmv<decltype(lc)::value><<<g_op, kBlock, 0, s>>>(n, indptr, indices, data, p, Ap,
+        k_pcg_spmv<decltype(lc)::value><<<g_op, kBlock, 0, s>>>(n, indptr, indices, data, u, w,
                                                                 P.pap, st);
         return 0;
       });
@@ -786,7 +870,8 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   k_csr_diag_inv<<<grid_for(n), kBlock, 0, s>>>(n, indptr, indices, data, dinv, st);
   rc = apply_op(x, b, r);  // r = b - A x
   if (rc) return rc;
-  k_pcg_init<double><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, P.rz(1), P.rr(1), P.bb);
+  // u = D^-1 r and the shares of r.u, r.r, b.b (parity 1: read by iteration 0)
+  k_pcg_init<double><<<gv, kBlock, 0, s>>>(n, r, dinv, b, u, P.rz(1), P.rr(1), P.bb);
   k_pcg_init_state<<<1, kBlock, 0, s>>>(P.rr(1), P.bb, gv, tol, maxiter, st);
   TATVA_CHECK_LAUNCH();
   const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
@@ -794,9 +879,8 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
       const int e = apply_op_dot();
       if (e) return e;
     }
-    k_pcg_update<double><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, P.pap, g_op, P.rz(q ^ 1),
-                                               P.rz(q), P.rr(q), st);
-    k_pcg_pupdate<double><<<gv, kBlock, 0, s>>>(n, p, r, dinv, P.rz(q ^ 1), P.rz(q), P.rr(q), st);
+    k_cg_step<double><<<gv, kBlock, 0, s>>>(n, x, r, u, p, sv, w, dinv, P.pap, g_op, P.rz(q ^ 1),
+                                            P.rr(q ^ 1), P.rz(q), P.rr(q), q, st);
     return TATVA_OK;
   };
   GraphKey key{};
@@ -843,7 +927,11 @@ int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
 
 }  // namespace tatva
 
-static size_t matfree_u_offset(int64_t n) { return ws_layout(n, 7, sizeof(float)).total; }
+// workspace: reduced FP32 vectors z (= D^-1 r), w, p, s, r, D^-1, x, b, D, the
+// partials and the state, then the full FP32 state u
+constexpr int kMatfreeVecs = 9;
+
+static size_t matfree_u_offset(int64_t n) { return ws_layout(n, kMatfreeVecs, sizeof(float)).total; }
 
 extern "C" int64_t tatva_pcg_matfree_workspace(tatva_mesh *m, int64_t n) {
   if (!m || n <= 0) return -1;
@@ -858,7 +946,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
                                      tatva_pcg_stats *stats, void *stream) {
   TATVA_REQUIRE(m && d && u_in && diag_in && b_in && x_io && n > 0, "invalid arguments");
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
-  const WsLayout L = ws_layout(n, 7, sizeof(float));
+  const WsLayout L = ws_layout(n, kMatfreeVecs, sizeof(float));
   TATVA_REQUIRE(workspace && ws_bytes >= tatva_pcg_matfree_workspace(m, n),
                 "PCG workspace too small");
   const int64_t n_full = m->n_nodes * density_ncomp(m, d);
@@ -868,16 +956,18 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   int rc = get_res(m->device, &res);
   if (rc) return rc;
   std::lock_guard<std::mutex> lock(res->mu);
-  char *w = (char *)workspace;
-  float *p = (float *)(w + L.vec[0]);
-  float *r = (float *)(w + L.vec[1]);
-  float *Ap = (float *)(w + L.vec[2]);
-  float *dinv = (float *)(w + L.vec[3]);
-  float *x = (float *)(w + L.vec[4]);
-  float *b = (float *)(w + L.vec[5]);
-  float *diag = (float *)(w + L.vec[6]);
-  float *u = (float *)(w + matfree_u_offset(n));
-  const PcgParts P = parts_at(w, L);
+  char *wsb = (char *)workspace;
+  float *z = (float *)(wsb + L.vec[0]);
+  float *w = (float *)(wsb + L.vec[1]);
+  float *p = (float *)(wsb + L.vec[2]);
+  float *sv = (float *)(wsb + L.vec[3]);
+  float *r = (float *)(wsb + L.vec[4]);
+  float *dinv = (float *)(wsb + L.vec[5]);
+  float *x = (float *)(wsb + L.vec[6]);
+  float *b = (float *)(wsb + L.vec[7]);
+  float *diag = (float *)(wsb + L.vec[8]);
+  float *u = (float *)(wsb + matfree_u_offset(n));
+  const PcgParts P = parts_at(wsb, L);
   PcgState *st = (PcgState *)(w + L.state);
   hipStream_t caller = (hipStream_t)stream;
   hipStream_t s = res->cap;
@@ -889,21 +979,22 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   TATVA_TRY_HIP(hipMemcpyAsync(u, u_in, n_full * sizeof(float), hipMemcpyDeviceToDevice, s));
   const int gv = vec_grid(n);
   const bool profiling = g_profiling.load() != 0;
-  rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, r, p, Ap, dinv, P.rz(1), P.rr(1), P.bb,
+  // r = b - H x, z = D^-1 r, shares of r.z, r.r, b.b (w is scratch here)
+  rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, r, z, w, dinv, P.rz(1), P.rr(1), P.bb,
                          gv, tol, maxiter, st, s);
   if (rc) return rc;
   const int g_op = hvp_gather_blocks(m);
+  // KA: w = H z with the shares of z.w
   const std::function<int()> apply_op = [&]() -> int {
-    return launch_hvp_dot_f32(m, d, u, p, dof_map, Ap, P.pap, s, &st->done);
+    return launch_hvp_dot_f32(m, d, u, z, dof_map, w, P.pap, s, &st->done);
   };
   const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
     if (with_op) {
       const int e = apply_op();
       if (e) return e;
     }
-    k_pcg_update<float><<<gv, kBlock, 0, s>>>(n, x, r, p, Ap, dinv, P.pap, g_op, P.rz(q ^ 1),
-                                              P.rz(q), P.rr(q), st);
-    k_pcg_pupdate<float><<<gv, kBlock, 0, s>>>(n, p, r, dinv, P.rz(q ^ 1), P.rz(q), P.rr(q), st);
+    k_cg_step<float><<<gv, kBlock, 0, s>>>(n, x, r, z, p, sv, w, dinv, P.pap, g_op, P.rz(q ^ 1),
+                                           P.rr(q ^ 1), P.rz(q), P.rr(q), q, st);
     return TATVA_OK;
   };
   GraphKey key{};

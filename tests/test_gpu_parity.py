@@ -308,6 +308,9 @@ def test_pcg_matches_spsolve_and_oracle_iterations():
     Kh = K.to_csr()
     xo, ito, relo = fem.pcg(Kh, b, tol=1e-10)
     assert abs(info.iterations - ito) <= 2
+    # the GPU solver runs the single-reduction form: same recurrence as the oracle's
+    x1, it1, rel1 = fem.pcg_single_reduction(Kh, b, tol=1e-10)
+    assert abs(info.iterations - it1) <= 1
     xs = spla.spsolve(Kh.tocsc(), b)
     assert rel_err(to_np(x), xs) < 1e-8
     true_rel = np.linalg.norm(b - Kh @ to_np(x)) / np.linalg.norm(b)

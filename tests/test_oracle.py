@@ -333,6 +333,18 @@ def test_pcg_matches_spsolve():
     x, it, rel = fem.pcg(K, b, tol=1e-12)
     np.testing.assert_allclose(x, spla.spsolve(K.tocsc(), b), rtol=1e-9, atol=1e-12)
     assert rel <= 1e-12
+    # single-reduction form: same solution, same iteration count up to rounding
+    x1, it1, rel1 = fem.pcg_single_reduction(K, b, tol=1e-12)
+    np.testing.assert_allclose(x1, spla.spsolve(K.tocsc(), b), rtol=1e-9, atol=1e-12)
+    assert rel1 <= 1e-12 and abs(it1 - it) <= 1
+    # the first iterates coincide with standard PCG (exact-arithmetic identity)
+    for k in (1, 2, 5):
+        xa, _, _ = fem.pcg(K, b, tol=0.0, maxiter=k)
+        xb, _, _ = fem.pcg_single_reduction(K, b, tol=0.0, maxiter=k)
+        np.testing.assert_allclose(xb, xa, rtol=1e-10, atol=1e-13 * np.abs(xa).max())
+    # zero right-hand side and the iteration limit
+    assert fem.pcg_single_reduction(K, np.zeros_like(b))[1] == 0
+    assert fem.pcg_single_reduction(K, b, tol=1e-12, maxiter=3)[1] == 3
 
 
 def test_tet4_generator_matches_reference_layout():
