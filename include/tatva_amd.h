@@ -171,6 +171,20 @@ int tatva_colored_hessian_mapped(tatva_mesh *mesh, const tatva_density *density,
                                  const int32_t *indices, int64_t nnz, int32_t flags, double *data,
                                  void *workspace, int64_t workspace_bytes, void *stream);
 
+/* L2 projection (Operator.project, tatva/operator.py:523-560 and the mass /
+ * right-hand-side forms of tatva/utils.py:232-262).  tatva_mass_matrix writes
+ * the consistent mass matrix of an ncomp-component field (block diagonal over
+ * components, sum_q N_a N_b detJ w) into CSR data on a (reduced) pattern, as
+ * tatva_colored_hessian does; tatva_project_rhs writes
+ * out[node, c] = sum_e sum_q N_a(xi_q) vals[e, q, c] detJ w_q (full nodes). */
+int tatva_mass_matrix(tatva_mesh *mesh, int32_t ncomp, const int32_t *dof_map,
+                      const int32_t *red_to_full, int64_t n_rows, const int32_t *colors,
+                      int32_t n_colors, const int32_t *indptr, const int32_t *indices,
+                      int64_t nnz, double *data, void *workspace, int64_t workspace_bytes,
+                      void *stream);
+int tatva_project_rhs(tatva_mesh *mesh, int32_t nc, const double *quad_values, double *out,
+                      void *stream);
+
 /* rows[k] = row of entry k and col_colors[k] = colors[indices[k]] for a CSR
  * pattern (compute_rows_cols, tatva/sparse/base.py:108-136). */
 int tatva_rows_cols(int64_t n, const int32_t *indptr, const int32_t *indices,

@@ -673,6 +673,32 @@ def pcg(A, b, x0=None, tol=1e-10, maxiter=100000, diag=None):
     return x, it, rel
 
 
+def project(elem, coords, elements, field_q, lifter=None):
+    """L2 projection (operator.py:523-560, utils.py:118-262) with the scalar
+    mass matrix and one right-hand side per component: M_ab = sum N_a N_b w,
+    b_a = sum N_a f w; with a lifter, rows/columns of the free dofs, solution
+    lifted from zeros.  Dense assembly; for small meshes."""
+    w = integration_weights(elem, coords, elements)                 # (E, Q)
+    Nq = np.stack([elem.N(xi) for xi in elem.quad_points])         # (Q, nne)
+    n = coords.shape[0]
+    M = np.zeros((n, n))
+    Me = np.einsum("eq,qa,qb->eab", w, Nq, Nq)
+    for e, el in enumerate(np.asarray(elements)):
+        M[np.ix_(el, el)] += Me[e]
+    f = np.asarray(field_q, dtype=np.float64)
+    vs = f.shape[2:]
+    f2 = f.reshape(f.shape[0], f.shape[1], -1)
+    b = np.zeros((n, f2.shape[2]))
+    np.add.at(b, np.asarray(elements), np.einsum("eq,qa,eqc->eac", w, Nq, f2))
+    if lifter is None:
+        x = np.linalg.solve(M, b)
+    else:
+        fr = lifter.free
+        xr = np.linalg.solve(M[np.ix_(fr, fr)], b[fr])
+        x = np.stack([lifter.lift_from_zeros(xr[:, c]) for c in range(xr.shape[1])], axis=1)
+    return x.reshape((n,) + vs)
+
+
 def pcg_single_reduction(A, b, x0=None, tol=1e-10, maxiter=100000, diag=None):
     """Jacobi-preconditioned CG in the single-reduction form of Chronopoulos and
     Gear (J. Comput. Appl. Math. 25, 1989): u = D^-1 r, w = A u,

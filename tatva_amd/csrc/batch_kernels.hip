@@ -437,7 +437,7 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   std::lock_guard<std::mutex> lock(B->mu);
   TATVA_TRY_HIP(hipSetDevice(B->device));
   hipStream_t s = (hipStream_t)stream;
-  char *w = (char *)workspace;
+  char *base = (char *)workspace;
   BatchSlot S{};
   S.coords = m->coords;
   S.conn = m->conn;
@@ -447,20 +447,20 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   S.dens = *d;
   S.dof_map = dof_map;
   S.u = u;
-  S.partial = (float *)(w + L.partial);
+  S.partial = (float *)(base + L.partial);
   S.x = x;
-  S.r = (float *)(w + L.r);
-  S.z = (float *)(w + L.z);
-  S.p = (float *)(w + L.p);
-  S.s = (float *)(w + L.s);
-  S.w = (float *)(w + L.w);
-  S.dinv = (float *)(w + L.dinv);
-  S.pap = (double *)(w + L.pap);
-  S.rz0 = (double *)(w + L.rz0);
-  S.rz1 = (double *)(w + L.rz1);
-  S.rr0 = (double *)(w + L.rr0);
-  S.rr1 = (double *)(w + L.rr1);
-  S.bb = (double *)(w + L.bb);
+  S.r = (float *)(base + L.r);
+  S.z = (float *)(base + L.z);
+  S.p = (float *)(base + L.p);
+  S.s = (float *)(base + L.s);
+  S.w = (float *)(base + L.w);
+  S.dinv = (float *)(base + L.dinv);
+  S.pap = (double *)(base + L.pap);
+  S.rz0 = (double *)(base + L.rz0);
+  S.rz1 = (double *)(base + L.rz1);
+  S.rr0 = (double *)(base + L.rr0);
+  S.rr1 = (double *)(base + L.rr1);
+  S.bb = (double *)(base + L.bb);
   S.st = B->d_states + slot;
   S.g_blocks = gather_grid(m->n_nodes);
   S.v_blocks = vec_grid(n);

@@ -447,6 +447,75 @@ __global__ __launch_bounds__(kBlock) void k_colored_rows_mapped(
   for (int32_t k = k0 + lane; k < k1; k += kWave) data[k] = J[colors[indices[k]]];
 }
 
+// Element mass matrix of an NC-component field: Ke[(a,c),(b,c')] =
+// delta_cc' sum_q N_a N_b detJ w (the Hessian of 1/2 |u|^2 integrated), in the
+// layout of k_elem_stiffness.
+template <int KIND, int NC>
+__global__ __launch_bounds__(kBlock) void k_mass_stiffness(int64_t n_elem, const double *__restrict__ coords,
+                                                           const int32_t *__restrict__ conn,
+                                                           QuadRule quad, double *__restrict__ Ke) {
+  using El = Elem<KIND>;
+  constexpr int D = El::D, NNE = El::NNE, ND = NNE * NC;
+  const int64_t e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  int32_t nodes[NNE];
+  double X[NNE][D];
+  load_element<KIND>(e, conn, coords, nodes, X);
+  double M[NNE][NNE];
+#pragma unroll
+  for (int a = 0; a < NNE; ++a)
+#pragma unroll
+    for (int b = 0; b < NNE; ++b) M[a][b] = 0.0;
+  for (int q = 0; q < quad.nq; ++q) {
+    const double w = qp_detJ<KIND>(X, quad.xi[q]) * quad.w[q];
+    double N[NNE];
+    El::N(quad.xi[q], N);
+#pragma unroll
+    for (int a = 0; a < NNE; ++a)
+#pragma unroll
+      for (int b = 0; b < NNE; ++b) M[a][b] += w * N[a] * N[b];
+  }
+  double *out = Ke + e * ND * ND;
+#pragma unroll
+  for (int a = 0; a < NNE; ++a)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int b = 0; b < NNE; ++b)
+#pragma unroll
+        for (int c2 = 0; c2 < NC; ++c2) out[(a * NC + c) * ND + b * NC + c2] = c == c2 ? M[a][b] : 0.0;
+}
+
+// out[i, c] = sum over the elements e of node i (local index a, ascending e)
+// of sum_q N_a(xi_q) vals[e, q, c] detJ w_q.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_project_rhs(int64_t n_nodes, const double *__restrict__ coords,
+                                                        const int32_t *__restrict__ conn,
+                                                        const int32_t *__restrict__ inv_ptr,
+                                                        const int32_t *__restrict__ inv_list,
+                                                        QuadRule quad, int nc,
+                                                        const double *__restrict__ vals,
+                                                        double *__restrict__ out) {
+  using El = Elem<KIND>;
+  constexpr int D = El::D, NNE = El::NNE;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  for (int c = 0; c < nc; ++c) out[i * nc + c] = 0.0;
+  for (int32_t p = inv_ptr[i]; p < inv_ptr[i + 1]; ++p) {
+    const int64_t e = inv_list[p] >> 3;
+    const int a = inv_list[p] & 7;
+    int32_t nodes[NNE];
+    double X[NNE][D];
+    load_element<KIND>(e, conn, coords, nodes, X);
+    for (int q = 0; q < quad.nq; ++q) {
+      double N[NNE];
+      El::N(quad.xi[q], N);
+      const double w = qp_detJ<KIND>(X, quad.xi[q]) * quad.w[q] * N[a];
+      for (int c = 0; c < nc; ++c) out[i * nc + c] += w * vals[(e * quad.nq + q) * nc + c];
+    }
+  }
+}
+
 }  // namespace tatva
 
 using namespace tatva;
@@ -1051,5 +1120,61 @@ extern "C" int tatva_colored_hessian_mapped(tatva_mesh *m, const tatva_density *
       TATVA_CHECK_LAUNCH();
       return TATVA_OK;
     });
+  });
+}
+
+extern "C" int tatva_mass_matrix(tatva_mesh *m, int32_t ncomp, const int32_t *dof_map,
+                                 const int32_t *red_to_full, int64_t n_rows,
+                                 const int32_t *colors, int32_t n_colors, const int32_t *indptr,
+                                 const int32_t *indices, int64_t nnz, double *data,
+                                 void *workspace, int64_t ws_bytes, void *stream) {
+  MESH_PRELUDE(m);
+  TATVA_REQUIRE(colors && indptr && indices && data, "invalid arguments");
+  TATVA_REQUIRE(ncomp >= 1 && ncomp <= 3, "ncomp must be in [1, 3]");
+  TATVA_REQUIRE(n_colors >= 1 && n_colors <= kMaxColors, "n_colors must be in [1, 2048]");
+  TATVA_REQUIRE((dof_map == nullptr) == (red_to_full == nullptr),
+                "dof_map and red_to_full must be given together");
+  if (!dof_map) TATVA_REQUIRE(n_rows == m->n_nodes * ncomp, "pattern size does not match the mesh dofs");
+  TATVA_REQUIRE(workspace && ws_bytes >= tatva_colored_hessian_workspace(m, ncomp),
+                "mass matrix workspace too small");
+  (void)nnz;
+  hipStream_t s = (hipStream_t)stream;
+  double *Ke = (double *)workspace;
+  const int rows_per_block = kBlock / kWave;
+  const size_t lds = (size_t)rows_per_block * n_colors * sizeof(double);
+  const int64_t nblk = (n_rows + rows_per_block - 1) / rows_per_block;
+  TATVA_REQUIRE(nblk < (int64_t(1) << 31), "too many rows");
+  return with_kind(m->kind, [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    constexpr int NNE = Elem<K>::NNE;
+    auto run = [&](auto ncc) -> int {
+      constexpr int NC = decltype(ncc)::value;
+      k_mass_stiffness<K, NC><<<grid_for(m->n_elements), kBlock, 0, s>>>(m->n_elements, m->coords,
+                                                                           m->conn, m->quad, Ke);
+      if (n_rows > 0)
+        k_colored_rows<NNE, NC><<<(int)nblk, kBlock, lds, s>>>(
+            n_rows, red_to_full, dof_map, colors, n_colors, indptr, indices, m->inv_ptr,
+            m->inv_list, m->conn, Ke, data);
+      TATVA_CHECK_LAUNCH();
+      return TATVA_OK;
+    };
+    if (ncomp == 1) return run(std::integral_constant<int, 1>{});
+    if (ncomp == 2) return run(std::integral_constant<int, 2>{});
+    return run(std::integral_constant<int, 3>{});
+  });
+}
+
+extern "C" int tatva_project_rhs(tatva_mesh *m, int32_t nc, const double *quad_values,
+                                 double *out, void *stream) {
+  MESH_PRELUDE(m);
+  TATVA_REQUIRE(quad_values && out && nc >= 1, "invalid arguments");
+  hipStream_t s = (hipStream_t)stream;
+  return with_kind(m->kind, [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    k_project_rhs<K><<<grid_for(m->n_nodes), kBlock, 0, s>>>(m->n_nodes, m->coords, m->conn,
+                                                            m->inv_ptr, m->inv_list, m->quad, nc,
+                                                            quad_values, out);
+    TATVA_CHECK_LAUNCH();
+    return TATVA_OK;
   });
 }

@@ -631,6 +631,16 @@ static WsLayout ws_layout(int64_t n, int nvec, size_t esz) {
   return L;
 }
 
+// Typed views of a workspace; base is the workspace's first byte.
+template <class T>
+static T *ws_vec(char *base, const WsLayout &L, int i) {
+  return reinterpret_cast<T *>(base + L.vec[i]);
+}
+
+static PcgState *ws_state(char *base, const WsLayout &L) {
+  return reinterpret_cast<PcgState *>(base + L.state);
+}
+
 static PcgParts parts_at(char *w, const WsLayout &L) {
   double *b = (double *)(w + L.parts);
   return PcgParts{b, b + kMaxReduceBlocks, b + 2 * kMaxReduceBlocks, b + 3 * kMaxReduceBlocks,
@@ -815,16 +825,16 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   if (rc) return rc;
   std::lock_guard<std::mutex> lock(res->mu);
   char *wsb = (char *)workspace;
-  double *u = (double *)(wsb + L.vec[0]);
-  double *w = (double *)(wsb + L.vec[1]);
-  double *p = (double *)(wsb + L.vec[2]);
-  double *sv = (double *)(wsb + L.vec[3]);
-  double *r = (double *)(wsb + L.vec[4]);
-  double *dinv = (double *)(wsb + L.vec[5]);
-  double *x = (double *)(wsb + L.vec[6]);
-  double *b = (double *)(wsb + L.vec[7]);
+  double *u = ws_vec<double>(wsb, L, 0);
+  double *w = ws_vec<double>(wsb, L, 1);
+  double *p = ws_vec<double>(wsb, L, 2);
+  double *sv = ws_vec<double>(wsb, L, 3);
+  double *r = ws_vec<double>(wsb, L, 4);
+  double *dinv = ws_vec<double>(wsb, L, 5);
+  double *x = ws_vec<double>(wsb, L, 6);
+  double *b = ws_vec<double>(wsb, L, 7);
   const PcgParts P = parts_at(wsb, L);
-  PcgState *st = (PcgState *)(w + L.state);
+  PcgState *st = ws_state(wsb, L);
   hipStream_t caller = (hipStream_t)stream;
   hipStream_t s = res->cap;
   TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
@@ -957,18 +967,18 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   if (rc) return rc;
   std::lock_guard<std::mutex> lock(res->mu);
   char *wsb = (char *)workspace;
-  float *z = (float *)(wsb + L.vec[0]);
-  float *w = (float *)(wsb + L.vec[1]);
-  float *p = (float *)(wsb + L.vec[2]);
-  float *sv = (float *)(wsb + L.vec[3]);
-  float *r = (float *)(wsb + L.vec[4]);
-  float *dinv = (float *)(wsb + L.vec[5]);
-  float *x = (float *)(wsb + L.vec[6]);
-  float *b = (float *)(wsb + L.vec[7]);
-  float *diag = (float *)(wsb + L.vec[8]);
-  float *u = (float *)(wsb + matfree_u_offset(n));
+  float *z = ws_vec<float>(wsb, L, 0);
+  float *w = ws_vec<float>(wsb, L, 1);
+  float *p = ws_vec<float>(wsb, L, 2);
+  float *sv = ws_vec<float>(wsb, L, 3);
+  float *r = ws_vec<float>(wsb, L, 4);
+  float *dinv = ws_vec<float>(wsb, L, 5);
+  float *x = ws_vec<float>(wsb, L, 6);
+  float *b = ws_vec<float>(wsb, L, 7);
+  float *diag = ws_vec<float>(wsb, L, 8);
+  float *u = reinterpret_cast<float *>(wsb + matfree_u_offset(n));
   const PcgParts P = parts_at(wsb, L);
-  PcgState *st = (PcgState *)(w + L.state);
+  PcgState *st = ws_state(wsb, L);
   hipStream_t caller = (hipStream_t)stream;
   hipStream_t s = res->cap;
   TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));

@@ -239,6 +239,81 @@ class Operator:
         from .energies import Energy  # noqa: PLC0415
         return Energy(self, density, external_force=external_force)
 
+    # --------------------------------------------------------- projection --
+
+    def project(self, field_q, colored_matrix=None, lifter=None, *, tol: float = 1e-13):
+        """L2 projection of a quadrature-point field onto the nodal space
+        (operator.py:523-560, utils.py:118-262): solve M x = b with the
+        consistent mass matrix M_ab = int N_a N_b and b_a = int N_a f.
+
+        ``field_q`` is (n_elements, n_quad, ...).  Without ``colored_matrix``
+        (or with a one-dof-per-node pattern) every component is a separate
+        right-hand side of the scalar mass matrix; a pattern with ``dim`` dofs
+        per node couples the ``dim`` trailing components (block-diagonal mass).
+        With a ``lifter`` the system is solved on its free dofs and the result
+        lifted (constraint values imposed, as the reference does).  The linear
+        solves are Jacobi-PCG to ``tol`` relative residual (the reference uses a
+        direct solve; the mass matrix is well conditioned).
+        """
+        from . import sparse as sp  # noqa: PLC0415
+        from .solvers import pcg  # noqa: PLC0415
+
+        t = nat.torch()
+        if lifter is not None and lifter.has_periodic:
+            raise NotImplementedError("project supports Fixed constraints only")
+        f = nat.to_device(field_q)
+        if f.ndim < 2 or f.shape[0] != self.n_elements or f.shape[1] != self.n_quad:
+            raise ValueError("field must be shaped (n_elements, n_quad, ...)")
+        value_shape = tuple(f.shape[2:])
+        nvals = int(np.prod(value_shape)) if value_shape else 1
+        N = self.n_nodes
+        if colored_matrix is None:
+            dim = 1
+            pat = sp.pattern_from_mesh(self.mesh, 1)
+            if lifter is not None:
+                pat = lifter.adapt_sparsity(pat)
+            colored_matrix = sp.ColoredMatrix.from_csr(pat)
+        else:
+            n_full = lifter.size if lifter is not None else colored_matrix.shape[0]
+            if lifter is not None and colored_matrix.shape[0] != lifter.size_reduced:
+                raise ValueError("Colored matrix size does not match lifter reduced size. "
+                                 f"Expected {lifter.size_reduced}, got {colored_matrix.shape[0]}")
+            if n_full % N:
+                raise ValueError("the pattern size is not a multiple of the number of nodes")
+            dim = n_full // N
+        if dim > 1 and (nvals % dim or (value_shape and value_shape[-1] != dim)):
+            raise ValueError(f"a {dim}-dof-per-node pattern needs fields with trailing size {dim}")
+        cm = colored_matrix
+        L = nat.lib()
+        ws = t.empty(int(L.tatva_colored_hessian_workspace(self.handle, dim)), dtype=t.uint8,
+                     device=f.device)
+        data = t.empty(cm.nnz, dtype=t.float64, device=f.device)
+        dmap = lifter.dof_map_device() if lifter is not None else None
+        r2f = lifter.free_device() if lifter is not None else None
+        nat.check(L.tatva_mass_matrix(self.handle, dim, nat.dptr(dmap), nat.dptr(r2f), cm.shape[0],
+                                      nat.dptr(cm.colors), cm.n_colors, nat.dptr(cm.indptr),
+                                      nat.dptr(cm.indices), cm.nnz, nat.dptr(data), nat.dptr(ws),
+                                      ws.numel(), nat.stream()))
+        from dataclasses import replace  # noqa: PLC0415
+        M = replace(cm, data=data)
+        vals = f.reshape(self.n_elements, self.n_quad, nvals).contiguous()
+        b = t.empty((N, nvals), dtype=t.float64, device=f.device)
+        nat.check(L.tatva_project_rhs(self.handle, nvals, nat.dptr(vals), nat.dptr(b),
+                                      nat.stream()))
+        # right-hand sides in solver layout: (n_rhs, N * dim)
+        rhs = b.reshape(N, nvals // dim, dim).permute(1, 0, 2).reshape(nvals // dim, N * dim)
+        out = t.empty_like(rhs)
+        for k in range(rhs.shape[0]):
+            bk = rhs[k].contiguous()
+            if lifter is not None:
+                bk = lifter.reduce(bk)      # test functions injected at the free dofs
+            xk, info = pcg(M, bk, tol=tol, block_size=dim if dim in (2, 3) else None)
+            if lifter is not None:
+                xk = lifter.lift_from_zeros(xk)
+            out[k] = xk
+        res = out.reshape(nvals // dim, N, dim).permute(1, 0, 2).reshape((N,) + value_shape)
+        return res.contiguous()
+
     def close(self) -> None:
         """Release the device-resident mesh now (otherwise freed with the Operator)."""
         if self._dev is not None:

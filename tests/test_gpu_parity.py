@@ -479,3 +479,46 @@ def test_periodic_hessian_when_slave_and_master_share_elements():
 def t_zeros(n):
     import torch
     return torch.zeros(n, dtype=torch.float64, device="cuda")
+
+
+# ----------------------------------------------------------- projection ----
+
+def _tri3_quadratic_rule():
+    return np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]), np.full(3, 1 / 6)
+
+
+def test_project_reproduces_linear_and_fe_fields():
+    """tests/test_operator_projection.py:21-86 on Tri3 (3-point rule) and Hex8:
+    scalar, vector (scalar pattern and coupled 2-dof pattern), tensor and the
+    default pattern; any field in the FE space is reproduced."""
+    qp, qw = _tri3_quadratic_rule()
+    m = ta.Mesh.unit_square(2, 2)
+    op = ta.Operator(m, ta.Tri3(qp, qw))
+    q = to_np(op.quads())
+    X = m.coords
+    s = op.project(q[:, :, 0] + q[:, :, 1], sparse.ColoredMatrix.from_csr(
+        sparse.pattern_from_mesh(m, 1)))
+    np.testing.assert_allclose(to_np(s), X[:, 0] + X[:, 1], atol=1e-10)
+    v = op.project(q)
+    assert tuple(v.shape) == (m.n_nodes, 2)
+    np.testing.assert_allclose(to_np(v), X, atol=1e-10)
+    vc = op.project(q, sparse.ColoredMatrix.from_csr(sparse.pattern_from_mesh(m, 2)))
+    np.testing.assert_allclose(to_np(vc), X, atol=1e-10)
+    T = np.zeros(q.shape[:2] + (2, 2))
+    T[..., 0, 0], T[..., 1, 1] = q[..., 0], q[..., 1]
+    Tp = to_np(op.project(T))
+    np.testing.assert_allclose(Tp[:, 0, 0], X[:, 0], atol=1e-10)
+    np.testing.assert_allclose(Tp[:, 1, 1], X[:, 1], atol=1e-10)
+    np.testing.assert_allclose(Tp[:, 0, 1], 0.0, atol=1e-10)
+    # Hex8 on a jittered cube: a random FE field and the oracle's projection
+    mh, el = make_mesh("hex8", 3)
+    oph = ta.Operator(mh, el)
+    nodal = np.random.default_rng(8).standard_normal((mh.n_nodes, 3))
+    f = fem.op_eval(fem.Hex8(), mh.coords, mh.elements, nodal)
+    np.testing.assert_allclose(to_np(oph.project(f)), nodal, atol=1e-9)
+    bnd = np.where(np.isclose(mh.coords[:, 0], 0.0))[0]
+    L = lifter.Lifter(mh.n_nodes, lifter.Fixed(bnd, 0.25))
+    Lo = fem.Lifter(mh.n_nodes, fem.Fixed(bnd, 0.25))
+    got = to_np(oph.project(f[..., 0], lifter=L))
+    want = fem.project(fem.Hex8(), mh.coords, mh.elements, f[..., 0], lifter=Lo)
+    assert rel_err(got, want) < 1e-9

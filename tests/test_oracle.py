@@ -439,3 +439,32 @@ def test_element_dof_pattern_oracle():
     want |= {(10, 10), (3, 3)}
     Sc = S.tocoo()
     assert set(zip(Sc.row.tolist(), Sc.col.tolist())) == want
+
+
+@pytest.mark.parametrize("gen", ["tri3", "tet4", "hex8"])
+def test_projection_oracle_reproduces_fe_fields(gen):
+    """A field that lies in the FE space (nodal values evaluated at the
+    quadrature points) projects back to its nodal values."""
+    # the one-point rules of Tri3/Tet4 give a singular mass matrix: use rules
+    # exact for quadratics (3 points on triangles, 4 on tetrahedra)
+    if gen == "tri3":
+        (X, E) = fem.unit_square_tri3(3, 2)
+        el = fem.Tri3(np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]),
+                      np.full(3, 1 / 6))
+    elif gen == "tet4":
+        (X, E) = fem.box_tet4(2, 2, 1, jitter=0.15, seed=2)
+        a, b = 0.5854101966249685, 0.1381966011250105
+        el = fem.Tet4(np.array([[b, b, b], [a, b, b], [b, a, b], [b, b, a]]), np.full(4, 1 / 24))
+    else:
+        (X, E), el = fem.cube_hex8(2), fem.Hex8()
+        inner = ~(np.isclose(X, 0) | np.isclose(X, 1)).any(axis=1)
+        X = X + 0.07 * np.random.default_rng(3).uniform(-1, 1, X.shape) * inner[:, None]
+    rng = np.random.default_rng(4)
+    nodal = rng.standard_normal((X.shape[0], 2, 2))
+    f = fem.op_eval(el, X, E, nodal)
+    np.testing.assert_allclose(fem.project(el, X, E, f), nodal, atol=1e-11)
+    # with constraints: free values reproduced, constrained set to their values
+    bnd = np.where(np.isclose(X[:, 0], 0.0))[0]
+    L = fem.Lifter(X.shape[0], fem.Fixed(bnd, 0.25))
+    got = fem.project(el, X, E, f[..., 0, 0], lifter=L)
+    assert np.all(got[bnd] == 0.25)
