@@ -303,7 +303,7 @@ def main(argv=None):
                          "comparison on the 64^3 elasticity / 48^3 Neo-Hookean cube; "
                          "c3/c5: large mixed-vs-FP64 Neo-Hookean runs")
     ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-qps", type=float, default=10.0)
+    ap.add_argument("--serve-qps", type=float, nargs="+", default=[10.0, 40.0])
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path's "
@@ -408,7 +408,7 @@ def main(argv=None):
         del NH
         extras["c1_poisson_tri3_64"] = c1_latency()
         if args.serve_requests > 0:
-            extras["serve_c4"] = serve_stream(args.serve_requests, args.serve_qps)
+            extras["serve_c4"] = [serve_stream(args.serve_requests, q) for q in args.serve_qps]
 
     # checks on the final state: true linear residual and the FP64 oracle's residual
     res_free = P["E"].reduced(L).residual_full_state(u)

@@ -185,6 +185,15 @@ int tatva_mass_matrix(tatva_mesh *mesh, int32_t ncomp, const int32_t *dof_map,
 int tatva_project_rhs(tatva_mesh *mesh, int32_t nc, const double *quad_values, double *out,
                       void *stream);
 
+/* Operator.interpolate (tatva/operator.py:399-468 with find_containing_polygons,
+ * tatva/mesh.py:295-390) on a 2-D Tri3 mesh: out[p, c] = nodal values (N x nc)
+ * interpolated at points (n_points x 2); the containing element is the first
+ * whose vertex polygon holds the point (edges included), mapped to reference
+ * coordinates by one Newton step from the first quadrature point.  Points in no
+ * element get NaN; their number is returned in *n_outside (synchronises stream). */
+int tatva_interpolate(tatva_mesh *mesh, int64_t n_points, const double *points, int32_t nc,
+                      const double *nodal, double *out, int32_t *n_outside, void *stream);
+
 /* rows[k] = row of entry k and col_colors[k] = colors[indices[k]] for a CSR
  * pattern (compute_rows_cols, tatva/sparse/base.py:108-136). */
 int tatva_rows_cols(int64_t n, const int32_t *indptr, const int32_t *indices,

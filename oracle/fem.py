@@ -673,6 +673,52 @@ def pcg(A, b, x0=None, tol=1e-10, maxiter=100000, diag=None):
     return x, it, rel
 
 
+def find_containing_polygons(points, polygons):
+    """mesh.py:295-390: index of the first polygon holding each point (edges
+    within 1e-8 count as inside; bounding-box reject; ray-casting parity), -1 if none."""
+    pts = np.asarray(points, dtype=np.float64)
+    out = np.full(pts.shape[0], -1, dtype=np.int64)
+    for i, (px, py) in enumerate(pts):
+        for e, V in enumerate(polygons):
+            lo, hi = V.min(axis=0), V.max(axis=0)
+            if not (lo[0] <= px <= hi[0] and lo[1] <= py <= hi[1]):
+                continue
+            W = np.roll(V, -1, axis=0)
+            cross = (W[:, 0] - V[:, 0]) * (py - V[:, 1]) - (W[:, 1] - V[:, 1]) * (px - V[:, 0])
+            on_seg = (np.minimum(V[:, 0], W[:, 0]) <= px) & (px <= np.maximum(V[:, 0], W[:, 0])) & \
+                (np.minimum(V[:, 1], W[:, 1]) <= py) & (py <= np.maximum(V[:, 1], W[:, 1]))
+            if np.any((np.abs(cross) <= 1e-8) & on_seg):
+                out[i] = e
+                break
+            ycond = ((V[:, 1] <= py) & (W[:, 1] > py)) | ((W[:, 1] <= py) & (V[:, 1] > py))
+            with np.errstate(divide="ignore", invalid="ignore"):
+                xint = (W[:, 0] - V[:, 0]) * (py - V[:, 1]) / (W[:, 1] - V[:, 1]) + V[:, 0]
+            if np.sum(ycond & (px < xint)) % 2 == 1:
+                out[i] = e
+                break
+    return out
+
+
+def interpolate(elem, coords, elements, nodal, points):
+    """Operator.interpolate (operator.py:399-468): one Newton step from the
+    first quadrature point in the containing element, NaN outside."""
+    X = np.asarray(coords)
+    el = np.asarray(elements)
+    idx = find_containing_polygons(points, X[el])
+    nodal = np.asarray(nodal, dtype=np.float64)
+    out = np.full((len(points),) + nodal.shape[1:], np.nan)
+    xi0 = elem.quad_points[0]
+    for i, e in enumerate(idx):
+        if e < 0:
+            continue
+        Xe = X[el[e]]
+        x0 = elem.N(xi0) @ Xe
+        J = (elem.dN(xi0) @ Xe).T                      # dx_i / dxi_k
+        xi = xi0 + np.linalg.solve(J, np.asarray(points[i]) - x0)
+        out[i] = np.tensordot(elem.N(xi), nodal[el[e]], axes=1)
+    return out
+
+
 def project(elem, coords, elements, field_q, lifter=None):
     """L2 projection (operator.py:523-560, utils.py:118-262) with the scalar
     mass matrix and one right-hand side per component: M_ab = sum N_a N_b w,

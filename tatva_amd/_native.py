@@ -69,6 +69,8 @@ _SIGS = {
     "tatva_mass_matrix": (c_int, [c_void, c_i32, c_void, c_void, c_i64, c_void, c_i32, c_void, c_void,
                                   c_i64, c_void, c_void, c_i64, c_void]),
     "tatva_project_rhs": (c_int, [c_void, c_i32, c_void, c_void, c_void]),
+    "tatva_interpolate": (c_int, [c_void, c_i64, c_void, c_i32, c_void, c_void, ctypes.POINTER(c_i32),
+                                  c_void]),
     "tatva_rows_cols": (c_int, [c_i64, c_void, c_void, c_void, c_void, c_void, c_void]),
     "tatva_spmv_csr": (c_int, [c_i64, c_i64, c_void, c_void, c_void, c_void, c_void, c_void]),
     "tatva_pcg_csr_workspace": (c_i64, [c_i64]),

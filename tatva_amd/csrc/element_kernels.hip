@@ -516,6 +516,93 @@ __global__ __launch_bounds__(kBlock) void k_project_rhs(int64_t n_nodes, const d
   }
 }
 
+// Operator.interpolate for 2-D elements (operator.py:399-468, mesh.py:295-390):
+// the first element (lowest index) whose vertex polygon contains the point
+// (bounding-box reject, points within 1e-8 of an edge count as inside, else
+// ray-casting parity), one Newton step x(xi) = p from the first quadrature
+// point, then the element interpolation of the nodal values.  Points in no
+// element get NaN and are counted in *outside.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_interpolate(int64_t n_points, const double *__restrict__ pts,
+                                                        int64_t n_elem, const double *__restrict__ coords,
+                                                        const int32_t *__restrict__ conn, QuadRule quad,
+                                                        int nc, const double *__restrict__ nodal,
+                                                        double *__restrict__ out,
+                                                        int *__restrict__ outside) {
+  using El = Elem<KIND>;
+  constexpr int NNE = El::NNE;
+  static_assert(El::D == 2, "polygon location is 2-D");
+  const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n_points) return;
+  const double px = pts[ip * 2], py = pts[ip * 2 + 1];
+  int64_t found = -1;
+  double X[NNE][2];
+  int32_t nodes[NNE];
+  for (int64_t e = 0; e < n_elem && found < 0; ++e) {
+    double lo0 = 1e300, lo1 = 1e300, hi0 = -1e300, hi1 = -1e300;
+#pragma unroll
+    for (int a = 0; a < NNE; ++a) {
+      nodes[a] = conn[e * NNE + a];
+      X[a][0] = coords[(int64_t)nodes[a] * 2];
+      X[a][1] = coords[(int64_t)nodes[a] * 2 + 1];
+      lo0 = fmin(lo0, X[a][0]);
+      hi0 = fmax(hi0, X[a][0]);
+      lo1 = fmin(lo1, X[a][1]);
+      hi1 = fmax(hi1, X[a][1]);
+    }
+    if (!(px >= lo0 && px <= hi0 && py >= lo1 && py <= hi1)) continue;
+    bool boundary = false;
+    int crossings = 0;
+#pragma unroll
+    for (int a = 0; a < NNE; ++a) {
+      const double x1 = X[a][0], y1 = X[a][1];
+      const double x2 = X[(a + 1) % NNE][0], y2 = X[(a + 1) % NNE][1];
+      const double cross = (x2 - x1) * (py - y1) - (y2 - y1) * (px - x1);
+      const bool on_seg = fmin(x1, x2) <= px && px <= fmax(x1, x2) && fmin(y1, y2) <= py &&
+                          py <= fmax(y1, y2);
+      if (fabs(cross) <= 1e-8 && on_seg) boundary = true;
+      const bool ycond = (y1 <= py && y2 > py) || (y2 <= py && y1 > py);
+      if (ycond) {
+        const double xi = (x2 - x1) * (py - y1) / (y2 - y1) + x1;
+        if (px < xi) ++crossings;
+      }
+    }
+    if (boundary || (crossings & 1)) found = e;
+  }
+  if (found < 0) {
+    for (int c = 0; c < nc; ++c) out[ip * nc + c] = __builtin_nan("");
+    atomicAdd(outside, 1);
+    return;
+  }
+  // one Newton step from the first quadrature point: J dxi = p - x(xi0)
+  const double *xi0 = quad.xi[0];
+  double N0[NNE], dn[2][NNE];
+  El::N(xi0, N0);
+  El::dN(xi0, dn);
+  double x0[2] = {0.0, 0.0}, J[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+  for (int a = 0; a < NNE; ++a)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      x0[i] += N0[a] * X[a][i];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) J[i][k] += X[a][i] * dn[k][a];  // dx_i / dxi_k
+    }
+  double inv[2][2];
+  det_inv(J, inv);
+  const double r0 = px - x0[0], r1 = py - x0[1];
+  double xi[3] = {xi0[0] + inv[0][0] * r0 + inv[0][1] * r1, xi0[1] + inv[1][0] * r0 + inv[1][1] * r1,
+                  0.0};
+  double N[NNE];
+  El::N(xi, N);
+  for (int c = 0; c < nc; ++c) {
+    double v = 0.0;
+#pragma unroll
+    for (int a = 0; a < NNE; ++a) v += N[a] * nodal[(int64_t)nodes[a] * nc + c];
+    out[ip * nc + c] = v;
+  }
+}
+
 }  // namespace tatva
 
 using namespace tatva;
@@ -1177,4 +1264,32 @@ extern "C" int tatva_project_rhs(tatva_mesh *m, int32_t nc, const double *quad_v
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   });
+}
+
+extern "C" int tatva_interpolate(tatva_mesh *m, int64_t n_points, const double *points, int32_t nc,
+                                 const double *nodal, double *out, int32_t *n_outside,
+                                 void *stream) {
+  MESH_PRELUDE(m);
+  TATVA_REQUIRE(n_points >= 0 && nc >= 1 && n_outside, "invalid arguments");
+  TATVA_REQUIRE(n_points == 0 || (points && nodal && out), "invalid arguments");
+  TATVA_REQUIRE(m->kind == TATVA_TRI3, "point location is implemented for 2-D (Tri3) meshes");
+  *n_outside = 0;
+  if (n_points == 0) return TATVA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int *d_out = reinterpret_cast<int *>(m->partials);  // the mesh's reduction scratch
+  hipError_t e = hipMemsetAsync(d_out, 0, sizeof(int), s);
+  if (e == hipSuccess) {
+    k_interpolate<TATVA_TRI3><<<grid_for(n_points), kBlock, 0, s>>>(
+        n_points, points, m->n_elements, m->coords, m->conn, m->quad, nc, nodal, out, d_out);
+    e = hipGetLastError();
+  }
+  int h = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d_out, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    set_error(std::string("interpolate: ") + hipGetErrorString(e));
+    return TATVA_EHIP;
+  }
+  *n_outside = h;
+  return TATVA_OK;
 }

@@ -239,6 +239,30 @@ class Operator:
         from .energies import Energy  # noqa: PLC0415
         return Energy(self, density, external_force=external_force)
 
+    # ------------------------------------------------------ interpolation --
+
+    def interpolate(self, arg, points):
+        """Nodal values interpolated at physical points (operator.py:399-468):
+        the first element whose vertex polygon holds the point (edges
+        included), one Newton step to reference coordinates from the first
+        quadrature point, then the element interpolation.  2-D (Tri3) meshes.
+        Raises RuntimeError when a point lies in no element."""
+        t = nat.torch()
+        a = nat.to_device(arg)
+        if a.shape[0] != self.n_nodes:
+            raise ValueError("interpolate needs nodal values (one row per mesh node)")
+        trailing = tuple(a.shape[1:])
+        nc = int(np.prod(trailing)) if trailing else 1
+        p = nat.to_device(points).reshape(-1, 2).contiguous()
+        out = t.empty((p.shape[0], nc), dtype=t.float64, device=a.device)
+        outside = ctypes.c_int32(0)
+        nat.check(nat.lib().tatva_interpolate(self.handle, p.shape[0], nat.dptr(p), nc,
+                                              nat.dptr(a.reshape(self.n_nodes, nc).contiguous()),
+                                              nat.dptr(out), ctypes.byref(outside), nat.stream()))
+        if outside.value:
+            raise RuntimeError("Some points are outside the mesh, revise the points")
+        return out.reshape((p.shape[0],) + trailing)
+
     # --------------------------------------------------------- projection --
 
     def project(self, field_q, colored_matrix=None, lifter=None, *, tol: float = 1e-13):

@@ -522,3 +522,25 @@ def test_project_reproduces_linear_and_fe_fields():
     got = to_np(oph.project(f[..., 0], lifter=L))
     want = fem.project(fem.Hex8(), mh.coords, mh.elements, f[..., 0], lifter=Lo)
     assert rel_err(got, want) < 1e-9
+
+
+def test_interpolate_matches_oracle():
+    """operator.py:399-468 on a jittered Tri3 mesh: interpolated values against
+    the oracle (same containing element, same Newton step); linear fields exact;
+    a point outside raises (operator.py:453-459)."""
+    m, el = make_mesh("tri3", 6)
+    op = ta.Operator(m, el)
+    rng = np.random.default_rng(10)
+    pts = np.vstack([rng.uniform(0, 1, (200, 2)), m.coords[::5], [[0.5, 0.5], [0.0, 0.3]]])
+    nodal = rng.standard_normal((m.n_nodes, 3))
+    got = to_np(op.interpolate(nodal, pts))
+    want = fem.interpolate(fem.Tri3(), m.coords, m.elements, nodal, pts)
+    assert np.isfinite(got).all()
+    # points on shared edges may be located in either neighbour: the field is
+    # continuous there, so values agree to rounding
+    assert rel_err(got, want) < 1e-10
+    lin = 0.5 + m.coords @ np.array([1.5, -2.0])
+    np.testing.assert_allclose(to_np(op.interpolate(lin, pts)), 0.5 + pts @ np.array([1.5, -2.0]),
+                               atol=1e-12)
+    with pytest.raises(RuntimeError):
+        op.interpolate(nodal, np.array([[1.2, 0.5]]))

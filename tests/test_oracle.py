@@ -468,3 +468,24 @@ def test_projection_oracle_reproduces_fe_fields(gen):
     L = fem.Lifter(X.shape[0], fem.Fixed(bnd, 0.25))
     got = fem.project(el, X, E, f[..., 0, 0], lifter=L)
     assert np.all(got[bnd] == 0.25)
+
+
+def test_interpolate_oracle_linear_fields_and_location():
+    """Operator.interpolate restated: linear fields are reproduced at any point,
+    the containing triangle is the first one holding the point, outside -> NaN."""
+    X, E = fem.unit_square_tri3(4, 3)
+    el = fem.Tri3()
+    rng = np.random.default_rng(6)
+    pts = rng.uniform(0, 1, (40, 2))
+    pts = np.vstack([pts, X[[0, 7, 19]], [[0.5, 1 / 3], [0.25, 0.2]]])   # vertices, edges
+    nodal = np.stack([1.0 + 2 * X[:, 0] - 3 * X[:, 1], X[:, 0] * 0.5], axis=1)
+    got = fem.interpolate(el, X, E, nodal, pts)
+    want = np.stack([1.0 + 2 * pts[:, 0] - 3 * pts[:, 1], pts[:, 0] * 0.5], axis=1)
+    np.testing.assert_allclose(got, want, atol=1e-12)
+    # cell (0, 0) of a 4 x 3 grid: nodes 0, 1 (= x step), 4 (= y step), 5; below the
+    # diagonal lies the first triangle [n0, n1, n3] = element 0, above it element 1
+    h = np.array([0.25, 1 / 3])
+    idx = fem.find_containing_polygons(np.array([[0.6, 0.2], [0.2, 0.6]]) * h, X[E])
+    np.testing.assert_array_equal(idx, [0, 1])
+    out = fem.interpolate(el, X, E, nodal, np.array([[1.5, 0.5]]))
+    assert np.isnan(out).all()
