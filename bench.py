@@ -103,10 +103,14 @@ def solve_load_steps(P, solver, n_steps, stretch):
         u.add_(ramp, alpha=stretch / n_steps)
         L = P["L"].at["load"].set(P["values"](stretch * s / n_steps))
         out.append(solver.solve(u, L))
+        if P.get("progress"):
+            st = out[-1]
+            print(f"[{solver.mode}] load step {s}/{n_steps}: {st.newton_iterations} Newton, "
+                  f"{st.cg_iterations} CG, {st.seconds:.2f} s", file=sys.stderr, flush=True)
     return u, out
 
 
-def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=False):
+def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=False, warmup=True):
     """The same problem solved by FP64 colored-CSR Newton and by mixed Newton
     (FP32 matrix-free PCG inside FP64 Newton/iterative refinement), both to
     ||r|| <= tol ||r_0||.  ``run(solver)`` performs the solve and returns
@@ -120,8 +124,12 @@ def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=Fals
     for mode in ("csr", "mixed"):
         kw = dict(lin_tol=1e-10, colored_matrix=colored_matrix) if mode == "csr" \
             else dict(inner_tol=1e-3)
+        t_build = time.perf_counter()
         solver = solvers.Newton(E, L0, mode=mode, tol=tol, chunk=64, **kw)
-        run(solver)     # warm-up (workspaces)
+        print(f"[{mode}] solver set up in {time.perf_counter() - t_build:.1f} s", file=sys.stderr,
+              flush=True)
+        if warmup:
+            run(solver)     # warm-up (workspaces)
         solvers.set_profiling(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -257,10 +265,15 @@ def large_mixed(workload, oracle_check=True):
     stretch = 0.3
     t0 = time.perf_counter()
     NH = stretch_problem(n, kind=kind, jitter=jitter, seed=0)
+    NH["progress"] = True
     L0 = NH["L"].at["load"].set(NH["values"](0.0))
     setup = time.perf_counter() - t0
+    print(f"{workload}: mesh {NH['mesh'].n_elements} elements, {setup:.1f} s", file=sys.stderr,
+          flush=True)
+    # no warm-up solve at this size: the first Newton step's set-up (graph
+    # capture, workspaces) is a negligible share of the timed solve
     res = mixed_vs_fp64(NH["E"], L0, lambda sv: solve_load_steps(NH, sv, steps, stretch),
-                        keep_solutions=True)
+                        keep_solutions=True, warmup=False)
     mesh = NH["mesh"]
     res["workload"] = (f"{kind} Neo-Hookean {n}^3{' x 6 tets' if kind == 'tet4' else ''}, "
                        f"mu=1, lambda=10, jitter={jitter}h (seed 0), x=0 clamped, u_x on x=1 "
