@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the node-block SpMV on the 64^3 line, from the two
 # rocprofv3 PMC passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
 # plain 512 MiB copy) recorded in profiles/k_pcg_bspmv_hex8_le_64cubed.md.
-SPMV_TRAFFIC_64 = {"fetch_kib": 276631.5, "fetch_factor": 2.000, "write_kib": 7453.9,
+SPMV_TRAFFIC_64 = {"fetch_kib": 273681.4, "fetch_factor": 2.000, "write_kib": 7401.9,
                    "write_factor": 1.000}
 
 
