@@ -3,10 +3,14 @@
 //
 // The elements are processed in tiles of kTile (= one workgroup) taken from a
 // Morton (Z-order) ordering of the element centroids, so a tile is spatially
-// compact.  A workgroup computes its elements' vectors in registers and sums
-// them into LDS accumulators of the tile's local nodes, one element color at
-// a time (no two elements of a color share a node; colors are separated by
-// barriers), then writes the tile's node sums to its slots of a partial buffer.
+// compact.  A workgroup loads its local nodes' data into LDS once, computes its
+// elements' vectors in registers and sums them per local node, then writes the
+// tile's node sums to its slots of a partial buffer.  The per-tile sum runs
+// one element color at a time into LDS accumulators (no two elements of a
+// color share a node; colors are separated by barriers) for Tri3 and Hex8
+// (6-13 colors a tile); Tet4 tiles have ~30 colors, so there every element
+// writes its vector to LDS and each local node adds its entries in ascending
+// (slot, node) order from a per-tile incidence list (two barriers).
 // A node gather adds, per node, the slots of the tiles containing it in tile
 // order.  The result is deterministic with no float atomics, and HBM sees
 // about (local nodes / elements) x ncomp values per element instead of
@@ -38,7 +42,18 @@ struct TileSet {
   const int32_t *node_ptr;   // (n_tiles + 1) first slot of each tile
   const int32_t *slot_ptr;   // (n_nodes + 1) node -> its slots
   const int32_t *slot_list;  // slots of each node, ascending tile
+  const int32_t *tnodes;     // (n_slots) global node of each slot
+  // Tet4 only: entries (slot * nne + a) of each local node, ascending, at
+  // inc_list[tile * kTile * nne + ...]; local node j ends at inc_end[node_ptr + j]
+  const uint16_t *inc_end;   // (n_slots)
+  const uint16_t *inc_list;  // (n_tiles * kTile * nne)
 };
+
+// Per-tile sum by the incidence lists instead of element colors.
+template <int KIND>
+constexpr bool tile_gather() {
+  return KIND == TATVA_TET4;
+}
 
 // ------------------------------------------------------ element kernels ----
 
@@ -394,39 +409,78 @@ inline bool hex_is_affine_host(const double (&X)[8][3]) {
 
 // ------------------------------------------------------ tile assembly ------
 
-// Body of one tile (one workgroup of kTile threads; acc = LDS of
-// max_local * NC values).  u: full state; v (HVP): tangent read through
-// dof_map (reduced, zero on constrained dofs).  Writes the tile's node sums to
-// partial[(node_ptr[tile] + j) * NC + c].
+// LDS of one tile: per local node its coordinates (FP64), then the
+// accumulator, the state and (HVP) the tangent in a T of tsize bytes; with
+// the incidence-list sum, at least the element vectors (kTile x nne x NC).
+constexpr size_t tile_lds_bytes(int kind, int max_local, int D, int nne, int NC, size_t tsize,
+                                int mode) {
+  const size_t staged = (size_t)max_local * (D * sizeof(double) + (mode == kModeHvp ? 3 : 2) * NC * tsize);
+  const size_t elem = (size_t)kTile * nne * NC * tsize;
+  return kind == TATVA_TET4 && elem > staged ? elem : staged;
+}
+constexpr size_t kTileLdsMax = 65536;  // LDS one workgroup may allocate
+
+// Body of one tile (one workgroup of kTile threads; lds of tile_lds_bytes).  u: full state; v (HVP): tangent read through dof_map
+// (reduced, zero on constrained dofs).  The tile's local nodes are loaded into
+// LDS once (one thread per node), so each element reads its nodes from LDS
+// instead of gathering them from global memory.  Writes the tile's node sums
+// to partial[(node_ptr[tile] + j) * NC + c].
 template <int KIND, class Dens, class T, int MODE, bool AFFINE>
 __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict__ coords,
-                                              const int32_t *__restrict__ conn,
                                               const QuadRule &quad, const Dens &dens,
                                               const T *__restrict__ u, const T *__restrict__ v,
                                               const int32_t *__restrict__ dof_map,
                                               const TileSet &ts, T *__restrict__ partial,
-                                              T *__restrict__ acc) {
+                                              unsigned char *__restrict__ lds) {
   using El = Elem<KIND>;
   constexpr int D = El::D, NNE = El::NNE, NC = Dens::NC;
   const int32_t nb = ts.node_ptr[tile];
   const int nloc = ts.node_ptr[tile + 1] - nb;
-  for (int i = threadIdx.x; i < nloc * NC; i += kTile) acc[i] = T(0);
+  double *Xs = reinterpret_cast<double *>(lds);
+  T *acc = reinterpret_cast<T *>(Xs + (size_t)ts.max_local * D);
+  T *us = acc + (size_t)ts.max_local * NC;
+  T *vs = us + (size_t)ts.max_local * NC;
+  for (int j = threadIdx.x; j < nloc; j += kTile) {
+    const int64_t g = ts.tnodes[nb + j];
+#pragma unroll
+    for (int k = 0; k < D; ++k) Xs[j * D + k] = coords[g * D + k];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int64_t dof = g * NC + c;
+      acc[j * NC + c] = T(0);
+      us[j * NC + c] = u[dof];
+      if constexpr (MODE == kModeHvp) {
+        if (dof_map) {
+          const int32_t r = dof_map[dof];
+          vs[j * NC + c] = r >= 0 ? v[r] : T(0);
+        } else {
+          vs[j * NC + c] = v[dof];
+        }
+      }
+    }
+  }
   const int64_t slot = (int64_t)tile * kTile + threadIdx.x;
   const int32_t e = ts.elems[slot];
+  const uint16_t *ln = ts.lnodes + slot * NNE;
+  int l[NNE];
+#pragma unroll
+  for (int a = 0; a < NNE; ++a) l[a] = e >= 0 ? (int)ln[a] : 0;
   T ye[NNE][NC];
 #pragma unroll
   for (int a = 0; a < NNE; ++a)
 #pragma unroll
     for (int c = 0; c < NC; ++c) ye[a][c] = T(0);
+  __syncthreads();
   if (e >= 0) {
-    int32_t nodes[NNE];
     double X[NNE][D];
-    load_element<KIND>(e, conn, coords, nodes, X);
     T ue[NNE][NC];
 #pragma unroll
-    for (int a = 0; a < NNE; ++a)
+    for (int a = 0; a < NNE; ++a) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) ue[a][c] = u[(int64_t)nodes[a] * NC + c];
+      for (int k = 0; k < D; ++k) X[a][k] = Xs[l[a] * D + k];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) ue[a][c] = us[l[a] * NC + c];
+    }
     if constexpr (KIND == TATVA_HEX8 && MODE == kModeResidual) {
       hex8_element<Dens, T, kModeResidual, AFFINE>(X, ue, ue, quad, dens, ye);
     } else if constexpr (MODE == kModeResidual) {
@@ -436,31 +490,41 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
 #pragma unroll
       for (int a = 0; a < NNE; ++a)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const int64_t dof = (int64_t)nodes[a] * NC + c;
-          if (dof_map) {
-            const int32_t r = dof_map[dof];
-            ve[a][c] = r >= 0 ? v[r] : T(0);
-          } else {
-            ve[a][c] = v[dof];
-          }
-        }
+        for (int c = 0; c < NC; ++c) ve[a][c] = vs[l[a] * NC + c];
       if constexpr (KIND == TATVA_HEX8) hex8_element<Dens, T, kModeHvp, AFFINE>(X, ue, ve, quad, dens, ye);
       else elem_hvp<KIND, Dens, T>(X, ue, ve, quad, dens, ye);
     } else {
       elem_diag<KIND, Dens, T>(X, ue, quad, dens, ye);
     }
   }
-  __syncthreads();
+  if constexpr (tile_gather<KIND>()) {
+    __syncthreads();  // every element has read its nodes: the LDS is free
+    T *ys = reinterpret_cast<T *>(lds);
+#pragma unroll
+    for (int a = 0; a < NNE; ++a)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) ys[((int)threadIdx.x * NNE + a) * NC + c] = ye[a][c];
+    __syncthreads();
+    const uint16_t *il = ts.inc_list + (int64_t)tile * kTile * NNE;
+    T *out = partial + (int64_t)nb * NC;
+    for (int i = threadIdx.x; i < nloc * NC; i += kTile) {
+      const int j = i / NC, c = i - j * NC;
+      const int q0 = j > 0 ? (int)ts.inc_end[nb + j - 1] : 0;
+      const int q1 = ts.inc_end[nb + j];
+      T sum = T(0);
+      for (int q = q0; q < q1; ++q) sum += ys[(int)il[q] * NC + c];
+      out[i] = sum;
+    }
+    return;
+  }
   const int16_t *off = ts.color_off + (int64_t)tile * (ts.max_colors + 1);
-  const uint16_t *ln = ts.lnodes + slot * NNE;
   for (int c = 0; c < ts.max_colors; ++c) {
     const int lo = off[c], hi = off[c + 1];
     if (lo >= hi) break;  // colors are numbered 0..ncolors-1 without gaps
     if ((int)threadIdx.x >= lo && (int)threadIdx.x < hi) {
 #pragma unroll
       for (int a = 0; a < NNE; ++a) {
-        T *dst = acc + (int)ln[a] * NC;
+        T *dst = acc + l[a] * NC;
 #pragma unroll
         for (int k = 0; k < NC; ++k) dst[k] += ye[a][k];
       }
@@ -473,7 +537,6 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
 
 template <int KIND, class Dens, class T, int MODE, bool AFFINE>
 __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restrict__ coords,
-                                                         const int32_t *__restrict__ conn,
                                                          QuadRule quad, Dens dens,
                                                          const T *__restrict__ u,
                                                          const T *__restrict__ v,
@@ -482,9 +545,8 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
                                                          const int *__restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ unsigned char lds_raw[];
-  tile_assemble<KIND, Dens, T, MODE, AFFINE>((int)xcd_block(blockIdx.x, gridDim.x), coords, conn, quad,
-                                     dens, u, v, dof_map, ts, partial,
-                                     reinterpret_cast<T *>(lds_raw));
+  tile_assemble<KIND, Dens, T, MODE, AFFINE>((int)xcd_block(blockIdx.x, gridDim.x), coords, quad, dens,
+                                             u, v, dof_map, ts, partial, lds_raw);
 }
 
 // Sum of node i's tile slots (ascending tile).

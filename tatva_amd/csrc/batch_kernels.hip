@@ -74,9 +74,8 @@ __global__ __launch_bounds__(kTile) void k_batch_assemble(const BatchSlot *__res
   if (S.st->done) return;
   extern __shared__ unsigned char lds_raw[];
   const Dens dens = make_density<Dens>(S.dens);
-  tile_assemble<KIND, Dens, float, kModeHvp, AFFINE>(b - S.a_blk0, S.coords, S.conn, quad, dens, S.u, S.z,
-                                             S.dof_map, S.tiles, S.partial,
-                                             reinterpret_cast<float *>(lds_raw));
+  tile_assemble<KIND, Dens, float, kModeHvp, AFFINE>(b - S.a_blk0, S.coords, quad, dens, S.u, S.z,
+                                                     S.dof_map, S.tiles, S.partial, lds_raw);
 }
 
 template <int NC>
@@ -434,6 +433,9 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   TATVA_REQUIRE(dof_map || n == n_full, "without dof_map n must equal the number of dofs");
   const SlotLayout L = slot_layout(m, ncomp, n);
   TATVA_REQUIRE(workspace && workspace_bytes >= (int64_t)L.total, "batch slot workspace too small");
+  const size_t lds = tile_lds_bytes(m->kind, m->tiles.max_local, m->dim, m->nne, ncomp, sizeof(float),
+                                    kModeHvp);
+  TATVA_REQUIRE(lds <= kTileLdsMax, "mesh tiles need more LDS than a workgroup holds");
   std::lock_guard<std::mutex> lock(B->mu);
   TATVA_TRY_HIP(hipSetDevice(B->device));
   hipStream_t s = (hipStream_t)stream;
@@ -478,7 +480,7 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   mi.ncomp = ncomp;
   mi.quad = m->quad;
   mi.affine = m->hex_affine;
-  mi.lds = (size_t)m->tiles.max_local * ncomp * sizeof(float);
+  mi.lds = lds;
   mi.mesh = m;
   B->dirty = true;
   return TATVA_OK;

@@ -130,6 +130,62 @@ __device__ __forceinline__ T sum_partials(const T *partials, int n) {
   return block_sum_all(v);
 }
 
+// Three partial sums at once, one barrier: every thread adds the four wave
+// sums itself in the same order, so all threads of all blocks get the same
+// bits.  Call at most once per kernel (its LDS is not re-armed).
+__device__ __forceinline__ void sum_partials3(const double *__restrict__ a, int na,
+                                              const double *__restrict__ b, int nb,
+                                              const double *__restrict__ c, int nc, double &sa,
+                                              double &sb, double &sc) {
+  __shared__ double sm[3][kBlock / kWave];
+  double va = 0.0, vb = 0.0, vc = 0.0;
+  for (int i = threadIdx.x; i < na; i += blockDim.x) va += a[i];
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) vb += b[i];
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) vc += c[i];
+  va = wave_sum(va);
+  vb = wave_sum(vb);
+  vc = wave_sum(vc);
+  const int lane = threadIdx.x % kWave;
+  const int wid = threadIdx.x / kWave;
+  if (lane == 0) {
+    sm[0][wid] = va;
+    sm[1][wid] = vb;
+    sm[2][wid] = vc;
+  }
+  __syncthreads();
+  sa = sb = sc = 0.0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    sa += sm[0][w];
+    sb += sm[1][w];
+    sc += sm[2][w];
+  }
+}
+
+// Two block sums at once (one barrier); results valid in thread 0.
+__device__ __forceinline__ void block_sum2(double &a, double &b) {
+  __shared__ double sm[2][kBlock / kWave];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x % kWave;
+  const int wid = threadIdx.x / kWave;
+  if (lane == 0) {
+    sm[0][wid] = a;
+    sm[1][wid] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      x += sm[0][w];
+      y += sm[1][w];
+    }
+    a = x;
+    b = y;
+  }
+}
+
 // out[0] = sum (subtract == 0) or out[0] -= sum of partials[0..n): one block.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_finalize_sum(const T *__restrict__ partials, int n,

@@ -619,6 +619,9 @@ struct HostTiles {
   std::vector<int32_t> node_ptr;    // n_tiles + 1
   std::vector<int32_t> slot_ptr;    // n_nodes + 1
   std::vector<int32_t> slot_list;   // slots, ascending tile per node
+  std::vector<int32_t> tnodes;      // global node of every slot
+  std::vector<uint16_t> inc_end;    // (incidence lists only) n_slots
+  std::vector<uint16_t> inc_list;   // (incidence lists only) n_tiles * kTile * nne
 };
 
 uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
@@ -635,7 +638,7 @@ uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
 // order, local node numbering per tile, greedy element coloring inside each
 // tile (elements of a color share no node), node -> slot lists.
 int build_tiles(const double *coords, int dim, int64_t N, const int32_t *elements, int64_t E,
-                int nne, HostTiles &T) {
+                int nne, bool incidence, HostTiles &T) {
   std::vector<double> cen((size_t)E * 3, 0.0);
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (int64_t e = 0; e < E; ++e)
@@ -667,9 +670,10 @@ int build_tiles(const double *coords, int dim, int64_t N, const int32_t *element
   T.n_tiles = (int32_t)n_tiles;
   T.elems.assign((size_t)n_tiles * kTile, -1);
   T.lnodes.assign((size_t)n_tiles * kTile * nne, 0);
+  if (incidence) T.inc_list.assign((size_t)n_tiles * kTile * nne, 0);
   T.node_ptr.assign((size_t)n_tiles + 1, 0);
   std::vector<std::vector<int16_t>> offs((size_t)n_tiles);
-  std::vector<int32_t> tile_nodes;  // global node id of every slot
+  std::vector<int32_t> &tile_nodes = T.tnodes;  // global node id of every slot
   std::vector<int32_t> loc;         // scratch
   std::vector<uint64_t> used;       // per local node: color bitmask (128 colors)
   for (int64_t t = 0; t < n_tiles; ++t) {
@@ -726,6 +730,17 @@ int build_tiles(const double *coords, int dim, int64_t N, const int32_t *element
       const int64_t src = ord[(size_t)k];
       T.elems[(size_t)(b + k)] = key[(size_t)(b + src)].second;
       for (int a = 0; a < nne; ++a) T.lnodes[(size_t)((b + k) * nne + a)] = ln[(size_t)src][a];
+    }
+    if (incidence) {
+      // entries k * nne + a of each local node, ascending (a counting sort)
+      std::vector<int32_t> cntn(loc.size() + 1, 0);
+      for (int64_t q = 0; q < cnt * nne; ++q) cntn[(size_t)T.lnodes[(size_t)(b * nne + q)] + 1]++;
+      for (size_t j = 0; j < loc.size(); ++j) cntn[j + 1] += cntn[j];
+      for (size_t j = 0; j < loc.size(); ++j) T.inc_end.push_back((uint16_t)cntn[j + 1]);
+      for (int64_t q = 0; q < cnt * nne; ++q) {
+        const uint16_t l = T.lnodes[(size_t)(b * nne + q)];
+        T.inc_list[(size_t)(b * nne + cntn[l]++)] = (uint16_t)q;
+      }
     }
     T.node_ptr[(size_t)t + 1] = T.node_ptr[(size_t)t] + (int32_t)loc.size();
     tile_nodes.insert(tile_nodes.end(), loc.begin(), loc.end());
@@ -827,7 +842,7 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
   };
   HostTiles T;
   {
-    const int rc = build_tiles(coords, dim, n_nodes, elements, n_elements, nne, T);
+    const int rc = build_tiles(coords, dim, n_nodes, elements, n_elements, nne, elem_kind == TATVA_TET4, T);
     if (rc) {
       delete m;
       return rc;
@@ -861,8 +876,14 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
   if ((err = upload(&m->t_node_ptr, T.node_ptr, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_slot_ptr, T.slot_ptr, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_slot_list, T.slot_list, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if ((err = upload(&m->t_tnodes, T.tnodes, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if (elem_kind == TATVA_TET4) {
+    if ((err = upload(&m->t_inc_end, T.inc_end, "tiles")) != hipSuccess) return fail(err, "tiles");
+    if ((err = upload(&m->t_inc_list, T.inc_list, "tiles")) != hipSuccess) return fail(err, "tiles");
+  }
   m->tiles = TileSet{T.n_tiles, T.max_local, T.max_colors, 0, m->t_elems, m->t_lnodes,
-                     m->t_color_off, m->t_node_ptr, m->t_slot_ptr, m->t_slot_list};
+                     m->t_color_off, m->t_node_ptr, m->t_slot_ptr, m->t_slot_list,
+                     m->t_tnodes, m->t_inc_end, m->t_inc_list};
   if ((err = hipMalloc(&m->coords, n_nodes * dim * sizeof(double))) != hipSuccess)
     return fail(err, "hipMalloc coords");
   if ((err = hipMalloc(&m->conn, total * sizeof(int32_t))) != hipSuccess)
@@ -899,7 +920,8 @@ extern "C" int tatva_mesh_destroy(tatva_mesh *m) {
   if (m->scratch) (void)hipFree(m->scratch);
   if (m->partials) (void)hipFree(m->partials);
   for (void *p : {(void *)m->t_elems, (void *)m->t_lnodes, (void *)m->t_color_off,
-                  (void *)m->t_node_ptr, (void *)m->t_slot_ptr, (void *)m->t_slot_list})
+                  (void *)m->t_node_ptr, (void *)m->t_slot_ptr, (void *)m->t_slot_list,
+                  (void *)m->t_tnodes, (void *)m->t_inc_end, (void *)m->t_inc_list})
     if (p) (void)hipFree(p);
   delete m;
   return TATVA_OK;
@@ -1040,13 +1062,17 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
     return with_density<Elem<K>::D, T>(d, [&](auto dens) -> int {
       using Dens = decltype(dens);
       constexpr int NC = Dens::NC;
-      const size_t lds = (size_t)m->tiles.max_local * NC * sizeof(T);
+      const size_t lds = tile_lds_bytes(K, m->tiles.max_local, Elem<K>::D, Elem<K>::NNE, NC, sizeof(T), MODE);
+      if (lds > kTileLdsMax) {
+        set_error("mesh tiles need more LDS than a workgroup holds");
+        return TATVA_EINVAL;
+      }
       if (K == TATVA_HEX8 && m->hex_affine)
         k_tile_assemble<K, Dens, T, MODE, true><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->conn, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
       else
         k_tile_assemble<K, Dens, T, MODE, false><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->conn, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
       if (part_dot)
         k_slot_gather_dot<NC, T><<<gather_blocks, kBlock, 0, s>>>(
             m->n_nodes, m->t_slot_ptr, m->t_slot_list, partial, dof_map, v, out, part_dot, skip);

@@ -37,6 +37,9 @@ struct tatva_mesh {
   int32_t *t_node_ptr;
   int32_t *t_slot_ptr;
   int32_t *t_slot_list;
+  int32_t *t_tnodes;
+  uint16_t *t_inc_end;
+  uint16_t *t_inc_list;
 };
 
 namespace tatva {

@@ -80,32 +80,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t *__res
   }
 }
 
-// K1 (CSR): Ap = A p and the block's share of p.Ap.
-template <int LPR>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int64_t n, const int32_t *__restrict__ indptr,
-                                                     const int32_t *__restrict__ indices,
-                                                     const double *__restrict__ data,
-                                                     const double *__restrict__ p,
-                                                     double *__restrict__ Ap,
-                                                     double *__restrict__ part_pap,
-                                                     const PcgState *__restrict__ st) {
-  if (st->done) return;
-  constexpr int G = kBlock / LPR;
-  const int lane = threadIdx.x % LPR;
-  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t stride = (int64_t)gridDim.x * G;
-  double dot = 0.0;
-  for (int64_t row = lb * G + threadIdx.x / LPR; row < n; row += stride) {
-    const double s = csr_row_dot<LPR>(row, lane, indptr, indices, data, p);
-    if (lane == 0) {
-      Ap[row] = s;
-      dot += s * p[row];
-    }
-  }
-  dot = block_sum(dot);
-  if (threadIdx.x == 0) part_pap[blockIdx.x] = dot;
-}
-
 // ------------------------------------------------------ node-block SpMV ----
 //
 // FEM patterns with B dofs per node have rows in groups of B (one node) that
@@ -233,6 +207,119 @@ __global__ __launch_bounds__(kBlock) void k_block_view(int64_t nbr, const int32_
   }
 }
 
+// ------------------------------------------------------- sliced ELL view ----
+//
+// A CSR matrix whose rows do not form node blocks (a node with some of its
+// dofs constrained, a scalar field) is re-laid for the PCG as sliced ELL:
+// slices of kSellRows consecutive rows, slice s padded to the length J_s of
+// its longest row, entry j of row kSellRows s + l at (soff[s] + j) kSellRows
+// + l (padding: value 0, column = the row itself).  One wave streams a slice
+// with one row per lane, so every load instruction reads kSellRows
+// consecutive values and no lane waits on a row pointer between entries.
+// Each row is summed in CSR order (ascending column), as k_spmv does.
+constexpr int kSellRows = kWave;
+constexpr int kSellSlicesPerBlock = kBlock / kSellRows;
+
+// soff[s + 1] = J_s (longest row of slice s); soff[0] = 0.
+__global__ __launch_bounds__(kBlock) void k_sell_len(int64_t n, int64_t nsl,
+                                                    const int32_t *__restrict__ indptr,
+                                                    int64_t *__restrict__ soff) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) soff[0] = 0;
+  if (s >= nsl) return;
+  const int64_t r0 = s * kSellRows;
+  const int64_t r1 = r0 + kSellRows < n ? r0 + kSellRows : n;
+  int32_t J = 0;
+  for (int64_t i = r0; i < r1; ++i) J = max(J, indptr[i + 1] - indptr[i]);
+  soff[s + 1] = J;
+}
+
+// In-place inclusive scan of a[0..m) by one block of kBlock threads (each
+// thread scans one contiguous chunk, then adds the sum of the chunks before).
+__global__ __launch_bounds__(kBlock) void k_scan_inplace(int64_t *__restrict__ a, int64_t m) {
+  __shared__ int64_t tot[kBlock];
+  const int64_t per = (m + kBlock - 1) / kBlock;
+  const int64_t lo = threadIdx.x * per;
+  const int64_t hi = lo + per < m ? lo + per : m;
+  int64_t run = 0;
+  for (int64_t i = lo; i < hi; ++i) {
+    run += a[i];
+    a[i] = run;
+  }
+  tot[threadIdx.x] = run;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int t = 1; t < kBlock; ++t) tot[t] += tot[t - 1];
+  __syncthreads();
+  const int64_t before = threadIdx.x > 0 ? tot[threadIdx.x - 1] : 0;
+  for (int64_t i = lo; i < hi; ++i) a[i] += before;
+}
+
+// One wave per slice: lane l copies row kSellRows s + l into the slice.
+__global__ __launch_bounds__(kBlock) void k_sell_fill(int64_t n, int64_t nsl,
+                                                     const int32_t *__restrict__ indptr,
+                                                     const int32_t *__restrict__ indices,
+                                                     const double *__restrict__ data,
+                                                     const int64_t *__restrict__ soff,
+                                                     int32_t *__restrict__ cols,
+                                                     double *__restrict__ vals) {
+  const int l = threadIdx.x % kSellRows;
+  const int64_t s = (int64_t)blockIdx.x * kSellSlicesPerBlock + threadIdx.x / kSellRows;
+  if (s >= nsl) return;
+  const int64_t i = s * kSellRows + l;
+  const int32_t base = i < n ? indptr[i] : 0;
+  const int32_t len = i < n ? indptr[i + 1] - base : 0;
+  const int64_t o = soff[s];
+  const int J = (int)(soff[s + 1] - o);
+  const int32_t self = i < n ? (int32_t)i : 0;
+  for (int j = 0; j < J; ++j) {
+    const int64_t pos = (o + j) * kSellRows + l;
+    cols[pos] = j < len ? indices[base + j] : self;
+    vals[pos] = j < len ? data[base + j] : 0.0;
+  }
+}
+
+// y = A x (b given: y = b - A x); with part: the block's share of y . x.
+// One wave per slice; skipped when st says the solve is done.
+__global__ __launch_bounds__(kBlock) void k_sell_spmv(int64_t n, int64_t nsl,
+                                                     const int64_t *__restrict__ soff,
+                                                     const int32_t *__restrict__ cols,
+                                                     const double *__restrict__ vals,
+                                                     const double *__restrict__ x,
+                                                     const double *__restrict__ b,
+                                                     double *__restrict__ y,
+                                                     double *__restrict__ part,
+                                                     const PcgState *__restrict__ st) {
+  if (st && st->done) return;
+  const int l = threadIdx.x % kSellRows;
+  const int64_t s = xcd_block(blockIdx.x, gridDim.x) * kSellSlicesPerBlock + threadIdx.x / kSellRows;
+  double dot = 0.0;
+  if (s < nsl) {
+    const int64_t o = soff[s];
+    const int J = (int)(soff[s + 1] - o);
+    const int32_t *__restrict__ c = cols + o * kSellRows + l;
+    const double *__restrict__ v = vals + o * kSellRows + l;
+    double a = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < J; ++j) a += v[j * kSellRows] * x[c[j * kSellRows]];
+    const int64_t i = s * kSellRows + l;
+    if (i < n) {
+      const double yi = b ? b[i] - a : a;
+      y[i] = yi;
+      dot = yi * x[i];
+    }
+  }
+  if (part) {
+    dot = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = dot;
+  }
+}
+
+static int64_t sell_slices(int64_t n) { return (n + kSellRows - 1) / kSellRows; }
+static int sell_grid(int64_t n) {
+  return (int)std::max<int64_t>(1, (sell_slices(n) + kSellSlicesPerBlock - 1) / kSellSlicesPerBlock);
+}
+
 // ------------------------------------------------------------ PCG steps ----
 
 // K1 for a matrix-free operator: the block's share of p.Ap.
@@ -323,6 +410,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict
     p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
+// Elements per thread whose loads the CG step issues before its scalar sums
+// (vec_grid caps the grid at 1024 blocks: 4 cover n <= 1,048,576 in one pass).
+constexpr int kStepPrefetch = 4;
+
 // KB of the single-reduction form (see the file comment).  u is D^-1 r.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x, T *__restrict__ r,
@@ -338,9 +429,28 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
                                                     PcgState *__restrict__ st) {
   if (st->done) return;
   const int iter = st->it[q ^ 1];
-  const double gamma = sum_partials(part_g_old, gridDim.x);
-  const double rr = sum_partials(part_rr_old, gridDim.x);
-  const double delta = sum_partials(part_delta, n_delta);
+  // the first kStepPrefetch elements of this thread are loaded before the
+  // scalars are summed, so their latency overlaps the partial sums
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  T ru[kStepPrefetch], rp[kStepPrefetch], rw[kStepPrefetch], rs[kStepPrefetch];
+  T rx[kStepPrefetch], rv[kStepPrefetch], rd[kStepPrefetch];
+#pragma unroll
+  for (int k = 0; k < kStepPrefetch; ++k) {
+    const int64_t i = i0 + k * stride;
+    if (i < n) {
+      ru[k] = u[i];
+      rp[k] = p[i];
+      rw[k] = w[i];
+      rs[k] = s[i];
+      rx[k] = x[i];
+      rv[k] = r[i];
+      rd[k] = dinv[i];
+    }
+  }
+  double gamma, rr, delta;
+  sum_partials3(part_g_old, gridDim.x, part_rr_old, gridDim.x, part_delta, n_delta, gamma, rr,
+                delta);
   const bool first = iter == 0;
   const double beta = first ? 0.0 : gamma / st->gamma[q ^ 1];
   const double denom = first ? delta : delta - beta * gamma / st->alpha[q ^ 1];
@@ -366,8 +476,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
   }
   const T a = T(alpha), bt = T(beta);
   double sg = 0.0, srr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  // p and s hold no values before iteration 0: selected away, never scaled
+#pragma unroll
+  for (int k = 0; k < kStepPrefetch; ++k) {
+    const int64_t i = i0 + k * stride;
+    if (i < n) {
+      const T pi = first ? ru[k] : ru[k] + bt * rp[k];
+      const T si = first ? rw[k] : rw[k] + bt * rs[k];
+      p[i] = pi;
+      s[i] = si;
+      x[i] = rx[k] + a * pi;
+      const T ri = rv[k] - a * si;
+      r[i] = ri;
+      const T ui = rd[k] * ri;
+      u[i] = ui;
+      sg += (double)ri * (double)ui;
+      srr += (double)ri * (double)ri;
+    }
+  }
+  for (int64_t i = i0 + kStepPrefetch * stride; i < n; i += stride) {
     const T pi = first ? u[i] : u[i] + bt * p[i];
     const T si = first ? w[i] : w[i] + bt * s[i];
     p[i] = pi;
@@ -380,8 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
     sg += (double)ri * (double)ui;
     srr += (double)ri * (double)ri;
   }
-  sg = block_sum(sg);
-  srr = block_sum(srr);
+  block_sum2(sg, srr);
   if (threadIdx.x == 0) {
     part_g_new[blockIdx.x] = sg;
     part_rr_new[blockIdx.x] = srr;
@@ -517,7 +643,7 @@ __global__ void k_state_reset(PcgState *st) {
 struct GraphKey {
   const void *ws;
   const void *a, *b, *c, *d, *e;  // operator arrays (CSR / mesh / state)
-  int64_t n, m;
+  int64_t n, m, k;
   int i0, i1, i2, i3;
   double f0, f1, f2;  // density parameters baked into captured kernel arguments
   bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
@@ -545,6 +671,11 @@ struct DeviceRes {
   hipEvent_t prof_a = nullptr, prof_b = nullptr;  // timing events around a sampled operator
   PcgState *host_state = nullptr;
   CachedGraph csr, matfree;
+  // sliced ELL copy of the CSR operator (grown on demand, kept until exit)
+  int64_t *sell_soff = nullptr;
+  size_t sell_soff_bytes = 0;
+  void *sell_data = nullptr;
+  size_t sell_data_bytes = 0;
 };
 
 static DeviceRes g_res[64];
@@ -576,6 +707,53 @@ static int get_res(int device, DeviceRes **out) {
     r.init = true;
   }
   *out = &r;
+  return TATVA_OK;
+}
+
+static hipError_t grow(void **buf, size_t *have, size_t need) {
+  if (*have >= need) return hipSuccess;
+  if (*buf) {
+    const hipError_t e = hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    if (e != hipSuccess) return e;
+  }
+  const hipError_t e = hipMalloc(buf, need);
+  if (e == hipSuccess) *have = need;
+  return e;
+}
+
+struct SellView {
+  int64_t nsl = 0, steps = 0;
+  const int64_t *soff = nullptr;
+  const int32_t *cols = nullptr;
+  const double *vals = nullptr;
+};
+
+// Lays the CSR matrix out as sliced ELL in the device's buffers (one host
+// synchronisation: the padded size decides the buffer size).
+static int build_sell(DeviceRes *res, int64_t n, const int32_t *indptr, const int32_t *indices,
+                      const double *data, hipStream_t s, SellView &V) {
+  V.nsl = sell_slices(n);
+  TATVA_TRY_HIP(grow((void **)&res->sell_soff, &res->sell_soff_bytes,
+                     (size_t)(V.nsl + 1) * sizeof(int64_t)));
+  k_sell_len<<<grid_for(V.nsl), kBlock, 0, s>>>(n, V.nsl, indptr, res->sell_soff);
+  k_scan_inplace<<<1, kBlock, 0, s>>>(res->sell_soff, V.nsl + 1);
+  TATVA_CHECK_LAUNCH();
+  TATVA_TRY_HIP(hipMemcpyAsync(&V.steps, res->sell_soff + V.nsl, sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s));
+  TATVA_TRY_HIP(hipStreamSynchronize(s));
+  const size_t entries = (size_t)V.steps * kSellRows;
+  TATVA_TRY_HIP(grow(&res->sell_data, &res->sell_data_bytes,
+                     std::max<size_t>(1, entries * (sizeof(double) + sizeof(int32_t)))));
+  double *vals = (double *)res->sell_data;
+  int32_t *cols = (int32_t *)(vals + entries);
+  k_sell_fill<<<sell_grid(n), kBlock, 0, s>>>(n, V.nsl, indptr, indices, data, res->sell_soff,
+                                              cols, vals);
+  TATVA_CHECK_LAUNCH();
+  V.soff = res->sell_soff;
+  V.cols = cols;
+  V.vals = vals;
   return TATVA_OK;
 }
 
@@ -643,30 +821,21 @@ static WsLayout ws_layout(int64_t n, int nvec, size_t esz, int64_t n_op_parts = 
   return L;
 }
 
-// out[0] = sum of p[0..n) in a fixed order (one block of 1024 threads, four
-// independent strided accumulators per thread).
-__global__ __launch_bounds__(1024) void k_sum_big(const double *__restrict__ p, int n,
-                                                  double *__restrict__ out,
-                                                  const PcgState *__restrict__ st) {
+// Blocks' shares of a dot product (more than a CG step re-sums) reduced to
+// kShareBlocks partials: block k sums the contiguous k-th part of p[0..n) in
+// a fixed order, out[k] = its sum.
+constexpr int kShareBlocks = 64;
+__global__ __launch_bounds__(kBlock) void k_sum_shares(const double *__restrict__ p, int n,
+                                                      double *__restrict__ out,
+                                                      const PcgState *__restrict__ st) {
   if (st->done) return;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int i = threadIdx.x;
-  for (; i + 3 * 1024 < n; i += 4 * 1024) {
-    a0 += p[i];
-    a1 += p[i + 1024];
-    a2 += p[i + 2048];
-    a3 += p[i + 3072];
-  }
-  for (; i < n; i += 1024) a0 += p[i];
-  double v = wave_sum((a0 + a1) + (a2 + a3));
-  __shared__ double sm[1024 / kWave];
-  if (threadIdx.x % kWave == 0) sm[threadIdx.x / kWave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < 1024 / kWave; ++w) t += sm[w];
-    out[0] = t;
-  }
+  const int per = (n + kShareBlocks - 1) / kShareBlocks;
+  const int lo = blockIdx.x * per;
+  const int hi = min(lo + per, n);
+  double v = 0.0;
+  for (int i = lo + threadIdx.x; i < hi; i += kBlock) v += p[i];
+  v = block_sum(v);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
 // Typed views of a workspace; base is the workspace's first byte.
@@ -888,31 +1057,30 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
   TATVA_TRY_HIP(hipMemcpyAsync(x, x_io, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipMemcpyAsync(b, b_in, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  const int lpr = choose_lpr(n, nnz);
   const int gv = vec_grid(n);
   const bool profiling = g_profiling.load() != 0;
   const int nblk = block_cols ? block_size : 1;
-  // operator launch geometry: node-block view when given, else CSR
-  const int g_op = nblk > 1 ? spmv_grid_full(n / nblk, kBLanes)
-                            : with_lpr(lpr, [&](auto lc) -> int {
-                                return spmv_grid_full(n, decltype(lc)::value);
-                              });
-  // more blocks than the partial buffers hold: the shares go to op_parts and a
-  // one-block sum writes their total to P.pap[0]
+  // operator: node-block view when given, else a sliced ELL copy of the CSR
+  SellView sv_ell;
+  if (nblk == 1) {
+    rc = build_sell(res, n, indptr, indices, data, s, sv_ell);
+    if (rc) return rc;
+  }
+  const int g_op = nblk > 1 ? spmv_grid_full(n / nblk, kBLanes) : sell_grid(n);
+  // more blocks than the partial buffers hold: the shares go to op_parts and
+  // k_sum_shares reduces them to kShareBlocks partials in P.pap
   const bool big = g_op > kMaxReduceBlocks;
   TATVA_REQUIRE(g_op <= csr_op_parts(n), "SpMV grid exceeds the workspace shares");
   double *op_part = big ? reinterpret_cast<double *>(wsb + L.op_parts) : P.pap;
-  const int n_delta = big ? 1 : g_op;
+  const int n_delta = big ? kShareBlocks : g_op;
   auto apply_op = [&](const double *xin, const double *bin, double *yout) -> int {
     if (nblk == 3)
       k_bspmv<3><<<g_op, kBlock, 0, s>>>(n / 3, indptr, block_cols, data, xin, bin, yout);
     else if (nblk == 2)
       k_bspmv<2><<<g_op, kBlock, 0, s>>>(n / 2, indptr, block_cols, data, xin, bin, yout);
     else
-      with_lpr(lpr, [&](auto lc) -> int {
-        k_spmv<decltype(lc)::value><<<g_op, kBlock, 0, s>>>(n, indptr, indices, data, xin, bin, yout);
-        return 0;
-      });
+      k_sell_spmv<<<g_op, kBlock, 0, s>>>(n, sv_ell.nsl, sv_ell.soff, sv_ell.cols, sv_ell.vals, xin,
+                                          bin, yout, nullptr, nullptr);
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   };
@@ -923,16 +1091,13 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
     else if (nblk == 2)
       k_pcg_bspmv<2><<<g_op, kBlock, 0, s>>>(n / 2, indptr, block_cols, data, u, w, op_part, st);
     else
-      with_lpr(lpr, [&](auto lc) -> int {
-        k_pcg_spmv<decltype(lc)::value><<<g_op, kBlock, 0, s>>>(n, indptr, indices, data, u, w,
-                                                                op_part, st);
-        return 0;
-      });
+      k_sell_spmv<<<g_op, kBlock, 0, s>>>(n, sv_ell.nsl, sv_ell.soff, sv_ell.cols, sv_ell.vals, u,
+                                          nullptr, w, op_part, st);
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   };
   const std::function<int()> sum_shares = [&]() -> int {
-    if (big) k_sum_big<<<1, 1024, 0, s>>>(op_part, g_op, P.pap, st);
+    if (big) k_sum_shares<<<kShareBlocks, kBlock, 0, s>>>(op_part, g_op, P.pap, st);
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   };
@@ -964,13 +1129,14 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   key.a = indptr;
   key.b = indices;
   key.c = data;
-  key.d = block_cols;
+  key.d = block_cols ? (const void *)block_cols : (const void *)sv_ell.soff;
+  key.e = sv_ell.vals;
   key.n = n;
   key.m = nnz;
+  key.k = sv_ell.steps;
   key.i0 = nblk;
   key.i1 = chunk;
   key.i2 = profiling ? 1 : 0;
-  key.i3 = lpr;
   rc = chunk_graph(s, chunk, step, profiling, key, res->csr);
   if (rc == TATVA_OK)
     rc = run_chunks(res, res->csr.exec, st, maxiter, chunk, profiling, spmv_dot, stats,

@@ -336,6 +336,44 @@ def test_pcg_matches_spsolve_and_oracle_iterations():
     assert info.iterations == 3 and not info.converged
 
 
+def test_pcg_many_spmv_blocks_both_layouts():
+    """n = 545,832: both operators run more SpMV blocks than the CG step re-sums,
+    so their shares pass through the 64-block share sum.  The node-block view and
+    the sliced ELL copy solve the same system; a pattern with one dof of the
+    x = 1 face constrained (not node-blocked) solves through the sliced ELL."""
+    import torch
+
+    m = ta.Mesh.unit_cube(56)
+    op = ta.Operator(m, ta.Hexahedron8())
+    E = op.energy(energies.LinearElasticity(0.4, 0.6))
+    x0n = np.where(np.isclose(m.coords[:, 0], 0.0))[0]
+    x1n = np.where(np.isclose(m.coords[:, 0], 1.0))[0]
+    L = lifter.Lifter(m.n_nodes * 3, lifter.Fixed((x0n[:, None] * 3 + np.arange(3)).ravel()))
+    cm = sparse.ColoredMatrix.from_csr(L.adapt_sparsity(sparse.pattern_from_mesh(m, 3)))
+    K = sparse.jacfwd(E.reduced(L).grad, cm)(np.zeros(L.size_reduced))
+    n = L.size_reduced
+    assert n == 545832 and n % 64 != 0
+    b = np.random.default_rng(11).standard_normal(n)
+    bt = torch.as_tensor(b, device="cuda")
+    xs, infos = solvers.pcg(K, b, tol=1e-10, chunk=16)
+    xb, infob = solvers.pcg(K, b, tol=1e-10, chunk=16, block_size=3)
+    assert infos.converged and infob.converged
+    assert abs(infos.iterations - infob.iterations) <= 1
+    for x in (xs, xb):
+        res = float(torch.linalg.norm(bt - K.matvec(x)) / torch.linalg.norm(bt))
+        assert res <= 2e-10
+    assert rel_err(to_np(xs), to_np(xb)) < 1e-8
+    L2 = lifter.Lifter(m.n_nodes * 3, lifter.Fixed(np.concatenate(
+        [(x0n[:, None] * 3 + np.arange(3)).ravel(), x1n * 3])))
+    cm2 = sparse.ColoredMatrix.from_csr(L2.adapt_sparsity(sparse.pattern_from_mesh(m, 3)))
+    assert cm2.block_view(3) is None
+    K2 = sparse.jacfwd(E.reduced(L2).grad, cm2)(np.zeros(L2.size_reduced))
+    b2 = torch.as_tensor(np.random.default_rng(12).standard_normal(L2.size_reduced), device="cuda")
+    x2, info2 = solvers.pcg(K2, b2, tol=1e-10, chunk=16)
+    assert info2.converged
+    assert float(torch.linalg.norm(b2 - K2.matvec(x2)) / torch.linalg.norm(b2)) <= 2e-10
+
+
 def test_newton_poisson_tri3_matches_oracle():
     n = 16
     m = ta.Mesh.unit_square(n, n)
