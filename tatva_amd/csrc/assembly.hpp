@@ -36,12 +36,11 @@ struct TileSet {
   int32_t max_local;       // largest number of local nodes in a tile
   int32_t max_colors;      // largest number of element colors in a tile
   int32_t pad;
-  const int32_t *elems;    // (n_tiles * kTile) element id per tile slot, -1 for padding
   const uint16_t *lnodes;  // (n_tiles * kTile * nne) local node index of each element node
   const int16_t *color_off;  // (n_tiles * (max_colors + 1)) start of each color in the tile
   const int32_t *node_ptr;   // (n_tiles + 1) first slot of each tile
   const int32_t *slot_ptr;   // (n_nodes + 1) node -> its slots
-  const int32_t *slot_list;  // slots of each node, ascending tile
+  const int32_t *slot_pos;   // (n_slots) node-major position of each tile slot's sums
   const int32_t *tnodes;     // (n_slots) global node of each slot
   // Tet4 only: entries (slot * nne + a) of each local node, ascending, at
   // inc_list[tile * kTile * nne + ...]; local node j ends at inc_end[node_ptr + j]
@@ -410,21 +409,31 @@ inline bool hex_is_affine_host(const double (&X)[8][3]) {
 // ------------------------------------------------------ tile assembly ------
 
 // LDS of one tile: per local node its coordinates (FP64), then the
-// accumulator, the state and (HVP) the tangent in a T of tsize bytes; with
-// the incidence-list sum, at least the element vectors (kTile x nne x NC).
+// accumulator, the state and (HVP) the tangent in a T of tsize bytes.  With
+// the incidence-list sum the element vectors (kTile x nne x NC) reuse that
+// space, and the tile's incidence lists follow it.
+constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+constexpr size_t tile_stage_bytes(int max_local, int D, int NC, size_t tsize, int mode) {
+  return (size_t)max_local * (D * sizeof(double) + (mode == kModeHvp ? 3 : 2) * NC * tsize);
+}
+constexpr size_t tile_union_bytes(int max_local, int D, int nne, int NC, size_t tsize, int mode) {
+  const size_t staged = tile_stage_bytes(max_local, D, NC, tsize, mode);
+  const size_t elem = (size_t)kTile * nne * NC * tsize;
+  return align16(elem > staged ? elem : staged);
+}
 constexpr size_t tile_lds_bytes(int kind, int max_local, int D, int nne, int NC, size_t tsize,
                                 int mode) {
-  const size_t staged = (size_t)max_local * (D * sizeof(double) + (mode == kModeHvp ? 3 : 2) * NC * tsize);
-  const size_t elem = (size_t)kTile * nne * NC * tsize;
-  return kind == TATVA_TET4 && elem > staged ? elem : staged;
+  if (kind != TATVA_TET4) return tile_stage_bytes(max_local, D, NC, tsize, mode);
+  return tile_union_bytes(max_local, D, nne, NC, tsize, mode) + (size_t)kTile * nne * sizeof(uint16_t) +
+         align16((size_t)max_local * sizeof(uint16_t));
 }
 constexpr size_t kTileLdsMax = 65536;  // LDS one workgroup may allocate
 
 // Body of one tile (one workgroup of kTile threads; lds of tile_lds_bytes).  u: full state; v (HVP): tangent read through dof_map
 // (reduced, zero on constrained dofs).  The tile's local nodes are loaded into
 // LDS once (one thread per node), so each element reads its nodes from LDS
-// instead of gathering them from global memory.  Writes the tile's node sums
-// to partial[(node_ptr[tile] + j) * NC + c].
+// instead of gathering them from global memory.  Writes the sums of local node
+// j to partial[slot_pos[node_ptr[tile] + j] * NC + c] (node-major order).
 template <int KIND, class Dens, class T, int MODE, bool AFFINE>
 __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict__ coords,
                                               const QuadRule &quad, const Dens &dens,
@@ -459,19 +468,30 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
       }
     }
   }
+  // Tet4: the tile's incidence lists are loaded here and stored to LDS after
+  // the element work, so their latency overlaps it
+  uint64_t inc_pk = 0;
+  uint32_t inc_e = 0;
+  if constexpr (tile_gather<KIND>()) {
+    static_assert(NNE == 4, "four uint16 incidence entries per thread");
+    inc_pk = reinterpret_cast<const uint64_t *>(ts.inc_list + (int64_t)tile * kTile * NNE)[threadIdx.x];
+    if ((int)threadIdx.x < nloc) inc_e = ts.inc_end[nb + threadIdx.x];
+  }
   const int64_t slot = (int64_t)tile * kTile + threadIdx.x;
-  const int32_t e = ts.elems[slot];
+  const int16_t *off = ts.color_off + (int64_t)tile * (ts.max_colors + 1);
+  // the tile's elements fill its first off[max_colors] slots (the rest is padding)
+  const bool real = (int)threadIdx.x < (int)off[ts.max_colors];
   const uint16_t *ln = ts.lnodes + slot * NNE;
   int l[NNE];
 #pragma unroll
-  for (int a = 0; a < NNE; ++a) l[a] = e >= 0 ? (int)ln[a] : 0;
+  for (int a = 0; a < NNE; ++a) l[a] = real ? (int)ln[a] : 0;
   T ye[NNE][NC];
 #pragma unroll
   for (int a = 0; a < NNE; ++a)
 #pragma unroll
     for (int c = 0; c < NC; ++c) ye[a][c] = T(0);
   __syncthreads();
-  if (e >= 0) {
+  if (real) {
     double X[NNE][D];
     T ue[NNE][NC];
 #pragma unroll
@@ -500,24 +520,27 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
   if constexpr (tile_gather<KIND>()) {
     __syncthreads();  // every element has read its nodes: the LDS is free
     T *ys = reinterpret_cast<T *>(lds);
+    uint16_t *il = reinterpret_cast<uint16_t *>(
+        lds + tile_union_bytes(ts.max_local, D, NNE, NC, sizeof(T), MODE));
+    uint16_t *ie = il + kTile * NNE;
 #pragma unroll
     for (int a = 0; a < NNE; ++a)
 #pragma unroll
       for (int c = 0; c < NC; ++c) ys[((int)threadIdx.x * NNE + a) * NC + c] = ye[a][c];
+    reinterpret_cast<uint64_t *>(il)[threadIdx.x] = inc_pk;
+    if ((int)threadIdx.x < nloc) ie[threadIdx.x] = (uint16_t)inc_e;
+    for (int j = threadIdx.x + kTile; j < nloc; j += kTile) ie[j] = ts.inc_end[nb + j];
     __syncthreads();
-    const uint16_t *il = ts.inc_list + (int64_t)tile * kTile * NNE;
-    T *out = partial + (int64_t)nb * NC;
     for (int i = threadIdx.x; i < nloc * NC; i += kTile) {
       const int j = i / NC, c = i - j * NC;
-      const int q0 = j > 0 ? (int)ts.inc_end[nb + j - 1] : 0;
-      const int q1 = ts.inc_end[nb + j];
+      const int q0 = j > 0 ? (int)ie[j - 1] : 0;
+      const int q1 = ie[j];
       T sum = T(0);
       for (int q = q0; q < q1; ++q) sum += ys[(int)il[q] * NC + c];
-      out[i] = sum;
+      partial[(int64_t)ts.slot_pos[nb + j] * NC + c] = sum;
     }
     return;
   }
-  const int16_t *off = ts.color_off + (int64_t)tile * (ts.max_colors + 1);
   for (int c = 0; c < ts.max_colors; ++c) {
     const int lo = off[c], hi = off[c + 1];
     if (lo >= hi) break;  // colors are numbered 0..ncolors-1 without gaps
@@ -531,8 +554,10 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     }
     __syncthreads();
   }
-  T *out = partial + (int64_t)nb * NC;
-  for (int i = threadIdx.x; i < nloc * NC; i += kTile) out[i] = acc[i];
+  for (int i = threadIdx.x; i < nloc * NC; i += kTile) {
+    const int j = i / NC;
+    partial[(int64_t)ts.slot_pos[nb + j] * NC + (i - j * NC)] = acc[i];
+  }
 }
 
 template <int KIND, class Dens, class T, int MODE, bool AFFINE>
@@ -549,16 +574,16 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
                                              u, v, dof_map, ts, partial, lds_raw);
 }
 
-// Sum of node i's tile slots (ascending tile).
+// Sum of node i's tile slots (ascending tile): the tiles wrote them to the
+// node-major range [slot_ptr[i], slot_ptr[i + 1]) of the partial buffer.
 template <int NC, class T>
 __device__ __forceinline__ void slot_sum(int64_t i, const int32_t *__restrict__ slot_ptr,
-                                         const int32_t *__restrict__ slot_list,
                                          const T *__restrict__ partial, T (&acc)[NC]) {
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = T(0);
   const int32_t b = slot_ptr[i], en = slot_ptr[i + 1];
   for (int32_t k = b; k < en; ++k) {
-    const T *src = partial + (int64_t)slot_list[k] * NC;
+    const T *src = partial + (int64_t)k * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] += src[c];
   }
@@ -567,7 +592,6 @@ __device__ __forceinline__ void slot_sum(int64_t i, const int32_t *__restrict__ 
 // out[map(node*NC + c)] = sum of the node's tile slots (ascending tile) - f_ext.
 template <int NC, class T>
 __global__ __launch_bounds__(kBlock) void k_slot_gather(int64_t n_nodes, const int32_t *__restrict__ slot_ptr,
-                                                        const int32_t *__restrict__ slot_list,
                                                         const T *__restrict__ partial,
                                                         const double *__restrict__ f_ext,
                                                         const int32_t *__restrict__ dof_map,
@@ -577,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_slot_gather(int64_t n_nodes, const i
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= n_nodes) return;
   T acc[NC];
-  slot_sum<NC, T>(i, slot_ptr, slot_list, partial, acc);
+  slot_sum<NC, T>(i, slot_ptr, partial, acc);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int64_t dof = i * NC + c;
@@ -597,7 +621,6 @@ __global__ __launch_bounds__(kBlock) void k_slot_gather(int64_t n_nodes, const i
 // y = H p with p.Hp in one pass.
 template <int NC, class T>
 __global__ __launch_bounds__(kBlock) void k_slot_gather_dot(int64_t n_nodes, const int32_t *__restrict__ slot_ptr,
-                                                            const int32_t *__restrict__ slot_list,
                                                             const T *__restrict__ partial,
                                                             const int32_t *__restrict__ dof_map,
                                                             const T *__restrict__ w,
@@ -609,7 +632,7 @@ __global__ __launch_bounds__(kBlock) void k_slot_gather_dot(int64_t n_nodes, con
   for (int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n_nodes;
        i += (int64_t)gridDim.x * blockDim.x) {
     T acc[NC];
-    slot_sum<NC, T>(i, slot_ptr, slot_list, partial, acc);
+    slot_sum<NC, T>(i, slot_ptr, partial, acc);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int64_t dof = i * NC + c;

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather(const BatchSlot *__rest
   for (int64_t i = (int64_t)lb * kBlock + threadIdx.x; i < S.n_nodes;
        i += (int64_t)S.g_blocks * kBlock) {
     float acc[NC];
-    slot_sum<NC, float>(i, S.tiles.slot_ptr, S.tiles.slot_list, S.partial, acc);
+    slot_sum<NC, float>(i, S.tiles.slot_ptr, S.partial, acc);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int64_t dof = i * NC + c;

@@ -613,12 +613,11 @@ namespace {
 
 struct HostTiles {
   int32_t n_tiles = 0, max_local = 0, max_colors = 0;
-  std::vector<int32_t> elems;       // n_tiles * kTile
   std::vector<uint16_t> lnodes;     // n_tiles * kTile * nne
   std::vector<int16_t> color_off;   // n_tiles * (max_colors + 1)
   std::vector<int32_t> node_ptr;    // n_tiles + 1
   std::vector<int32_t> slot_ptr;    // n_nodes + 1
-  std::vector<int32_t> slot_list;   // slots, ascending tile per node
+  std::vector<int32_t> slot_pos;    // slot -> node-major position (ascending tile per node)
   std::vector<int32_t> tnodes;      // global node of every slot
   std::vector<uint16_t> inc_end;    // (incidence lists only) n_slots
   std::vector<uint16_t> inc_list;   // (incidence lists only) n_tiles * kTile * nne
@@ -668,7 +667,6 @@ int build_tiles(const double *coords, int dim, int64_t N, const int32_t *element
     return TATVA_EINVAL;
   }
   T.n_tiles = (int32_t)n_tiles;
-  T.elems.assign((size_t)n_tiles * kTile, -1);
   T.lnodes.assign((size_t)n_tiles * kTile * nne, 0);
   if (incidence) T.inc_list.assign((size_t)n_tiles * kTile * nne, 0);
   T.node_ptr.assign((size_t)n_tiles + 1, 0);
@@ -728,7 +726,6 @@ int build_tiles(const double *coords, int dim, int64_t N, const int32_t *element
     offs[(size_t)t] = off;
     for (int64_t k = 0; k < cnt; ++k) {
       const int64_t src = ord[(size_t)k];
-      T.elems[(size_t)(b + k)] = key[(size_t)(b + src)].second;
       for (int a = 0; a < nne; ++a) T.lnodes[(size_t)((b + k) * nne + a)] = ln[(size_t)src][a];
     }
     if (incidence) {
@@ -760,9 +757,9 @@ int build_tiles(const double *coords, int dim, int64_t N, const int32_t *element
   T.slot_ptr.assign((size_t)N + 1, 0);
   for (int32_t nd : tile_nodes) T.slot_ptr[(size_t)nd + 1]++;
   for (int64_t i = 0; i < N; ++i) T.slot_ptr[(size_t)i + 1] += T.slot_ptr[(size_t)i];
-  T.slot_list.assign(tile_nodes.size(), 0);
+  T.slot_pos.assign(tile_nodes.size(), 0);
   std::vector<int32_t> fill(T.slot_ptr.begin(), T.slot_ptr.end() - 1);
-  for (size_t sl = 0; sl < tile_nodes.size(); ++sl) T.slot_list[(size_t)fill[(size_t)tile_nodes[sl]]++] = (int32_t)sl;
+  for (size_t sl = 0; sl < tile_nodes.size(); ++sl) T.slot_pos[sl] = fill[(size_t)tile_nodes[sl]]++;
   return TATVA_OK;
 }
 
@@ -870,19 +867,18 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     (void)what;
     return hipMemcpy(*dst, vec.data(), vec.size() * sizeof(V), hipMemcpyHostToDevice);
   };
-  if ((err = upload(&m->t_elems, T.elems, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_lnodes, T.lnodes, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_color_off, T.color_off, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_node_ptr, T.node_ptr, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_slot_ptr, T.slot_ptr, "tiles")) != hipSuccess) return fail(err, "tiles");
-  if ((err = upload(&m->t_slot_list, T.slot_list, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if ((err = upload(&m->t_slot_pos, T.slot_pos, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_tnodes, T.tnodes, "tiles")) != hipSuccess) return fail(err, "tiles");
   if (elem_kind == TATVA_TET4) {
     if ((err = upload(&m->t_inc_end, T.inc_end, "tiles")) != hipSuccess) return fail(err, "tiles");
     if ((err = upload(&m->t_inc_list, T.inc_list, "tiles")) != hipSuccess) return fail(err, "tiles");
   }
-  m->tiles = TileSet{T.n_tiles, T.max_local, T.max_colors, 0, m->t_elems, m->t_lnodes,
-                     m->t_color_off, m->t_node_ptr, m->t_slot_ptr, m->t_slot_list,
+  m->tiles = TileSet{T.n_tiles, T.max_local, T.max_colors, 0, m->t_lnodes,
+                     m->t_color_off, m->t_node_ptr, m->t_slot_ptr, m->t_slot_pos,
                      m->t_tnodes, m->t_inc_end, m->t_inc_list};
   if ((err = hipMalloc(&m->coords, n_nodes * dim * sizeof(double))) != hipSuccess)
     return fail(err, "hipMalloc coords");
@@ -919,8 +915,8 @@ extern "C" int tatva_mesh_destroy(tatva_mesh *m) {
   if (m->inv_list) (void)hipFree(m->inv_list);
   if (m->scratch) (void)hipFree(m->scratch);
   if (m->partials) (void)hipFree(m->partials);
-  for (void *p : {(void *)m->t_elems, (void *)m->t_lnodes, (void *)m->t_color_off,
-                  (void *)m->t_node_ptr, (void *)m->t_slot_ptr, (void *)m->t_slot_list,
+  for (void *p : {(void *)m->t_lnodes, (void *)m->t_color_off,
+                  (void *)m->t_node_ptr, (void *)m->t_slot_ptr, (void *)m->t_slot_pos,
                   (void *)m->t_tnodes, (void *)m->t_inc_end, (void *)m->t_inc_list})
     if (p) (void)hipFree(p);
   delete m;
@@ -1075,10 +1071,10 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
             m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
       if (part_dot)
         k_slot_gather_dot<NC, T><<<gather_blocks, kBlock, 0, s>>>(
-            m->n_nodes, m->t_slot_ptr, m->t_slot_list, partial, dof_map, v, out, part_dot, skip);
+            m->n_nodes, m->t_slot_ptr, partial, dof_map, v, out, part_dot, skip);
       else
         k_slot_gather<NC, T><<<grid_for(m->n_nodes), kBlock, 0, s>>>(
-            m->n_nodes, m->t_slot_ptr, m->t_slot_list, partial, f_ext, dof_map, out, skip);
+            m->n_nodes, m->t_slot_ptr, partial, f_ext, dof_map, out, skip);
       TATVA_CHECK_LAUNCH();
       return TATVA_OK;
     });

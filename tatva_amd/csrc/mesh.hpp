@@ -31,12 +31,11 @@ struct tatva_mesh {
   // LDS tiling (assembly.hpp)
   tatva::TileSet tiles;
   int64_t n_slots;
-  int32_t *t_elems;
   uint16_t *t_lnodes;
   int16_t *t_color_off;
   int32_t *t_node_ptr;
   int32_t *t_slot_ptr;
-  int32_t *t_slot_list;
+  int32_t *t_slot_pos;
   int32_t *t_tnodes;
   uint16_t *t_inc_end;
   uint16_t *t_inc_list;

@@ -411,8 +411,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict
 }
 
 // Elements per thread whose loads the CG step issues before its scalar sums
-// (vec_grid caps the grid at 1024 blocks: 4 cover n <= 1,048,576 in one pass).
-constexpr int kStepPrefetch = 4;
+// (vec_grid caps the grid at 1024 blocks: 4 cover n <= 1,048,576 in one pass,
+// the 8 of the FP32 step n <= 2,097,152).
+template <class T>
+constexpr int step_prefetch() {
+  return sizeof(T) == 4 ? 8 : 4;
+}
 
 // KB of the single-reduction form (see the file comment).  u is D^-1 r.
 template <class T>
@@ -433,6 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
   // scalars are summed, so their latency overlaps the partial sums
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int kStepPrefetch = step_prefetch<T>();
   T ru[kStepPrefetch], rp[kStepPrefetch], rw[kStepPrefetch], rs[kStepPrefetch];
   T rx[kStepPrefetch], rv[kStepPrefetch], rd[kStepPrefetch];
 #pragma unroll
