@@ -88,10 +88,11 @@ def stretch_problem(n, kind="hex8", mu=1.0, lmbda=10.0, jitter=0.0, seed=0):
     return dict(mesh=mesh, op=op, E=E, L=L, values=values)
 
 
-def solve_load_steps(P, solver, n_steps, stretch):
+def solve_load_steps(P, solver, n_steps, stretch, upto=None):
     """Load-stepped Newton: each step adds a homogeneous stretch increment
     (u_x += dx * x) as predictor, sets u_x = stretch * s / n_steps on x = 1 and
-    solves.  Returns (u, [NewtonStats])."""
+    solves; with upto, only the first upto of the n_steps steps.  Returns
+    (u, [NewtonStats])."""
     import torch
 
     mesh = P["mesh"]
@@ -99,7 +100,7 @@ def solve_load_steps(P, solver, n_steps, stretch):
     ramp[0::3] = torch.as_tensor(mesh.coords[:, 0], device="cuda")
     u = torch.zeros(mesh.n_nodes * 3, dtype=torch.float64, device="cuda")
     out = []
-    for s in range(1, n_steps + 1):
+    for s in range(1, (upto or n_steps) + 1):
         u.add_(ramp, alpha=stretch / n_steps)
         L = P["L"].at["load"].set(P["values"](stretch * s / n_steps))
         out.append(solver.solve(u, L))
@@ -252,10 +253,11 @@ def chunked_oracle_residual(elem, coords, elements, dens, u, chunk=262144):
     return r
 
 
-def large_mixed(workload, oracle_check=True):
+def large_mixed(workload, oracle_check=True, upto=None):
     """C3 (Tet4 Neo-Hookean 96^3 x 6, jittered, 10 load steps to stretch 1.3) or
     C5 (Hex8 Neo-Hookean 160^3, 5 load steps to stretch 1.3): mixed Newton vs
-    the FP64 colored-CSR Newton on the same problem, both to 1e-8 relative."""
+    the FP64 colored-CSR Newton on the same problem, both to 1e-8 relative
+    (upto: only the first upto load steps, for short profiled runs)."""
     import numpy as np
 
     if workload == "c3":
@@ -272,12 +274,15 @@ def large_mixed(workload, oracle_check=True):
           flush=True)
     # no warm-up solve at this size: the first Newton step's set-up (graph
     # capture, workspaces) is a negligible share of the timed solve
-    res = mixed_vs_fp64(NH["E"], L0, lambda sv: solve_load_steps(NH, sv, steps, stretch),
+    done = min(upto or steps, steps)
+    res = mixed_vs_fp64(NH["E"], L0, lambda sv: solve_load_steps(NH, sv, steps, stretch, done),
                         keep_solutions=True, warmup=False)
     mesh = NH["mesh"]
     res["workload"] = (f"{kind} Neo-Hookean {n}^3{' x 6 tets' if kind == 'tet4' else ''}, "
                        f"mu=1, lambda=10, jitter={jitter}h (seed 0), x=0 clamped, u_x on x=1 "
-                       f"in {steps} load steps to {stretch}, Newton to 1e-8 relative per step")
+                       f"in {steps} load steps to {stretch}"
+                       f"{f' (first {done} run)' if done < steps else ''}, "
+                       f"Newton to 1e-8 relative per step")
     res["elements"] = mesh.n_elements
     res["nodes"] = mesh.n_nodes
     res["setup_s"] = setup
@@ -285,7 +290,7 @@ def large_mixed(workload, oracle_check=True):
         from oracle import fem  # the checker only
 
         el = fem.Tet4() if kind == "tet4" else fem.Hex8()
-        L_end = NH["L"].at["load"].set(NH["values"](stretch))
+        L_end = NH["L"].at["load"].set(NH["values"](stretch * done / steps))
         free = L_end.free_dofs
         for mode in ("mixed", "csr"):
             uh = res.pop(f"_u_{mode}").cpu().numpy()
@@ -311,10 +316,13 @@ def main(argv=None):
     ap.add_argument("--nh-n", type=int, default=48, help="cells per side, Neo-Hookean comparison")
     ap.add_argument("--nh-steps", type=int, default=2, help="load steps, Neo-Hookean comparison")
     ap.add_argument("--nh-stretch", type=float, default=0.1, help="final u_x on x = 1")
-    ap.add_argument("--workload", choices=("c2", "c2mixed", "c3", "c5", "nh48"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c2mixed", "c3", "c4", "c5", "nh48"), default="c2",
                     help="c2: the default line; c2mixed / nh48: only the mixed-vs-FP64 "
                          "comparison on the 64^3 elasticity / 48^3 Neo-Hookean cube; "
-                         "c3/c5: large mixed-vs-FP64 Neo-Hookean runs")
+                         "c3/c5: large mixed-vs-FP64 Neo-Hookean runs; c4: the serving "
+                         "stream only (--serve-requests, --serve-qps)")
+    ap.add_argument("--large-steps", type=int, default=None,
+                    help="c3/c5: run only the first N load steps (short profiled runs)")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[10.0, 40.0])
     args = ap.parse_args(argv)
@@ -330,8 +338,13 @@ def main(argv=None):
 
     torch.cuda.set_device(0)
     if args.workload in ("c3", "c5"):
-        r = large_mixed(args.workload, oracle_check=not args.no_cpu_baseline)
+        r = large_mixed(args.workload, oracle_check=not args.no_cpu_baseline,
+                        upto=args.large_steps)
         print(json.dumps({"workload": args.workload, **r}), flush=True)
+        return 0
+    if args.workload == "c4":
+        r = [serve_stream(args.serve_requests, q) for q in args.serve_qps]
+        print(json.dumps({"workload": args.workload, "serve": r}), flush=True)
         return 0
     if args.workload == "c2mixed":
         P = build_problem(args.n)
@@ -393,8 +406,8 @@ def main(argv=None):
         kernel = "k_pcg_bspmv<3> (node-block CSR SpMV fused with p.Ap)"
         spmv_bytes = 8 * nnz + 4 * (nnz // 9) + 8 * (n_free // 3) + 16 * n_free
     else:
-        kernel = "k_pcg_spmv<LPR> (CSR SpMV fused with p.Ap)"
-        spmv_bytes = 12 * nnz + 4 * (n_free + 1) + 16 * n_free
+        kernel = "k_sell_spmv (sliced ELL copy of the CSR, fused with p.Ap)"
+        spmv_bytes = 12 * nnz + 8 * (n_free // 64) + 16 * n_free
     achieved = spmv_bytes / spmv_s / 1e9 if op_samples else None
     spmv_traffic = None
     if blocked and args.n == 64:
