@@ -9,8 +9,10 @@ same few kernel launches, whatever the request sizes.  A request that
 finishes its linear solve takes its FP64 Newton step (residual, update,
 convergence test, next load step) between batch launches and re-enters the
 batch with its next linear system; a request arriving meanwhile joins at the
-next launch.  The arithmetic of a request does not depend on what else is in
-the batch, so its result is the same served alone or in any mix.
+next launch.  Arriving requests are prepared (mesh, tiling, constraints,
+device buffers) on a worker thread while the GPU iterates the admitted ones.
+The arithmetic of a request does not depend on what else is in the batch, so
+its result is the same served alone or in any mix.
 
 There is no counterpart in the reference; the request workload follows the
 mixed Hex8 linear-elastic / Neo-Hookean cube stream: n = round(exp(U(ln 8,
@@ -23,6 +25,8 @@ from __future__ import annotations
 import ctypes
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import wait as futures_wait
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -188,12 +192,14 @@ class CubeProblem:
 
 class _Job:
     """A request resident on the GPU: mesh, constraints, FP64 state, the FP32
-    linear system of its current Newton step and its batch slot."""
+    linear system of its current Newton step and its batch slot.  Built
+    (mesh, tiling, constraints, device buffers) before it has a slot, on the
+    server's preparation thread; ``attach`` gives it the slot."""
 
-    def __init__(self, req: SolveRequest, slot: int, server: "Server", admitted: float):
+    def __init__(self, req: SolveRequest, server: "Server"):
         t = nat.torch()
         self.req = req
-        self.slot = slot
+        self.slot = -1
         self.server = server
         self.prob = CubeProblem(req)
         self.op, self.energy = self.prob.op, self.prob.energy
@@ -212,11 +218,16 @@ class _Job:
             nat.check(nat.EINVAL)
         self.ws = t.empty(int(nbytes), dtype=t.uint8, device=dev)
         self.result = RequestResult(rid=req.rid, n=req.n, material=req.material, n_free=self.n,
-                                    arrival=req.arrival, admitted=admitted, finished=float("nan"))
+                                    arrival=req.arrival, admitted=float("nan"),
+                                    finished=float("nan"))
         self.step = 0
         self.r0 = None
         self.newton_in_step = 0
         self.done = False
+
+    def attach(self, slot: int, admitted: float) -> None:
+        self.slot = slot
+        self.result.admitted = admitted
 
     # ------------------------------------------------------------ phases --
 
@@ -340,13 +351,18 @@ class Server:
     from the start of the run) and returns a ServeReport.  ``max_slots``
     bounds the requests in flight, ``hbm_budget`` their device memory
     (SolveRequest.hbm_bytes); ``chunk`` PCG iterations run between polls.
+    A worker thread prepares up to ``prefetch`` arrived requests (mesh,
+    tiling, constraints, device buffers) while the GPU iterates the others;
+    a request's bytes count against the budget from the start of its
+    preparation.
     """
 
     def __init__(self, *, max_slots: int = 64, hbm_budget: float = 200e9, chunk: int = 8,
                  tol: float = 1e-8, inner_tol: float = 1e-3, max_newton: int = 30,
-                 maxiter_factor: int = 10, keep_solutions: bool = False):
-        if max_slots < 1 or chunk < 1:
-            raise ValueError("max_slots and chunk must be >= 1")
+                 maxiter_factor: int = 10, keep_solutions: bool = False, prefetch: int = 2):
+        if max_slots < 1 or chunk < 1 or prefetch < 1:
+            raise ValueError("max_slots, chunk and prefetch must be >= 1")
+        self.prefetch = int(prefetch)
         self.max_slots = int(max_slots)
         self.hbm_budget = float(hbm_budget)
         self.chunk = int(chunk)
@@ -379,6 +395,7 @@ class Server:
             if r.hbm_bytes() > self.hbm_budget:
                 raise ValueError(f"request {r.rid} needs more than the HBM budget")
         waiting: list[SolveRequest] = []
+        preparing: list = []  # (request, future of its _Job), FIFO
         jobs: dict[int, _Job] = {}
         free_slots = list(range(self.max_slots))[::-1]
         used_bytes = 0
@@ -387,45 +404,55 @@ class Server:
         peak_adm = 0
         peak_bytes = 0
         dof_iters = 0
+        pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="tatva-prepare")
         t.cuda.synchronize()
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else None
-        vclock = 0.0  # non-realtime: every request is present from the start
-        while pending or waiting or jobs:
-            now = clock() if realtime else vclock
-            while pending and (pending[0].arrival <= now or not realtime):
-                waiting.append(pending.pop(0))
-            # admission: FIFO, slots and HBM budget
-            while waiting and free_slots and \
-                    used_bytes + waiting[0].hbm_bytes() <= self.hbm_budget:
-                req = waiting.pop(0)
-                slot = free_slots.pop()
-                job = _Job(req, slot, self, time.perf_counter() - t0)
-                used_bytes += req.hbm_bytes()
-                jobs[slot] = job
-                if not job.begin():
-                    self._retire(job, jobs, free_slots, results, t0)
-                    used_bytes -= req.hbm_bytes()
-            peak_adm = max(peak_adm, len(jobs))
-            peak_bytes = max(peak_bytes, used_bytes)
-            if not jobs:
-                if pending and realtime:
-                    time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
-                continue
-            nat.check(L.tatva_batch_iterate(self._batch, self.chunk, nat.stream()))
-            launches += 1
-            running = ctypes.c_int32(0)
-            nat.check(L.tatva_batch_poll(self._batch, self._stats, ctypes.byref(running),
-                                         nat.stream()))
-            for slot in list(jobs):
-                st = self._stats[slot]
-                if not st.reserved:  # still iterating
+        try:
+            while pending or waiting or preparing or jobs:
+                now = clock() if realtime else 0.0
+                while pending and (not realtime or pending[0].arrival <= now):
+                    waiting.append(pending.pop(0))
+                # preparation: FIFO, at most `prefetch` ahead, within the HBM budget
+                while waiting and len(preparing) < self.prefetch and \
+                        used_bytes + waiting[0].hbm_bytes() <= self.hbm_budget:
+                    req = waiting.pop(0)
+                    used_bytes += req.hbm_bytes()
+                    preparing.append((req, pool.submit(_Job, req, self)))
+                # admission: FIFO, as prepared requests and slots become available
+                while preparing and free_slots and preparing[0][1].done():
+                    req, fut = preparing.pop(0)
+                    job = fut.result()
+                    slot = free_slots.pop()
+                    job.attach(slot, time.perf_counter() - t0)
+                    jobs[slot] = job
+                    if not job.begin():
+                        used_bytes -= req.hbm_bytes()
+                        self._retire(job, jobs, free_slots, results, t0)
+                peak_adm = max(peak_adm, len(jobs))
+                peak_bytes = max(peak_bytes, used_bytes)
+                if not jobs:
+                    if preparing:
+                        futures_wait([preparing[0][1]], timeout=0.01)
+                    elif pending and realtime:
+                        time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
                     continue
-                job = jobs[slot]
-                dof_iters += job.n * int(st.iterations)
-                if not job.on_linear_done(st):
-                    used_bytes -= job.req.hbm_bytes()
-                    self._retire(job, jobs, free_slots, results, t0)
+                nat.check(L.tatva_batch_iterate(self._batch, self.chunk, nat.stream()))
+                launches += 1
+                running = ctypes.c_int32(0)
+                nat.check(L.tatva_batch_poll(self._batch, self._stats, ctypes.byref(running),
+                                             nat.stream()))
+                for slot in list(jobs):
+                    st = self._stats[slot]
+                    if not st.reserved:  # still iterating
+                        continue
+                    job = jobs[slot]
+                    dof_iters += job.n * int(st.iterations)
+                    if not job.on_linear_done(st):
+                        used_bytes -= job.req.hbm_bytes()
+                        self._retire(job, jobs, free_slots, results, t0)
+        finally:
+            pool.shutdown(wait=True)
         t.cuda.synchronize()
         seconds = time.perf_counter() - t0
         results.sort(key=lambda r: r.rid)

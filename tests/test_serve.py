@@ -131,3 +131,86 @@ def test_realtime_stream_reports_latency():
     assert s["requests"] == 8 and s["converged"] == 8
     assert 0 < s["p50_ms"] <= s["p99_ms"]
     assert all(r.finished >= r.admitted >= r.arrival - 1e-3 for r in rep.results)
+
+
+@pytest.mark.gpu
+def test_batch_slots_of_every_element_kind_solve_their_systems():
+    """One tatva_batch holding a Tet4 Neo-Hookean, a Tri3 Poisson and a Hex8
+    linear-elastic slot (three launch classes; Tet4 sums its tiles through the
+    incidence lists).  Each slot's FP32 PCG solution of H(u) x = b is checked
+    against the oracle's dense FP64 Hessian: true relative residual <= 1e-4 for
+    an inner tolerance of 1e-6 (FP32 operator rounding times the conditioning,
+    ~3e3 for the Neo-Hookean slot; a wrong operator leaves O(1) residuals).
+    The state is small enough (1e-3) that every Hessian is positive definite."""
+    import ctypes
+
+    import torch
+
+    import tatva_amd as ta
+    from oracle import fem
+    from tatva_amd import _native as nat
+    from tatva_amd import energies, lifter
+
+    rng = np.random.default_rng(21)
+    cases = []
+    m = ta.Mesh.box((5, 4, 6), type="tetrahedron", jitter=0.2, seed=3)
+    cases.append((m, ta.Tetrahedron4(), energies.NeoHookean(1.0, 10.0), fem.Tet4(),
+                  fem.NeoHookean(1.0, 10.0), 3, np.where(np.isclose(m.coords[:, 0], 0.0))[0]))
+    m = ta.Mesh.unit_square(8, 9)
+    bnd = np.where(np.isclose(m.coords, 0).any(1) | np.isclose(m.coords, 1).any(1))[0]
+    cases.append((m, ta.Tri3(), energies.Poisson(1.0), fem.Tri3(), fem.Poisson(1.0), 1, bnd))
+    m = ta.Mesh.unit_cube(4)
+    cases.append((m, ta.Hexahedron8(), energies.LinearElasticity(0.4, 0.6), fem.Hex8(),
+                  fem.LinearElasticity(0.4, 0.6), 3, np.where(np.isclose(m.coords[:, 0], 0.0))[0]))
+    L = nat.lib()
+    s = nat.stream()
+    B = ctypes.c_void_p()
+    nat.check(L.tatva_batch_create(0, 4, ctypes.byref(B)))
+    keep, refs = [], []
+    for k, (m, el, dens, oel, odens, nc, fixed_nodes) in enumerate(cases):
+        op = ta.Operator(m, el)
+        E = op.energy(dens)
+        size = m.n_nodes * nc
+        fixed = (fixed_nodes[:, None] * nc + np.arange(nc)).ravel()
+        lf = lifter.Lifter(size, lifter.Fixed(fixed, 0.0))
+        red = E.reduced(lf)
+        n = red.size
+        u = 1e-3 * rng.standard_normal(size)
+        u[fixed] = 0.0
+        b = rng.standard_normal(n)
+        prob = fem.Problem(oel, m.coords, m.elements, odens)
+        free = np.setdiff1d(np.arange(size), fixed)
+        H = np.empty((n, n))
+        for j, dof in enumerate(free):
+            e = np.zeros(size)
+            e[dof] = 1.0
+            H[:, j] = prob.hvp(u, e)[free]
+        u32 = torch.as_tensor(u, dtype=torch.float32, device="cuda")
+        dmap = lf.dof_map_device()
+        diag = E.hessian_diag(u32, dof_map=dmap, n_out=n, dtype=torch.float32)
+        b32 = torch.as_tensor(b, dtype=torch.float32, device="cuda")
+        x32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+        ws = torch.empty(int(L.tatva_batch_slot_workspace(op.handle, nc, n)), dtype=torch.uint8,
+                         device="cuda")
+        nat.check(L.tatva_batch_start(B, k, op.handle, ctypes.byref(E._desc), nat.dptr(u32),
+                                      nat.dptr(dmap), n, nat.dptr(diag), nat.dptr(b32),
+                                      nat.dptr(x32), 1e-6, 10 * n, nat.dptr(ws), ws.numel(), s))
+        keep.append((op, E, lf, u32, dmap, diag, b32, x32, ws))
+        refs.append((H, b))
+    stats = (nat.PcgStats * 4)()
+    running = ctypes.c_int32(0)
+    for _ in range(400):
+        nat.check(L.tatva_batch_iterate(B, 8, s))
+        nat.check(L.tatva_batch_poll(B, stats, ctypes.byref(running), s))
+        if running.value == 0:
+            break
+    assert running.value == 0
+    for k, (H, b) in enumerate(refs):
+        assert stats[k].reserved != 0 and stats[k].status == 0, k
+        x = keep[k][7].double().cpu().numpy()
+        assert np.linalg.norm(H @ x - b) / np.linalg.norm(b) <= 1e-4, k
+    for k in range(len(cases)):
+        nat.check(L.tatva_batch_release(B, k))
+    nat.check(L.tatva_batch_destroy(B))
+    for item in keep:
+        item[0].close()
