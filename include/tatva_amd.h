@@ -223,7 +223,9 @@ int tatva_spmv_bcsr(int64_t n, const int32_t *indptr, const int32_t *block_cols,
  * replaces the user-side linear solve, cf. spsolve in tatva/utils.py:186-198).
  * x holds x0 on entry.  Iterations run in hipGraph-captured chunks of `chunk`.
  * With block_cols (from tatva_csr_block_view) the operator reads the CSR data
- * through the node-block view. */
+ * through the node-block view; without, the call first copies the matrix into
+ * a sliced-ELL layout (64-row slices) held in a device buffer the library owns
+ * and grows as needed (one host synchronisation per call). */
 int64_t tatva_pcg_csr_workspace(int64_t n);
 int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
                   const double *data, const int32_t *block_cols, int32_t block_size,
@@ -244,7 +246,7 @@ int tatva_pcg_matfree_f32(tatva_mesh *mesh, const tatva_density *density, const 
 /* A batch runs many matrix-free FP32 Jacobi-PCG solves (each with its own
  * mesh, density, constraints, state, tolerance) in shared launches: one tile
  * assembly and one gather launch per (element kind, density kind, quadrature)
- * class and two vector launches per iteration for all slots.  No counterpart
+ * class and one vector launch per iteration for all slots.  No counterpart
  * in the reference (it is the serving loop's solver; the linear algebra is the
  * same as tatva_pcg_matfree_f32).  A slot's arithmetic does not depend on the
  * other slots, so its result is identical alone and in any mix. */
