@@ -55,7 +55,8 @@ struct BatchSlot {
   float *x, *r, *z, *p, *s, *w, *dinv;
   double *pap, *rz0, *rz1, *rr0, *rr1, *bb;
   PcgState *st;
-  int32_t g_blocks, v_blocks, start_tick;
+  int64_t start_tick;  // batch tick of the slot's first iteration
+  int32_t g_blocks, v_blocks;
   int32_t a_blk0, g_blk0, v_blk0;
 };
 
@@ -107,19 +108,22 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather(const BatchSlot *__rest
   if (threadIdx.x == 0) S.pap[lb] = dot;
 }
 
+// Iteration k of a batch launch sequence runs at batch tick *tick + k (the
+// tick advances by the sequence length at its end, k_batch_tick), so a
+// captured sequence replays with no per-launch arguments.
 __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchSlot *__restrict__ slots,
                                                        const int32_t *__restrict__ bmap,
-                                                       int tick) {
+                                                       const int64_t *__restrict__ tick, int k) {
   const int blk = (int)xcd_block(blockIdx.x, gridDim.x);
   const BatchSlot &S = slots[bmap[blk]];
   PcgState *st = S.st;
   if (st->done) return;
   const int lb = blk - S.v_blk0;
-  const int q = (tick - S.start_tick) & 1;
+  const int q = (int)((*tick + k - S.start_tick) & 1);
   const int iter = st->it[q ^ 1];
-  const double gamma = sum_partials(q ? S.rz0 : S.rz1, S.v_blocks);
-  const double rr = sum_partials(q ? S.rr0 : S.rr1, S.v_blocks);
-  const double delta = sum_partials(S.pap, S.g_blocks);
+  double gamma, rr, delta;
+  sum_partials3(q ? S.rz0 : S.rz1, S.v_blocks, q ? S.rr0 : S.rr1, S.v_blocks, S.pap, S.g_blocks,
+                gamma, rr, delta);
   const bool first = iter == 0;
   const double beta = first ? 0.0 : gamma / st->gamma[q ^ 1];
   const double denom = first ? delta : delta - beta * gamma / st->alpha[q ^ 1];
@@ -165,12 +169,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchSlot *__restri
     sg += (double)ri * (double)zi;
     srr += (double)ri * (double)ri;
   }
-  sg = block_sum(sg);
-  srr = block_sum(srr);
+  block_sum2(sg, srr);
   if (threadIdx.x == 0) {
     (q ? S.rz1 : S.rz0)[lb] = sg;
     (q ? S.rr1 : S.rr0)[lb] = srr;
   }
+}
+
+__global__ void k_batch_tick(int64_t *__restrict__ tick, int iters) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *tick += iters;
 }
 
 // Per-slot workspace: z, p, s, w, r, dinv (n floats each), tile partials
@@ -245,6 +252,20 @@ struct tatva_batch {
   size_t stage_bmap_cap = 0;
   hipEvent_t uploaded = nullptr;
   bool upload_pending = false;
+  int64_t *d_tick = nullptr;  // device copy of tick (advanced by the launch sequence)
+  // the launch sequence of `iters` iterations, captured once per launch plan
+  hipStream_t cap = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int64_t> plan;  // what the captured sequence was built for
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    plan.clear();
+  }
 };
 
 namespace {
@@ -392,6 +413,15 @@ extern "C" int tatva_batch_create(int device, int32_t capacity, tatva_batch **ou
     return fail(e, "hipHostMalloc slots");
   if ((e = hipEventCreateWithFlags(&B->uploaded, hipEventDisableTiming)) != hipSuccess)
     return fail(e, "hipEventCreate");
+  if ((e = hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming)) != hipSuccess)
+    return fail(e, "hipEventCreate");
+  if ((e = hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming)) != hipSuccess)
+    return fail(e, "hipEventCreate");
+  if ((e = hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking)) != hipSuccess)
+    return fail(e, "hipStreamCreate");
+  if ((e = hipMalloc(&B->d_tick, sizeof(int64_t))) != hipSuccess) return fail(e, "hipMalloc tick");
+  if ((e = hipMemcpy(B->d_tick, &B->tick, sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy tick");
   std::memset(B->h_states, 0, capacity * sizeof(PcgState));
   *out = B;
   return TATVA_OK;
@@ -408,6 +438,11 @@ extern "C" int tatva_batch_destroy(tatva_batch *B) {
   if (B->h_stage_slots) (void)hipHostFree(B->h_stage_slots);
   if (B->h_stage_bmap) (void)hipHostFree(B->h_stage_bmap);
   if (B->uploaded) (void)hipEventDestroy(B->uploaded);
+  B->drop_graph();
+  if (B->ev_in) (void)hipEventDestroy(B->ev_in);
+  if (B->ev_out) (void)hipEventDestroy(B->ev_out);
+  if (B->cap) (void)hipStreamDestroy(B->cap);
+  if (B->d_tick) (void)hipFree(B->d_tick);
   delete B;
   return TATVA_OK;
 }
@@ -466,8 +501,7 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   S.st = B->d_states + slot;
   S.g_blocks = gather_grid(m->n_nodes);
   S.v_blocks = vec_grid(n);
-  if (B->tick >= (int64_t)1 << 30) B->tick &= 1;  // keeps tick and start_tick in int range
-  S.start_tick = (int32_t)B->tick;
+  S.start_tick = B->tick;
   // r = b - H x, z = D^-1 r, shares of r.z, r.r, b.b (w is scratch here)
   const int rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, S.r, S.z, S.w, S.dinv, S.rz1,
                                    S.rr1, S.bb, S.v_blocks, tol, maxiter, S.st, s);
@@ -498,19 +532,13 @@ extern "C" int tatva_batch_release(tatva_batch *B, int32_t slot) {
   return TATVA_OK;
 }
 
-extern "C" int tatva_batch_iterate(tatva_batch *B, int32_t iters, void *stream) {
-  TATVA_REQUIRE(B, "batch is NULL");
-  TATVA_REQUIRE(iters >= 0, "iters must be >= 0");
-  std::lock_guard<std::mutex> lock(B->mu);
-  TATVA_TRY_HIP(hipSetDevice(B->device));
-  hipStream_t s = (hipStream_t)stream;
-  int rc = batch_upload(B, s);
-  if (rc) return rc;
-  if (B->v_n == 0) return TATVA_OK;
+namespace {
+
+// Enqueues `iters` batch iterations (the launch plan of batch_upload) on s.
+int enqueue_iterations(tatva_batch *B, int iters, hipStream_t s) {
   for (int it = 0; it < iters; ++it) {
-    const int tick = (int)B->tick;
     for (const auto &L : B->launches) {
-      rc = with_class(L.kind, L.dkind, [&](auto kc, auto dens_tag) -> int {
+      const int rc = with_class(L.kind, L.dkind, [&](auto kc, auto dens_tag) -> int {
         constexpr int K = decltype(kc)::value;
         using Dens = decltype(dens_tag);
         if (K == TATVA_HEX8 && L.affine)
@@ -524,10 +552,62 @@ extern "C" int tatva_batch_iterate(tatva_batch *B, int32_t iters, void *stream) 
       });
       if (rc) return rc;
     }
-    k_batch_step<<<B->v_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + B->v_off, tick);
-    TATVA_CHECK_LAUNCH();
-    B->tick += 1;
+    k_batch_step<<<B->v_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + B->v_off, B->d_tick, it);
   }
+  k_batch_tick<<<1, 1, 0, s>>>(B->d_tick, iters);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
+// Everything the captured launch sequence depends on besides device memory.
+std::vector<int64_t> launch_plan(const tatva_batch *B, int iters) {
+  std::vector<int64_t> p{iters, B->v_off, B->v_n, (int64_t)(intptr_t)B->d_bmap,
+                         (int64_t)(intptr_t)B->d_slots};
+  for (const auto &L : B->launches) {
+    p.insert(p.end(), {L.kind, L.dkind, L.ncomp, L.affine, (int64_t)L.lds, L.a_off, L.a_n,
+                       L.g_off, L.g_n});
+    const int64_t *q = reinterpret_cast<const int64_t *>(&L.quad);
+    p.insert(p.end(), q, q + sizeof(QuadRule) / sizeof(int64_t));
+  }
+  return p;
+}
+
+}  // namespace
+
+extern "C" int tatva_batch_iterate(tatva_batch *B, int32_t iters, void *stream) {
+  TATVA_REQUIRE(B, "batch is NULL");
+  TATVA_REQUIRE(iters >= 0, "iters must be >= 0");
+  std::lock_guard<std::mutex> lock(B->mu);
+  TATVA_TRY_HIP(hipSetDevice(B->device));
+  hipStream_t s = (hipStream_t)stream;
+  int rc = batch_upload(B, s);
+  if (rc) return rc;
+  if (B->v_n == 0 || iters == 0) return TATVA_OK;
+  // the sequence runs from a captured graph on the batch's own stream (capture
+  // is not allowed on the legacy default stream), joined to s with events; it
+  // is re-captured when the launch plan changes (slots joining or leaving)
+  std::vector<int64_t> plan = launch_plan(B, iters);
+  if (!B->exec || plan != B->plan) {
+    B->drop_graph();
+    TATVA_TRY_HIP(hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_iterations(B, iters, B->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(B->cap, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    TATVA_TRY_HIP(e);
+    B->graph = g;
+    TATVA_TRY_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    B->plan = std::move(plan);
+  }
+  TATVA_TRY_HIP(hipEventRecord(B->ev_in, s));
+  TATVA_TRY_HIP(hipStreamWaitEvent(B->cap, B->ev_in, 0));
+  TATVA_TRY_HIP(hipGraphLaunch(B->exec, B->cap));
+  TATVA_TRY_HIP(hipEventRecord(B->ev_out, B->cap));
+  TATVA_TRY_HIP(hipStreamWaitEvent(s, B->ev_out, 0));
+  B->tick += iters;
   return TATVA_OK;
 }
 
