@@ -250,6 +250,8 @@ def chunked_oracle_residual(elem, coords, elements, dens, u, chunk=262144):
     r = np.zeros(u.size)
     for a in range(0, elements.shape[0], chunk):
         r += fem.Problem(elem, coords, elements[a:a + chunk], dens).residual(u)
+        print(f"[oracle] residual: {min(a + chunk, elements.shape[0])}/{elements.shape[0]} "
+              "elements", file=sys.stderr, flush=True)
     return r
 
 
