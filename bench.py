@@ -294,11 +294,18 @@ def large_mixed(workload, oracle_check=True, upto=None):
         el = fem.Tet4() if kind == "tet4" else fem.Hex8()
         L_end = NH["L"].at["load"].set(NH["values"](stretch * done / steps))
         free = L_end.free_dofs
+        # scale: the oracle residual of the final load applied to the undeformed body
+        # (free dofs zero, prescribed dofs at their values)
+        u_load = L_end.lift_from_zeros(np.zeros(free.size)).cpu().numpy()
+        r_load = np.linalg.norm(chunked_oracle_residual(el, mesh.coords, mesh.elements,
+                                                        fem.NeoHookean(1.0, 10.0), u_load)[free])
+        res["oracle_load_residual_norm"] = float(r_load)
         for mode in ("mixed", "csr"):
             uh = res.pop(f"_u_{mode}").cpu().numpy()
             r = chunked_oracle_residual(el, mesh.coords, mesh.elements,
                                         fem.NeoHookean(1.0, 10.0), uh)[free]
             res[mode]["oracle_final_residual_norm"] = float(np.linalg.norm(r))
+            res[mode]["oracle_rel_residual"] = float(np.linalg.norm(r) / r_load)
     else:
         res.pop("_u_mixed", None)
         res.pop("_u_csr", None)

@@ -32,6 +32,24 @@ def _as_np(a, dtype):
     return np.ascontiguousarray(np.asarray(a), dtype=dtype)
 
 
+def _min_corner_volume(coords, elements, kind) -> float:
+    """Smallest signed volume (times 6) of the corner tetrahedra of the elements:
+    each Tet4 itself; for Hex8 the tet spanned by the three edges at each of the
+    eight corners (its sign is that of det J at the corner)."""
+    X = coords[elements]
+    if kind is ElementType.TETRAHEDRON:
+        corners = [(0, 1, 2, 3)]
+    else:
+        # corner, then its neighbours along +xi, +eta, +zeta (Hexahedron8 order)
+        corners = [(0, 1, 3, 4), (1, 2, 0, 5), (2, 3, 1, 6), (3, 0, 2, 7),
+                   (4, 7, 5, 0), (5, 4, 6, 1), (6, 5, 7, 2), (7, 6, 4, 3)]
+    vmin = np.inf
+    for c, a, b, d in corners:
+        e1, e2, e3 = X[:, a] - X[:, c], X[:, b] - X[:, c], X[:, d] - X[:, c]
+        vmin = min(vmin, float(np.einsum("ij,ij->i", e1, np.cross(e2, e3)).min()))
+    return vmin
+
+
 @dataclass(frozen=True, eq=False)
 class Mesh:
     """``Mesh(coords, elements)``: coords (n_nodes, n_dim), elements (n_elements, nne)."""
@@ -142,6 +160,8 @@ class Mesh:
             hi = np.isclose(coords, o + L)
             interior = ~(lo | hi).any(axis=1)
             coords = coords + delta * interior[:, None]
+            if _min_corner_volume(coords, elements, kind) <= 0.0:
+                raise ValueError(f"jitter={jitter} inverts an element; use a smaller jitter")
         return cls(coords=coords, elements=elements.astype(np.int32))
 
     @classmethod

@@ -255,3 +255,18 @@ def test_field_patterns_match_oracle():
     np.testing.assert_array_equal(D.indptr, [0, 0, 1, 2, 2])
     with pytest.raises(ValueError):
         sparse.pattern_from_element_dofs(np.array([[0, 7]]), 5)
+
+
+def test_jittered_box_keeps_every_element_positive():
+    """C3's jittered Tet4 box must not invert an element (every signed volume,
+    and every Hex8 corner Jacobian, positive); a jitter that inverts raises."""
+    import tatva_amd as ta
+    from tatva_amd.mesh import ElementType, _min_corner_volume
+
+    for kind, et in (("tetrahedron", ElementType.TETRAHEDRON), ("hexahedron", ElementType.HEXAHEDRON)):
+        m = ta.Mesh.box(6, type=kind, jitter=0.2, seed=0)
+        assert _min_corner_volume(m.coords, m.elements, et) > 0.0
+        plain = ta.Mesh.box(6, type=kind)
+        assert np.isclose(_min_corner_volume(plain.coords, plain.elements, et), (1 / 6) ** 3)
+    with pytest.raises(ValueError):
+        ta.Mesh.box(8, type="tetrahedron", jitter=0.9, seed=0)
