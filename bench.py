@@ -334,6 +334,7 @@ def main(argv=None):
                     help="c3/c5: run only the first N load steps (short profiled runs)")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[10.0, 40.0])
+    ap.add_argument("--serve-chunk", type=int, default=8, help="batched PCG iterations per poll")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path's "
@@ -352,7 +353,7 @@ def main(argv=None):
         print(json.dumps({"workload": args.workload, **r}), flush=True)
         return 0
     if args.workload == "c4":
-        r = [serve_stream(args.serve_requests, q) for q in args.serve_qps]
+        r = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk) for q in args.serve_qps]
         print(json.dumps({"workload": args.workload, "serve": r}), flush=True)
         return 0
     if args.workload == "c2mixed":
@@ -443,7 +444,8 @@ def main(argv=None):
         del NH
         extras["c1_poisson_tri3_64"] = c1_latency()
         if args.serve_requests > 0:
-            extras["serve_c4"] = [serve_stream(args.serve_requests, q) for q in args.serve_qps]
+            extras["serve_c4"] = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk)
+                                  for q in args.serve_qps]
 
     # checks on the final state: true linear residual and the FP64 oracle's residual
     res_free = P["E"].reduced(L).residual_full_state(u)

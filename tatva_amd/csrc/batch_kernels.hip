@@ -109,8 +109,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather(const BatchSlot *__rest
 }
 
 // Iteration k of a batch launch sequence runs at batch tick *tick + k (the
-// tick advances by the sequence length at its end, k_batch_tick), so a
-// captured sequence replays with no per-launch arguments.
+// tick advances by the sequence length at its end, k_batch_tick), so the
+// graph of a sequence replays with no per-launch arguments.
 __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchSlot *__restrict__ slots,
                                                        const int32_t *__restrict__ bmap,
                                                        const int64_t *__restrict__ tick, int k) {
@@ -253,7 +253,7 @@ struct tatva_batch {
   hipEvent_t uploaded = nullptr;
   bool upload_pending = false;
   int64_t *d_tick = nullptr;  // device copy of tick (advanced by the launch sequence)
-  // the launch sequence of `iters` iterations, captured once per launch plan
+  // the launch sequence of `iters` iterations, built once per launch plan
   hipStream_t cap = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
@@ -534,32 +534,72 @@ extern "C" int tatva_batch_release(tatva_batch *B, int32_t slot) {
 
 namespace {
 
-// Enqueues `iters` batch iterations (the launch plan of batch_upload) on s.
-int enqueue_iterations(tatva_batch *B, int iters, hipStream_t s) {
-  for (int it = 0; it < iters; ++it) {
+// Builds the graph of `iters` batch iterations (the launch plan of
+// batch_upload) node by node: a chain of kernel nodes, no stream capture, so
+// HIP calls made meanwhile by other host threads cannot disturb it.
+int build_iterations(tatva_batch *B, int iters, hipGraph_t *out) {
+  hipGraph_t g = nullptr;
+  TATVA_TRY_HIP(hipGraphCreate(&g, 0));
+  hipGraphNode_t prev = nullptr;
+  const BatchSlot *slots = B->d_slots;
+  int64_t *tick = B->d_tick;
+  auto add = [&](const void *fn, int grid, int block, size_t lds, void **args) -> hipError_t {
+    hipKernelNodeParams p{};
+    p.func = const_cast<void *>(fn);
+    p.gridDim = dim3((unsigned)grid);
+    p.blockDim = dim3((unsigned)block);
+    p.sharedMemBytes = (unsigned)lds;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    const hipError_t e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+    if (e == hipSuccess) prev = node;
+    return e;
+  };
+  hipError_t err = hipSuccess;
+  int rc = TATVA_OK;
+  for (int it = 0; it < iters && err == hipSuccess && rc == TATVA_OK; ++it) {
     for (const auto &L : B->launches) {
-      const int rc = with_class(L.kind, L.dkind, [&](auto kc, auto dens_tag) -> int {
+      rc = with_class(L.kind, L.dkind, [&](auto kc, auto dens_tag) -> int {
         constexpr int K = decltype(kc)::value;
         using Dens = decltype(dens_tag);
-        if (K == TATVA_HEX8 && L.affine)
-          k_batch_assemble<K, Dens, true><<<L.a_n, kTile, L.lds, s>>>(B->d_slots,
-                                                                      B->d_bmap + L.a_off, L.quad);
-        else
-          k_batch_assemble<K, Dens, false><<<L.a_n, kTile, L.lds, s>>>(B->d_slots,
-                                                                       B->d_bmap + L.a_off, L.quad);
-        k_batch_gather<Dens::NC><<<L.g_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + L.g_off);
+        const int32_t *bm_a = B->d_bmap + L.a_off;
+        const int32_t *bm_g = B->d_bmap + L.g_off;
+        QuadRule quad = L.quad;
+        void *a_args[] = {(void *)&slots, (void *)&bm_a, (void *)&quad};
+        const void *fa = (K == TATVA_HEX8 && L.affine)
+                             ? reinterpret_cast<const void *>(&k_batch_assemble<K, Dens, true>)
+                             : reinterpret_cast<const void *>(&k_batch_assemble<K, Dens, false>);
+        err = add(fa, L.a_n, kTile, L.lds, a_args);
+        if (err != hipSuccess) return TATVA_OK;
+        void *g_args[] = {(void *)&slots, (void *)&bm_g};
+        err = add(reinterpret_cast<const void *>(&k_batch_gather<Dens::NC>), L.g_n, kBlock, 0, g_args);
         return TATVA_OK;
       });
-      if (rc) return rc;
+      if (rc || err != hipSuccess) break;
     }
-    k_batch_step<<<B->v_n, kBlock, 0, s>>>(B->d_slots, B->d_bmap + B->v_off, B->d_tick, it);
+    if (rc || err != hipSuccess) break;
+    const int32_t *bm_v = B->d_bmap + B->v_off;
+    int k = it;
+    void *s_args[] = {(void *)&slots, (void *)&bm_v, (void *)&tick, (void *)&k};
+    err = add(reinterpret_cast<const void *>(&k_batch_step), B->v_n, kBlock, 0, s_args);
   }
-  k_batch_tick<<<1, 1, 0, s>>>(B->d_tick, iters);
-  TATVA_CHECK_LAUNCH();
+  if (err == hipSuccess && rc == TATVA_OK) {
+    int n_it = iters;
+    void *t_args[] = {(void *)&tick, (void *)&n_it};
+    err = add(reinterpret_cast<const void *>(&k_batch_tick), 1, 1, 0, t_args);
+  }
+  if (err != hipSuccess || rc != TATVA_OK) {
+    (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    set_error(std::string("hipGraphAddKernelNode: ") + hipGetErrorString(err));
+    return TATVA_EHIP;
+  }
+  *out = g;
   return TATVA_OK;
 }
 
-// Everything the captured launch sequence depends on besides device memory.
+// Everything the graph of a launch sequence depends on besides device memory.
 std::vector<int64_t> launch_plan(const tatva_batch *B, int iters) {
   std::vector<int64_t> p{iters, B->v_off, B->v_n, (int64_t)(intptr_t)B->d_bmap,
                          (int64_t)(intptr_t)B->d_slots};
@@ -583,22 +623,14 @@ extern "C" int tatva_batch_iterate(tatva_batch *B, int32_t iters, void *stream) 
   int rc = batch_upload(B, s);
   if (rc) return rc;
   if (B->v_n == 0 || iters == 0) return TATVA_OK;
-  // the sequence runs from a captured graph on the batch's own stream (capture
-  // is not allowed on the legacy default stream), joined to s with events; it
-  // is re-captured when the launch plan changes (slots joining or leaving)
+  // the sequence runs as a graph on the batch's own stream, joined to s with
+  // events; the graph is rebuilt when the launch plan changes (slots joining
+  // or leaving)
   std::vector<int64_t> plan = launch_plan(B, iters);
   if (!B->exec || plan != B->plan) {
     B->drop_graph();
-    TATVA_TRY_HIP(hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_iterations(B, iters, B->cap);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(B->cap, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    TATVA_TRY_HIP(e);
-    B->graph = g;
+    rc = build_iterations(B, iters, &B->graph);
+    if (rc) return rc;
     TATVA_TRY_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
     B->plan = std::move(plan);
   }
