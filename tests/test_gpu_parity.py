@@ -336,14 +336,17 @@ def test_pcg_matches_spsolve_and_oracle_iterations():
     assert info.iterations == 3 and not info.converged
 
 
-def test_pcg_many_spmv_blocks_both_layouts():
-    """n = 545,832: both operators run more SpMV blocks than the CG step re-sums,
-    so their shares pass through the 64-block share sum.  The node-block view and
-    the sliced ELL copy solve the same system; a pattern with one dof of the
-    x = 1 face constrained (not node-blocked) solves through the sliced ELL."""
+@pytest.mark.parametrize("cells,n_expected", [(24, 45000), (56, 545832)])
+def test_pcg_graph_path_both_layouts(cells, n_expected):
+    """Systems above the one-workgroup size run the graph-captured PCG.  At
+    n = 45,000 both operators' block shares fit the CG step's re-sum; at
+    n = 545,832 they exceed it and pass through the 64-block share sum.  The
+    node-block view and the sliced ELL copy solve the same system; a pattern
+    with one dof of the x = 1 face constrained (not node-blocked) solves
+    through the sliced ELL."""
     import torch
 
-    m = ta.Mesh.unit_cube(56)
+    m = ta.Mesh.unit_cube(cells)
     op = ta.Operator(m, ta.Hexahedron8())
     E = op.energy(energies.LinearElasticity(0.4, 0.6))
     x0n = np.where(np.isclose(m.coords[:, 0], 0.0))[0]
@@ -352,7 +355,7 @@ def test_pcg_many_spmv_blocks_both_layouts():
     cm = sparse.ColoredMatrix.from_csr(L.adapt_sparsity(sparse.pattern_from_mesh(m, 3)))
     K = sparse.jacfwd(E.reduced(L).grad, cm)(np.zeros(L.size_reduced))
     n = L.size_reduced
-    assert n == 545832 and n % 64 != 0
+    assert n == n_expected and n % 64 != 0
     b = np.random.default_rng(11).standard_normal(n)
     bt = torch.as_tensor(b, device="cuda")
     xs, infos = solvers.pcg(K, b, tol=1e-10, chunk=16)
