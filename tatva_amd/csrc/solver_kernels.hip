@@ -157,13 +157,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_bspmv(int64_t nbr, const int32_t
   const int64_t stride = (int64_t)gridDim.x * G;
   double dot = 0.0;
   for (int64_t I = lb * G + threadIdx.x / kBLanes; I < nbr; I += stride) {
+    // the row's own p entries, loaded with the row so the dot needs no extra round trip
+    double pv[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) pv[c] = p[B * I + c];  // same address in every lane
     double acc[B];
     bcsr_block_row<B>(I, lane, indptr, bcol, data, p, acc);
     if (lane == 0) {
 #pragma unroll
       for (int c = 0; c < B; ++c) {
         Ap[B * I + c] = acc[c];
-        dot += acc[c] * p[B * I + c];
+        dot += acc[c] * pv[c];
       }
     }
   }
@@ -295,6 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv(int64_t n, int64_t nsl,
   const int64_t s = xcd_block(blockIdx.x, gridDim.x) * kSellSlicesPerBlock + threadIdx.x / kSellRows;
   double dot = 0.0;
   if (s < nsl) {
+    const int64_t i = s * kSellRows + l;
+    const double xi = i < n ? x[i] : 0.0;  // for the dot, loaded before the row
     const int64_t o = soff[s];
     const int J = (int)(soff[s + 1] - o);
     const int32_t *__restrict__ c = cols + o * kSellRows + l;
@@ -302,11 +308,10 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv(int64_t n, int64_t nsl,
     double a = 0.0;
 #pragma unroll 8
     for (int j = 0; j < J; ++j) a += v[j * kSellRows] * x[c[j * kSellRows]];
-    const int64_t i = s * kSellRows + l;
     if (i < n) {
       const double yi = b ? b[i] - a : a;
       y[i] = yi;
-      dot = yi * x[i];
+      dot = yi * xi;
     }
   }
   if (part) {
