@@ -115,7 +115,12 @@ def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=Fals
     """The same problem solved by FP64 colored-CSR Newton and by mixed Newton
     (FP32 matrix-free PCG inside FP64 Newton/iterative refinement), both to
     ||r|| <= tol ||r_0||.  ``run(solver)`` performs the solve and returns
-    (u, [NewtonStats]); it is called once untimed first."""
+    (u, [NewtonStats]); it is called once untimed first.  Device memory per
+    solver: the peak torch reserved during its solve, and the device memory in
+    use (total - free, library buffers included) with the solver alive, above
+    the level before it was built."""
+    import gc
+
     import torch
 
     from tatva_amd import solvers
@@ -123,6 +128,11 @@ def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=Fals
     res = {}
     sols = {}
     for mode in ("csr", "mixed"):
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        torch.cuda.reset_peak_memory_stats()
+        free0, _total = torch.cuda.mem_get_info()
         kw = dict(lin_tol=1e-10, colored_matrix=colored_matrix) if mode == "csr" \
             else dict(inner_tol=1e-3)
         t_build = time.perf_counter()
@@ -150,7 +160,11 @@ def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=Fals
                                       for s in stats),
             "operator_us": (op_ms / op_n * 1e3) if op_n else None,
         }
+        free1, _total = torch.cuda.mem_get_info()
+        res[mode]["device_gb_in_use"] = (free0 - free1) / 1e9
+        res[mode]["torch_peak_reserved_gb"] = torch.cuda.max_memory_reserved() / 1e9
         sols[mode] = u
+        del solver
     a, b = sols["mixed"], sols["csr"]
     res["mixed_vs_fp64_rel_error"] = float((a - b).abs().max() / b.abs().max())
     if keep_solutions:
