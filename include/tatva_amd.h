@@ -149,7 +149,7 @@ int64_t tatva_colored_hessian_workspace(tatva_mesh *mesh, int ncomp);
 /* Colored sparse Hessian decompressed into CSR data.  Replaces sparse.jacfwd /
  * colored_jacobian_batch (tatva/sparse/base.py:139-176,230-270): one JVP per
  * color with seed (colors == c), J_compressed[row, color], then
- * data[k] = J_compressed[rows[k], colors[indices[k]]] (base.py:167-174).
+ * data[k] = J_compressed[rows[k], colors[indices[k]]] (sparse/base.py:167-174).
  * The pattern (indptr, indices) and colors refer to the reduced (free) dofs
  * when dof_map != NULL; red_to_full (n_rows) then lists the full dof of each row. */
 int tatva_colored_hessian(tatva_mesh *mesh, const tatva_density *density, const double *u,
@@ -171,7 +171,7 @@ int tatva_colored_hessian_mapped(tatva_mesh *mesh, const tatva_density *density,
                                  const int32_t *indices, int64_t nnz, int32_t flags, double *data,
                                  void *workspace, int64_t workspace_bytes, void *stream);
 
-/* L2 projection (Operator.project, tatva/operator.py:523-560 and the mass /
+/* L2 projection (Operator.project, tatva/operator.py:523-554 and the mass /
  * right-hand-side forms of tatva/utils.py:232-262).  tatva_mass_matrix writes
  * the consistent mass matrix of an ncomp-component field (block diagonal over
  * components, sum_q N_a N_b detJ w) into CSR data on a (reduced) pattern, as

@@ -720,7 +720,7 @@ def interpolate(elem, coords, elements, nodal, points):
 
 
 def project(elem, coords, elements, field_q, lifter=None):
-    """L2 projection (operator.py:523-560, utils.py:118-262) with the scalar
+    """L2 projection (operator.py:523-554, utils.py:118-262) with the scalar
     mass matrix and one right-hand side per component: M_ab = sum N_a N_b w,
     b_a = sum N_a f w; with a lifter, rows/columns of the free dofs, solution
     lifted from zeros.  Dense assembly; for small meshes."""

@@ -267,7 +267,7 @@ class Operator:
 
     def project(self, field_q, colored_matrix=None, lifter=None, *, tol: float = 1e-13):
         """L2 projection of a quadrature-point field onto the nodal space
-        (operator.py:523-560, utils.py:118-262): solve M x = b with the
+        (operator.py:523-554, utils.py:118-262): solve M x = b with the
         consistent mass matrix M_ab = int N_a N_b and b_a = int N_a f.
 
         ``field_q`` is (n_elements, n_quad, ...).  Without ``colored_matrix``
