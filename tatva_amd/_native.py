@@ -134,10 +134,18 @@ class NotConverged(TatvaError):
 
 
 def lib():
-    """Load the native library (raises if it has not been built)."""
+    """Load the native library (raises if it has not been built).
+
+    torch is imported first: its wheel carries its own HIP and HSA runtimes
+    under the same sonames as ROCm's, so the library then binds to the copy
+    torch already mapped.  Loaded the other way round, the process maps both
+    copies, and the second HSA runtime sees no GPU (hipSetDevice reports no
+    ROCm-capable device); device pointers from torch would also belong to a
+    runtime the library does not use."""
     global _lib
     if _lib is not None:
         return _lib
+    torch()
     with _lock:
         if _lib is not None:
             return _lib

@@ -33,6 +33,23 @@ def test_library_exports_every_declared_symbol():
     assert L.tatva_version() >= 1000
 
 
+def test_library_binds_to_the_hip_runtime_torch_maps():
+    """Loaded first thing in a fresh process, the library must not map a
+    second HIP/HSA runtime beside torch's (that second runtime sees no GPU)."""
+    import subprocess
+    import sys
+    code = ("import os\n"
+            "from tatva_amd import _native as nat\n"
+            "nat.lib()\n"
+            "maps = open(f'/proc/{os.getpid()}/maps').read().splitlines()\n"
+            "for key in ('amdhip64', 'hsa-runtime64'):\n"
+            "    print(key, len({l.split()[-1] for l in maps if key in l}))\n")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert "amdhip64 1" in out.stdout and "hsa-runtime64 1" in out.stdout, out.stdout
+
+
 def test_error_plumbing():
     h = nat.c_void()
     rc = nat.lib().tatva_pattern_from_mesh(None, 1, 3, 3, 1, None, 3, nat.ctypes.byref(h))
