@@ -8,10 +8,15 @@
 // connectivity, the node coordinates and the nodal vectors.
 #include <hip/hip_runtime.h>
 
+#include <sched.h>
+
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <new>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -633,133 +638,266 @@ uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
   return v;
 }
 
+// Host threads for the set-up passes: the CPUs this process may run on, at
+// most 16, and no more than there are work items.
+int host_threads(int64_t items) {
+  int n = 1;
+  cpu_set_t cs;
+  if (sched_getaffinity(0, sizeof(cs), &cs) == 0) n = CPU_COUNT(&cs);
+  n = std::min(n, 16);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(n, items));
+}
+
+// f(begin, end, worker) over [0, n) in `threads` contiguous ranges; false if
+// a worker ran out of memory.
+template <class F>
+bool parallel_ranges(int64_t n, int threads, F &&f) {
+  std::atomic<bool> ok{true};
+  auto run = [&](int64_t b, int64_t e, int w) {
+    try {
+      f(b, e, w);
+    } catch (const std::bad_alloc &) {
+      ok = false;
+    }
+  };
+  if (threads <= 1 || n < 2) {
+    run(0, n, 0);
+    return ok;
+  }
+  std::vector<std::thread> pool;
+  for (int w = 0; w < threads; ++w) {
+    const int64_t b = n * w / threads, e = n * (w + 1) / threads;
+    if (b < e) pool.emplace_back(run, b, e, w);
+  }
+  for (auto &t : pool) t.join();
+  return ok;
+}
+
+// Sorts (code, element) pairs: sorted ranges, then rounds of pairwise merges.
+// The pairs are distinct, so the result equals one std::sort.
+using MortonKey = std::pair<uint64_t, int32_t>;
+bool sort_keys(std::vector<MortonKey> &v, int threads) {
+  const int64_t n = (int64_t)v.size();
+  if (threads <= 1 || n < (int64_t(1) << 16)) {
+    std::sort(v.begin(), v.end());
+    return true;
+  }
+  std::vector<int64_t> bnd((size_t)threads + 1);
+  for (int i = 0; i <= threads; ++i) bnd[(size_t)i] = n * i / threads;
+  if (!parallel_ranges(threads, threads, [&](int64_t b, int64_t e, int) {
+        for (int64_t i = b; i < e; ++i) std::sort(v.begin() + bnd[(size_t)i], v.begin() + bnd[(size_t)i + 1]);
+      }))
+    return false;
+  std::vector<MortonKey> tmp((size_t)n);
+  std::vector<MortonKey> *src = &v, *dst = &tmp;
+  for (int width = 1; width < threads; width *= 2) {
+    const int64_t pairs = (threads + 2 * width - 1) / (2 * width);
+    if (!parallel_ranges(pairs, host_threads(pairs), [&](int64_t b, int64_t e, int) {
+          for (int64_t q = b; q < e; ++q) {
+            const int lo = (int)(q * 2 * width);
+            const int mid = std::min(threads, lo + width), hi = std::min(threads, lo + 2 * width);
+            auto s0 = src->begin() + bnd[(size_t)lo], s1 = src->begin() + bnd[(size_t)mid];
+            auto s2 = src->begin() + bnd[(size_t)hi];
+            std::merge(s0, s1, s1, s2, dst->begin() + bnd[(size_t)lo]);
+          }
+        }))
+      return false;
+    std::swap(src, dst);
+  }
+  if (src != &v) v.swap(*src);
+  return true;
+}
+
 // Z-order of element centroids, tiles of kTile consecutive elements in that
-// order, local node numbering per tile, greedy element coloring inside each
-// tile (elements of a color share no node), node -> slot lists.
+// order, local node numbering per tile (ascending global node), greedy
+// element coloring inside each tile (elements of a color share no node),
+// node -> slot lists.  Tiles are independent, so they are built by several
+// host threads and concatenated in tile order (the result does not depend on
+// the thread count).
 int build_tiles(const double *coords, int dim, int64_t N, const int32_t *elements, int64_t E,
                 int nne, bool incidence, HostTiles &T) {
-  std::vector<double> cen((size_t)E * 3, 0.0);
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int64_t e = 0; e < E; ++e)
-    for (int j = 0; j < dim; ++j) {
-      double s = 0.0;
-      for (int a = 0; a < nne; ++a) s += coords[(int64_t)elements[e * nne + a] * dim + j];
-      s /= nne;
-      cen[(size_t)e * 3 + j] = s;
-      lo[j] = std::min(lo[j], s);
-      hi[j] = std::max(hi[j], s);
-    }
-  std::vector<std::pair<uint64_t, int32_t>> key((size_t)E);
-  for (int64_t e = 0; e < E; ++e) {
-    uint64_t code = 0;
-    for (int j = 0; j < dim; ++j) {
-      const double span = hi[j] - lo[j];
-      const double t = span > 0 ? (cen[(size_t)e * 3 + j] - lo[j]) / span : 0.0;
-      const uint64_t qv = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
-      code |= spread3(qv) << j;
-    }
-    key[(size_t)e] = {code, (int32_t)e};
-  }
-  std::sort(key.begin(), key.end());
+  constexpr int kMaxColors = 128;
   const int64_t n_tiles = (E + kTile - 1) / kTile;
   if (n_tiles >= (int64_t(1) << 31)) {
     set_error("too many tiles");
     return TATVA_EINVAL;
   }
+  const int th_e = host_threads(E / 65536 + 1);
+  // centroids, their bounding box, Morton codes
+  std::vector<double> cen((size_t)E * 3, 0.0);
+  std::vector<std::array<double, 6>> box((size_t)th_e, {1e300, 1e300, 1e300, -1e300, -1e300, -1e300});
+  bool ok = parallel_ranges(E, th_e, [&](int64_t b, int64_t e1, int w) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t e = b; e < e1; ++e)
+      for (int j = 0; j < dim; ++j) {
+        double s = 0.0;
+        for (int a = 0; a < nne; ++a) s += coords[(int64_t)elements[e * nne + a] * dim + j];
+        s /= nne;
+        cen[(size_t)e * 3 + j] = s;
+        lo[j] = std::min(lo[j], s);
+        hi[j] = std::max(hi[j], s);
+      }
+    box[(size_t)w] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+  });
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (const auto &bx : box)
+    for (int j = 0; j < 3; ++j) {
+      lo[j] = std::min(lo[j], bx[(size_t)j]);
+      hi[j] = std::max(hi[j], bx[(size_t)j + 3]);
+    }
+  std::vector<MortonKey> key((size_t)E);
+  ok = ok && parallel_ranges(E, th_e, [&](int64_t b, int64_t e1, int) {
+    for (int64_t e = b; e < e1; ++e) {
+      uint64_t code = 0;
+      for (int j = 0; j < dim; ++j) {
+        const double span = hi[j] - lo[j];
+        const double t = span > 0 ? (cen[(size_t)e * 3 + j] - lo[j]) / span : 0.0;
+        const uint64_t qv = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+        code |= spread3(qv) << j;
+      }
+      key[(size_t)e] = {code, (int32_t)e};
+    }
+  });
+  ok = ok && sort_keys(key, th_e);
+  if (!ok) {
+    set_error("out of host memory while tiling the mesh");
+    return TATVA_ENOMEM;
+  }
+  std::vector<double>().swap(cen);
+  // per tile: local nodes, colors, color-ordered local connectivity
   T.n_tiles = (int32_t)n_tiles;
   T.lnodes.assign((size_t)n_tiles * kTile * nne, 0);
   if (incidence) T.inc_list.assign((size_t)n_tiles * kTile * nne, 0);
+  std::vector<std::vector<int32_t>> tloc((size_t)n_tiles);  // sorted local -> global node
+  std::vector<std::vector<uint16_t>> tinc((size_t)n_tiles);  // incidence list ends per local node
+  std::vector<int16_t> toff((size_t)n_tiles * (kMaxColors + 1), 0);
+  std::vector<int32_t> tcol((size_t)n_tiles, 0);
+  std::atomic<int> bad{0};  // 1: too many local nodes, 2: too many colors
+  const int th_t = host_threads(n_tiles / 16 + 1);
+  ok = parallel_ranges(n_tiles, th_t, [&](int64_t t0, int64_t t1, int) {
+    std::vector<int32_t> mark((size_t)N, -1), lid((size_t)N);
+    std::vector<uint64_t> used;
+    std::vector<int> color(kTile);
+    std::vector<std::array<uint16_t, 8>> ln(kTile);
+    std::vector<int> ord(kTile);
+    std::vector<int32_t> cntn;
+    for (int64_t t = t0; t < t1 && !bad; ++t) {
+      const int64_t b = t * kTile, cnt = std::min<int64_t>(kTile, E - b);
+      std::vector<int32_t> &loc = tloc[(size_t)t];
+      for (int64_t k = 0; k < cnt; ++k)
+        for (int a = 0; a < nne; ++a) {
+          const int32_t nd = elements[(int64_t)key[(size_t)(b + k)].second * nne + a];
+          if (mark[(size_t)nd] != (int32_t)t) {
+            mark[(size_t)nd] = (int32_t)t;
+            loc.push_back(nd);
+          }
+        }
+      std::sort(loc.begin(), loc.end());
+      if (loc.size() > 65535) {
+        bad = 1;
+        return;
+      }
+      for (size_t i = 0; i < loc.size(); ++i) lid[(size_t)loc[i]] = (int32_t)i;
+      used.assign(loc.size() * 2, 0);
+      int ncol = 0;
+      for (int64_t k = 0; k < cnt; ++k) {
+        const int32_t e = key[(size_t)(b + k)].second;
+        uint64_t m0 = 0, m1 = 0;
+        for (int a = 0; a < nne; ++a) {
+          const uint16_t l = (uint16_t)lid[(size_t)elements[(int64_t)e * nne + a]];
+          ln[(size_t)k][a] = l;
+          m0 |= used[2 * l];
+          m1 |= used[2 * l + 1];
+        }
+        int c = -1;
+        if (~m0) c = __builtin_ctzll(~m0);
+        else if (~m1) c = 64 + __builtin_ctzll(~m1);
+        if (c < 0) {
+          bad = 2;
+          return;
+        }
+        color[(size_t)k] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < nne; ++a) {
+          const uint16_t l = ln[(size_t)k][a];
+          used[2 * l + (c >> 6)] |= uint64_t(1) << (c & 63);
+        }
+      }
+      tcol[(size_t)t] = ncol;
+      // the tile's elements ordered by color, ties in tile order (counting sort)
+      int16_t *off = &toff[(size_t)t * (kMaxColors + 1)];
+      for (int64_t k = 0; k < cnt; ++k) off[color[(size_t)k] + 1]++;
+      for (int c = 0; c < ncol; ++c) off[c + 1] += off[c];
+      int16_t fillc[kMaxColors];
+      for (int c = 0; c < ncol; ++c) fillc[c] = off[c];
+      for (int64_t k = 0; k < cnt; ++k) ord[(size_t)fillc[color[(size_t)k]]++] = (int)k;
+      for (int64_t k = 0; k < cnt; ++k) {
+        const int src = ord[(size_t)k];
+        for (int a = 0; a < nne; ++a) T.lnodes[(size_t)((b + k) * nne + a)] = ln[(size_t)src][a];
+      }
+      if (incidence) {
+        // entries k * nne + a of each local node, ascending (a counting sort)
+        cntn.assign(loc.size() + 1, 0);
+        for (int64_t q = 0; q < cnt * nne; ++q) cntn[(size_t)T.lnodes[(size_t)(b * nne + q)] + 1]++;
+        for (size_t j = 0; j < loc.size(); ++j) cntn[j + 1] += cntn[j];
+        tinc[(size_t)t].assign(cntn.begin() + 1, cntn.end());
+        for (int64_t q = 0; q < cnt * nne; ++q) {
+          const uint16_t l = T.lnodes[(size_t)(b * nne + q)];
+          T.inc_list[(size_t)(b * nne + cntn[l]++)] = (uint16_t)q;
+        }
+      }
+    }
+  });
+  if (!ok) {
+    set_error("out of host memory while tiling the mesh");
+    return TATVA_ENOMEM;
+  }
+  if (bad == 1) {
+    set_error("tile has too many local nodes");
+    return TATVA_EINVAL;
+  }
+  if (bad == 2) {
+    set_error("more than 128 element colors in a tile");
+    return TATVA_EINVAL;
+  }
+  // concatenation in tile order
   T.node_ptr.assign((size_t)n_tiles + 1, 0);
-  std::vector<std::vector<int16_t>> offs((size_t)n_tiles);
-  std::vector<int32_t> &tile_nodes = T.tnodes;  // global node id of every slot
-  std::vector<int32_t> loc;         // scratch
-  std::vector<uint64_t> used;       // per local node: color bitmask (128 colors)
+  T.max_local = 0;
+  T.max_colors = 0;
   for (int64_t t = 0; t < n_tiles; ++t) {
-    const int64_t b = t * kTile, cnt = std::min<int64_t>(kTile, E - b);
-    loc.clear();
-    for (int64_t k = 0; k < cnt; ++k)
-      for (int a = 0; a < nne; ++a) loc.push_back(elements[(int64_t)key[(size_t)(b + k)].second * nne + a]);
-    std::sort(loc.begin(), loc.end());
-    loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
-    if (loc.size() > 65535) {
-      set_error("tile has too many local nodes");
-      return TATVA_EINVAL;
-    }
-    T.max_local = std::max<int32_t>(T.max_local, (int32_t)loc.size());
-    used.assign(loc.size() * 2, 0);
-    std::vector<int> color((size_t)cnt);
-    std::vector<std::array<uint16_t, 8>> ln((size_t)cnt);
-    int ncol = 0;
-    for (int64_t k = 0; k < cnt; ++k) {
-      const int32_t e = key[(size_t)(b + k)].second;
-      uint64_t m0 = 0, m1 = 0;
-      for (int a = 0; a < nne; ++a) {
-        const int32_t nd = elements[(int64_t)e * nne + a];
-        const uint16_t l = (uint16_t)(std::lower_bound(loc.begin(), loc.end(), nd) - loc.begin());
-        ln[(size_t)k][a] = l;
-        m0 |= used[2 * l];
-        m1 |= used[2 * l + 1];
-      }
-      int c = -1;
-      if (~m0) c = __builtin_ctzll(~m0);
-      else if (~m1) c = 64 + __builtin_ctzll(~m1);
-      if (c < 0) {
-        set_error("more than 128 element colors in a tile");
-        return TATVA_EINVAL;
-      }
-      color[(size_t)k] = c;
-      ncol = std::max(ncol, c + 1);
-      for (int a = 0; a < nne; ++a) {
-        const uint16_t l = ln[(size_t)k][a];
-        used[2 * l + (c >> 6)] |= uint64_t(1) << (c & 63);
-      }
-    }
-    T.max_colors = std::max(T.max_colors, ncol);
-    // order the tile's elements by color (stable)
-    std::vector<int64_t> ord((size_t)cnt);
-    for (int64_t k = 0; k < cnt; ++k) ord[(size_t)k] = k;
-    std::stable_sort(ord.begin(), ord.end(),
-                     [&](int64_t x, int64_t y) { return color[(size_t)x] < color[(size_t)y]; });
-    std::vector<int16_t> off((size_t)ncol + 1, 0);
-    for (int64_t k = 0; k < cnt; ++k) off[(size_t)color[(size_t)k] + 1]++;
-    for (int c = 0; c < ncol; ++c) off[(size_t)c + 1] += off[(size_t)c];
-    offs[(size_t)t] = off;
-    for (int64_t k = 0; k < cnt; ++k) {
-      const int64_t src = ord[(size_t)k];
-      for (int a = 0; a < nne; ++a) T.lnodes[(size_t)((b + k) * nne + a)] = ln[(size_t)src][a];
-    }
-    if (incidence) {
-      // entries k * nne + a of each local node, ascending (a counting sort)
-      std::vector<int32_t> cntn(loc.size() + 1, 0);
-      for (int64_t q = 0; q < cnt * nne; ++q) cntn[(size_t)T.lnodes[(size_t)(b * nne + q)] + 1]++;
-      for (size_t j = 0; j < loc.size(); ++j) cntn[j + 1] += cntn[j];
-      for (size_t j = 0; j < loc.size(); ++j) T.inc_end.push_back((uint16_t)cntn[j + 1]);
-      for (int64_t q = 0; q < cnt * nne; ++q) {
-        const uint16_t l = T.lnodes[(size_t)(b * nne + q)];
-        T.inc_list[(size_t)(b * nne + cntn[l]++)] = (uint16_t)q;
-      }
-    }
-    T.node_ptr[(size_t)t + 1] = T.node_ptr[(size_t)t] + (int32_t)loc.size();
-    tile_nodes.insert(tile_nodes.end(), loc.begin(), loc.end());
-    if ((int64_t)tile_nodes.size() >= (int64_t(1) << 31)) {
+    const int64_t sz = (int64_t)tloc[(size_t)t].size();
+    if ((int64_t)T.node_ptr[(size_t)t] + sz >= (int64_t(1) << 31)) {
       set_error("too many tile slots");
       return TATVA_EINVAL;
     }
+    T.node_ptr[(size_t)t + 1] = T.node_ptr[(size_t)t] + (int32_t)sz;
+    T.max_local = std::max<int32_t>(T.max_local, (int32_t)sz);
+    T.max_colors = std::max(T.max_colors, tcol[(size_t)t]);
   }
+  const size_t n_slots = (size_t)T.node_ptr.back();
+  T.tnodes.resize(n_slots);
+  if (incidence) T.inc_end.resize(n_slots);
+  for (int64_t t = 0; t < n_tiles; ++t) {
+    const size_t at = (size_t)T.node_ptr[(size_t)t];
+    std::copy(tloc[(size_t)t].begin(), tloc[(size_t)t].end(), T.tnodes.begin() + at);
+    if (incidence) std::copy(tinc[(size_t)t].begin(), tinc[(size_t)t].end(), T.inc_end.begin() + at);
+  }
+  std::vector<std::vector<int32_t>>().swap(tloc);
+  std::vector<std::vector<uint16_t>>().swap(tinc);
   T.color_off.assign((size_t)n_tiles * (T.max_colors + 1), 0);
   for (int64_t t = 0; t < n_tiles; ++t) {
-    const auto &off = offs[(size_t)t];
-    const int16_t last = off.back();
+    const int16_t *off = &toff[(size_t)t * (kMaxColors + 1)];
+    const int ncol = tcol[(size_t)t];
     for (int c = 0; c <= T.max_colors; ++c)
-      T.color_off[(size_t)t * (T.max_colors + 1) + c] = c < (int)off.size() ? off[(size_t)c] : last;
+      T.color_off[(size_t)t * (T.max_colors + 1) + c] = off[std::min(c, ncol)];
   }
   // node -> slots (tiles visited in order, so each list ascends by tile)
   T.slot_ptr.assign((size_t)N + 1, 0);
-  for (int32_t nd : tile_nodes) T.slot_ptr[(size_t)nd + 1]++;
+  for (int32_t nd : T.tnodes) T.slot_ptr[(size_t)nd + 1]++;
   for (int64_t i = 0; i < N; ++i) T.slot_ptr[(size_t)i + 1] += T.slot_ptr[(size_t)i];
-  T.slot_pos.assign(tile_nodes.size(), 0);
+  T.slot_pos.assign(n_slots, 0);
   std::vector<int32_t> fill(T.slot_ptr.begin(), T.slot_ptr.end() - 1);
-  for (size_t sl = 0; sl < tile_nodes.size(); ++sl) T.slot_pos[sl] = fill[(size_t)tile_nodes[sl]]++;
+  for (size_t sl = 0; sl < n_slots; ++sl) T.slot_pos[sl] = fill[(size_t)T.tnodes[sl]]++;
   return TATVA_OK;
 }
 
@@ -832,8 +970,16 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     for (int j = 0; j < 3; ++j) m->quad.xi[q][j] = j < dim ? quad_points[q * dim + j] : 0.0;
     m->quad.w[q] = quad_weights[q];
   }
+  // Uploads go through a private non-blocking stream: mesh creation on a
+  // helper thread (a server preparing requests) then does not wait for work
+  // queued on the default stream.
+  hipStream_t up = nullptr;
   auto fail = [&](hipError_t e, const char *what) {
     set_error(std::string(what) + ": " + hipGetErrorString(e));
+    if (up) {
+      (void)hipStreamSynchronize(up);
+      (void)hipStreamDestroy(up);
+    }
     tatva_mesh_destroy(m);
     return TATVA_EHIP;
   };
@@ -858,6 +1004,10 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
   }
   hipError_t err = hipSetDevice(device);
   if (err != hipSuccess) return fail(err, "hipSetDevice");
+  if ((err = hipStreamCreateWithFlags(&up, hipStreamNonBlocking)) != hipSuccess) {
+    up = nullptr;
+    return fail(err, "hipStreamCreateWithFlags");
+  }
   // tile partial sums: n_slots x ncomp (ncomp <= dim) values of at most 8 bytes
   m->scratch_bytes = std::max<int64_t>(1, m->n_slots) * dim * (int64_t)sizeof(double);
   auto upload = [&](auto **dst, const auto &vec, const char *what) -> hipError_t {
@@ -865,7 +1015,7 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     hipError_t e2 = hipMalloc((void **)dst, std::max<size_t>(1, vec.size()) * sizeof(V));
     if (e2 != hipSuccess) return e2;
     (void)what;
-    return hipMemcpy(*dst, vec.data(), vec.size() * sizeof(V), hipMemcpyHostToDevice);
+    return hipMemcpyAsync(*dst, vec.data(), vec.size() * sizeof(V), hipMemcpyHostToDevice, up);
   };
   if ((err = upload(&m->t_lnodes, T.lnodes, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_color_off, T.color_off, "tiles")) != hipSuccess) return fail(err, "tiles");
@@ -892,14 +1042,20 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     return fail(err, "hipMalloc scratch");
   if ((err = hipMalloc(&m->partials, kMaxReduceBlocks * 4 * sizeof(double))) != hipSuccess)
     return fail(err, "hipMalloc partials");
-  if ((err = hipMemcpy(m->coords, coords, n_nodes * dim * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+  if ((err = hipMemcpyAsync(m->coords, coords, n_nodes * dim * sizeof(double), hipMemcpyHostToDevice, up)) != hipSuccess)
     return fail(err, "copy coords");
-  if ((err = hipMemcpy(m->conn, elements, total * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
+  if ((err = hipMemcpyAsync(m->conn, elements, total * sizeof(int32_t), hipMemcpyHostToDevice, up)) != hipSuccess)
     return fail(err, "copy conn");
-  if ((err = hipMemcpy(m->inv_ptr, ptr.data(), (n_nodes + 1) * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
+  if ((err = hipMemcpyAsync(m->inv_ptr, ptr.data(), (n_nodes + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                            up)) != hipSuccess)
     return fail(err, "copy inv_ptr");
-  if ((err = hipMemcpy(m->inv_list, list.data(), total * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
+  if ((err = hipMemcpyAsync(m->inv_list, list.data(), total * sizeof(int32_t), hipMemcpyHostToDevice,
+                            up)) != hipSuccess)
     return fail(err, "copy inv_list");
+  // the host arrays above must outlive the copies
+  if ((err = hipStreamSynchronize(up)) != hipSuccess) return fail(err, "mesh upload");
+  (void)hipStreamDestroy(up);
+  up = nullptr;
   *out = m;
   return TATVA_OK;
 }

@@ -24,8 +24,9 @@ from __future__ import annotations
 
 import ctypes
 import math
+import threading
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor
 from concurrent.futures import wait as futures_wait
 from dataclasses import dataclass, field
 
@@ -344,6 +345,17 @@ class ServeReport:
         }
 
 
+def _warm_host() -> None:
+    """Pays one-time host costs of the process at server start-up instead of
+    inside a request.  The first numpy expression that could reuse a large
+    temporary (arrays of 256 KiB and more) checks its caller with
+    backtrace(), whose first call walks the unwind tables of every loaded
+    shared library; with torch's ROCm libraries mapped that takes ~0.1-0.3 s
+    and holds the GIL, stalling the serving loop with it."""
+    a = np.ones(1 << 16)
+    _ = (a + a) + a
+
+
 class Server:
     """Continuous-batching solve server on one GPU.
 
@@ -351,10 +363,13 @@ class Server:
     from the start of the run) and returns a ServeReport.  ``max_slots``
     bounds the requests in flight, ``hbm_budget`` their device memory
     (SolveRequest.hbm_bytes); ``chunk`` PCG iterations run between polls.
-    A worker thread prepares up to ``prefetch`` arrived requests (mesh,
-    tiling, constraints, device buffers) while the GPU iterates the others;
-    a request's bytes count against the budget from the start of its
-    preparation.
+    ``prefetch`` worker threads prepare arrived requests (mesh, tiling,
+    constraints, device buffers) while the GPU iterates the others, each on
+    a torch stream of its own so that preparation does not wait for the
+    batch work queued on the serving stream; a prepared request is admitted
+    as soon as it is ready and a slot is free (preparation starts in arrival
+    order).  A request's bytes count against the budget from the start of
+    its preparation.
     """
 
     def __init__(self, *, max_slots: int = 64, hbm_budget: float = 200e9, chunk: int = 8,
@@ -375,6 +390,21 @@ class Server:
                                                ctypes.byref(h)))
         self._batch = h.value
         self._stats = (nat.PcgStats * self.max_slots)()
+        self._local = threading.local()
+        _warm_host()
+
+    def _prepare(self, req: SolveRequest):
+        """Worker thread: the request's _Job, built on the thread's own stream,
+        and an event the serving stream waits for before using it."""
+        t = nat.torch()
+        st = getattr(self._local, "stream", None)
+        if st is None:
+            st = self._local.stream = t.cuda.Stream(device=nat.device())
+        with t.cuda.stream(st):
+            job = _Job(req, self)
+            ready = t.cuda.Event()
+            ready.record(st)
+        return job, ready
 
     def close(self):
         if getattr(self, "_batch", None):
@@ -404,7 +434,7 @@ class Server:
         peak_adm = 0
         peak_bytes = 0
         dof_iters = 0
-        pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="tatva-prepare")
+        pool = ThreadPoolExecutor(max_workers=self.prefetch, thread_name_prefix="tatva-prepare")
         t.cuda.synchronize()
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else None
@@ -413,16 +443,21 @@ class Server:
                 now = clock() if realtime else 0.0
                 while pending and (not realtime or pending[0].arrival <= now):
                     waiting.append(pending.pop(0))
-                # preparation: FIFO, at most `prefetch` ahead, within the HBM budget
+                # preparation: in arrival order, at most `prefetch` at once, within
+                # the HBM budget
                 while waiting and len(preparing) < self.prefetch and \
                         used_bytes + waiting[0].hbm_bytes() <= self.hbm_budget:
                     req = waiting.pop(0)
                     used_bytes += req.hbm_bytes()
-                    preparing.append((req, pool.submit(_Job, req, self)))
-                # admission: FIFO, as prepared requests and slots become available
-                while preparing and free_slots and preparing[0][1].done():
-                    req, fut = preparing.pop(0)
-                    job = fut.result()
+                    preparing.append((req, pool.submit(self._prepare, req)))
+                # admission: every prepared request, in arrival order, while slots last
+                for item in [p for p in preparing if p[1].done()]:
+                    if not free_slots:
+                        break
+                    preparing.remove(item)
+                    req, fut = item
+                    job, ready = fut.result()
+                    t.cuda.current_stream().wait_event(ready)
                     slot = free_slots.pop()
                     job.attach(slot, time.perf_counter() - t0)
                     jobs[slot] = job
@@ -433,7 +468,8 @@ class Server:
                 peak_bytes = max(peak_bytes, used_bytes)
                 if not jobs:
                     if preparing:
-                        futures_wait([preparing[0][1]], timeout=0.01)
+                        futures_wait([p[1] for p in preparing], timeout=0.01,
+                                     return_when=FIRST_COMPLETED)
                     elif pending and realtime:
                         time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
                     continue
