@@ -111,6 +111,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather(const BatchSlot *__rest
 // Iteration k of a batch launch sequence runs at batch tick *tick + k (the
 // tick advances by the sequence length at its end, k_batch_tick), so the
 // graph of a sequence replays with no per-launch arguments.
+// Elements per thread the batched CG step loads before its share sums.
+constexpr int kBatchPrefetch = 4;
+
 __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchSlot *__restrict__ slots,
                                                        const int32_t *__restrict__ bmap,
                                                        const int64_t *__restrict__ tick, int k) {
@@ -121,6 +124,33 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchSlot *__restri
   const int lb = blk - S.v_blk0;
   const int q = (int)((*tick + k - S.start_tick) & 1);
   const int iter = st->it[q ^ 1];
+  float *__restrict__ x = S.x;
+  float *__restrict__ r = S.r;
+  float *__restrict__ z = S.z;
+  float *__restrict__ p = S.p;
+  float *__restrict__ s = S.s;
+  const float *__restrict__ w = S.w;
+  const float *__restrict__ dinv = S.dinv;
+  // the first kBatchPrefetch elements of this thread are loaded before the
+  // shares are summed, so their latency overlaps the sums (p and s hold no
+  // values before iteration 0: selected away below, never scaled)
+  const int64_t stride = (int64_t)S.v_blocks * kBlock;
+  const int64_t i0 = (int64_t)lb * kBlock + threadIdx.x;
+  float rz[kBatchPrefetch], rp[kBatchPrefetch], rw[kBatchPrefetch], rs[kBatchPrefetch];
+  float rx[kBatchPrefetch], rv[kBatchPrefetch], rd[kBatchPrefetch];
+#pragma unroll
+  for (int j = 0; j < kBatchPrefetch; ++j) {
+    const int64_t i = i0 + j * stride;
+    if (i < S.n) {
+      rz[j] = z[i];
+      rp[j] = p[i];
+      rw[j] = w[i];
+      rs[j] = s[i];
+      rx[j] = x[i];
+      rv[j] = r[i];
+      rd[j] = dinv[i];
+    }
+  }
   double gamma, rr, delta;
   sum_partials3(q ? S.rz0 : S.rz1, S.v_blocks, q ? S.rr0 : S.rr1, S.v_blocks, S.pap, S.g_blocks,
                 gamma, rr, delta);
@@ -148,15 +178,25 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchSlot *__restri
     st->rr = rr;
   }
   const float a = float(alpha), bt = float(beta);
-  float *__restrict__ x = S.x;
-  float *__restrict__ r = S.r;
-  float *__restrict__ z = S.z;
-  float *__restrict__ p = S.p;
-  float *__restrict__ s = S.s;
-  const float *__restrict__ w = S.w;
-  const float *__restrict__ dinv = S.dinv;
   double sg = 0.0, srr = 0.0;
-  for (int64_t i = (int64_t)lb * kBlock + threadIdx.x; i < S.n; i += (int64_t)S.v_blocks * kBlock) {
+#pragma unroll
+  for (int j = 0; j < kBatchPrefetch; ++j) {
+    const int64_t i = i0 + j * stride;
+    if (i < S.n) {
+      const float pi = first ? rz[j] : rz[j] + bt * rp[j];
+      const float si = first ? rw[j] : rw[j] + bt * rs[j];
+      p[i] = pi;
+      s[i] = si;
+      x[i] = rx[j] + a * pi;
+      const float ri = rv[j] - a * si;
+      r[i] = ri;
+      const float zi = rd[j] * ri;
+      z[i] = zi;
+      sg += (double)ri * (double)zi;
+      srr += (double)ri * (double)ri;
+    }
+  }
+  for (int64_t i = i0 + kBatchPrefetch * stride; i < S.n; i += stride) {
     const float pi = first ? z[i] : z[i] + bt * p[i];
     const float si = first ? w[i] : w[i] + bt * s[i];
     p[i] = pi;
@@ -205,7 +245,7 @@ static SlotLayout slot_layout(const tatva_mesh *m, int ncomp, int64_t n) {
   L.dinv = take(n * sizeof(float));
   L.partial = take((size_t)std::max<int64_t>(1, m->n_slots) * ncomp * sizeof(float));
   L.pap = take(gather_grid(m->n_nodes) * sizeof(double));
-  const size_t vb = vec_grid(n) * sizeof(double);
+  const size_t vb = step_grid(n, kBatchPrefetch) * sizeof(double);
   L.rz0 = take(vb);
   L.rz1 = take(vb);
   L.rr0 = take(vb);
@@ -500,7 +540,7 @@ extern "C" int tatva_batch_start(tatva_batch *B, int32_t slot, tatva_mesh *m,
   S.bb = (double *)(base + L.bb);
   S.st = B->d_states + slot;
   S.g_blocks = gather_grid(m->n_nodes);
-  S.v_blocks = vec_grid(n);
+  S.v_blocks = step_grid(n, kBatchPrefetch);
   S.start_tick = B->tick;
   // r = b - H x, z = D^-1 r, shares of r.z, r.r, b.b (w is scratch here)
   const int rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, S.r, S.z, S.w, S.dinv, S.rz1,

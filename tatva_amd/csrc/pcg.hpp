@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "common.hpp"
@@ -36,6 +37,20 @@ inline int vec_grid(int64_t n) {
   if (b < 1) b = 1;
   if (b > kVecBlocks) b = kVecBlocks;
   return (int)b;
+}
+
+// Grid of a CG step kernel whose threads load `per_thread` elements before
+// summing the shares of the previous step, and of the kernels that write
+// those shares.  Every step block sums all of them, so the grid is no larger
+// than the prefetch needs: enough blocks for the prefetched elements to cover
+// n, at least one block per CU (fewer only when n gives fewer blocks of one
+// element per thread), at most kVecBlocks.
+constexpr int kStepMinBlocks = 256;
+
+inline int step_grid(int64_t n, int per_thread) {
+  const int64_t per_block = (int64_t)kBlock * per_thread;
+  const int64_t lo = std::min<int64_t>(kStepMinBlocks, vec_grid(n));
+  return (int)std::min<int64_t>(kVecBlocks, std::max(lo, (n + per_block - 1) / per_block));
 }
 
 // Matrix-free FP32 PCG set-up on stream s: st reset, dinv = 1/diag,

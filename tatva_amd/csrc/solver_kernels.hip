@@ -415,9 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict
     p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
-// Elements per thread whose loads the CG step issues before its scalar sums
-// (vec_grid caps the grid at 1024 blocks: 4 cover n <= 1,048,576 in one pass,
-// the 8 of the FP32 step n <= 2,097,152).
+// Elements per thread whose loads the CG step issues before its scalar sums.
 template <class T>
 constexpr int step_prefetch() {
   return sizeof(T) == 4 ? 8 : 4;
@@ -1069,7 +1067,7 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
   TATVA_TRY_HIP(hipMemcpyAsync(x, x_io, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipMemcpyAsync(b, b_in, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  const int gv = vec_grid(n);
+  const int gv = step_grid(n, step_prefetch<double>());
   const bool profiling = g_profiling.load() != 0;
   const int nblk = block_cols ? block_size : 1;
   // operator: node-block view when given, else a sliced ELL copy of the CSR
@@ -1232,7 +1230,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   TATVA_TRY_HIP(hipMemcpyAsync(b, b_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipMemcpyAsync(diag, diag_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipMemcpyAsync(u, u_in, n_full * sizeof(float), hipMemcpyDeviceToDevice, s));
-  const int gv = vec_grid(n);
+  const int gv = step_grid(n, step_prefetch<float>());
   const bool profiling = g_profiling.load() != 0;
   // r = b - H x, z = D^-1 r, shares of r.z, r.r, b.b (w is scratch here)
   rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, r, z, w, dinv, P.rz(1), P.rr(1), P.bb,
