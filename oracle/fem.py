@@ -699,12 +699,44 @@ def find_containing_polygons(points, polygons):
     return out
 
 
-def interpolate(elem, coords, elements, nodal, points):
+def find_containing_polygons_batched(points, polygons, chunk=64):
+    """find_containing_polygons for meshes too large for the per-pair loop:
+    the same bounding-box, edge (|cross| <= 1e-8) and ray-parity tests and the
+    same first-polygon rule, evaluated for chunks of points against all
+    polygons at once with numpy broadcasting."""
+    pts = np.asarray(points, dtype=np.float64)
+    V = np.asarray(polygons, dtype=np.float64)
+    W = np.roll(V, -1, axis=1)
+    lo, hi = V.min(axis=1), V.max(axis=1)                     # (E, 2)
+    V0, V1, W0, W1 = V[None, :, :, 0], V[None, :, :, 1], W[None, :, :, 0], W[None, :, :, 1]
+    out = np.full(pts.shape[0], -1, dtype=np.int64)
+    for a in range(0, pts.shape[0], chunk):
+        px = pts[a:a + chunk, 0][:, None]
+        py = pts[a:a + chunk, 1][:, None]
+        inbox = (lo[None, :, 0] <= px) & (px <= hi[None, :, 0]) & \
+            (lo[None, :, 1] <= py) & (py <= hi[None, :, 1])   # (c, E)
+        PX, PY = px[:, :, None], py[:, :, None]
+        cross = (W0 - V0) * (PY - V1) - (W1 - V1) * (PX - V0)
+        on_seg = (np.minimum(V0, W0) <= PX) & (PX <= np.maximum(V0, W0)) & \
+            (np.minimum(V1, W1) <= PY) & (PY <= np.maximum(V1, W1))
+        boundary = np.any((np.abs(cross) <= 1e-8) & on_seg, axis=2)
+        ycond = ((V1 <= PY) & (W1 > PY)) | ((W1 <= PY) & (V1 > PY))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xint = (W0 - V0) * (PY - V1) / (W1 - V1) + V0
+        parity = np.sum(ycond & (PX < xint), axis=2) % 2 == 1
+        inside = inbox & (boundary | parity)
+        out[a:a + chunk] = np.where(inside.any(axis=1), np.argmax(inside, axis=1), -1)
+    return out
+
+
+def interpolate(elem, coords, elements, nodal, points, batched=False):
     """Operator.interpolate (operator.py:399-468): one Newton step from the
-    first quadrature point in the containing element, NaN outside."""
+    first quadrature point in the containing element, NaN outside
+    (``batched``: locate with find_containing_polygons_batched)."""
     X = np.asarray(coords)
     el = np.asarray(elements)
-    idx = find_containing_polygons(points, X[el])
+    locate = find_containing_polygons_batched if batched else find_containing_polygons
+    idx = locate(points, X[el])
     nodal = np.asarray(nodal, dtype=np.float64)
     out = np.full((len(points),) + nodal.shape[1:], np.nan)
     xi0 = elem.quad_points[0]

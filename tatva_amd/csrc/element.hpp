@@ -21,6 +21,25 @@ struct QuadRule {
   double w[kMaxQuad];
 };
 
+// Uniform 2-D grid of point-location cells: n[j] cells of width h[j] from
+// lo[j] along axis j, cell (cx, cy) at index cy * n[0] + cx.
+struct PointGrid {
+  int n[2];
+  double lo[2];
+  double h[2];
+};
+
+// Cell coordinate of v along an axis.  Host and device evaluate the same
+// correctly rounded operations, and the result does not decrease as v grows,
+// so a point inside an element's bounding box [a, b] always falls in a cell
+// between the cells of a and b.  Outside the grid (and NaN) it clamps.
+__host__ __device__ inline int grid_cell(double v, double lo, double h, int n) {
+  double t = (v - lo) / h;
+  t = t >= 0.0 ? t : 0.0;
+  t = t < double(n - 1) ? t : double(n - 1);
+  return (int)floor(t);
+}
+
 template <int KIND>
 struct Elem;
 

@@ -529,7 +529,10 @@ __global__ __launch_bounds__(kBlock) void k_project_rhs(int64_t n_nodes, const d
 // element get NaN and are counted in *outside.
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_interpolate(int64_t n_points, const double *__restrict__ pts,
-                                                        int64_t n_elem, const double *__restrict__ coords,
+                                                        PointGrid grid,
+                                                        const int32_t *__restrict__ cell_ptr,
+                                                        const int32_t *__restrict__ cell_elems,
+                                                        const double *__restrict__ coords,
                                                         const int32_t *__restrict__ conn, QuadRule quad,
                                                         int nc, const double *__restrict__ nodal,
                                                         double *__restrict__ out,
@@ -543,7 +546,14 @@ __global__ __launch_bounds__(kBlock) void k_interpolate(int64_t n_points, const 
   int64_t found = -1;
   double X[NNE][2];
   int32_t nodes[NNE];
-  for (int64_t e = 0; e < n_elem && found < 0; ++e) {
+  // every element holding the point overlaps the point's cell, and the cell
+  // lists its elements ascending: the first hit is the lowest element index,
+  // as a scan over all elements would find
+  const int64_t cell = (int64_t)grid_cell(py, grid.lo[1], grid.h[1], grid.n[1]) * grid.n[0] +
+                       grid_cell(px, grid.lo[0], grid.h[0], grid.n[0]);
+  const int32_t k1 = cell_ptr[cell + 1];
+  for (int32_t k = cell_ptr[cell]; k < k1 && found < 0; ++k) {
+    const int64_t e = cell_elems[k];
     double lo0 = 1e300, lo1 = 1e300, hi0 = -1e300, hi1 = -1e300;
 #pragma unroll
     for (int a = 0; a < NNE; ++a) {
@@ -1073,7 +1083,8 @@ extern "C" int tatva_mesh_destroy(tatva_mesh *m) {
   if (m->partials) (void)hipFree(m->partials);
   for (void *p : {(void *)m->t_lnodes, (void *)m->t_color_off,
                   (void *)m->t_node_ptr, (void *)m->t_slot_ptr, (void *)m->t_slot_pos,
-                  (void *)m->t_tnodes, (void *)m->t_inc_end, (void *)m->t_inc_list})
+                  (void *)m->t_tnodes, (void *)m->t_inc_end, (void *)m->t_inc_list,
+                  (void *)m->loc_ptr, (void *)m->loc_elems})
     if (p) (void)hipFree(p);
   delete m;
   return TATVA_OK;
@@ -1444,6 +1455,94 @@ extern "C" int tatva_project_rhs(tatva_mesh *m, int32_t nc, const double *quad_v
   });
 }
 
+// Point-location grid of a 2-D mesh, built once from the device arrays: about
+// two elements per cell, cells shaped after the bounding box of the nodes;
+// each element is listed in every cell its bounding box overlaps, in
+// ascending element order.
+static int build_point_grid(tatva_mesh *m, hipStream_t s) {
+  if (m->loc_ptr) return TATVA_OK;
+  const int64_t N = m->n_nodes, E = m->n_elements;
+  const int nne = m->nne;
+  std::vector<double> X((size_t)N * 2);
+  std::vector<int32_t> C((size_t)E * nne);
+  hipError_t e = hipMemcpyAsync(X.data(), m->coords, X.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(C.data(), m->conn, C.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    set_error(std::string("point grid: ") + hipGetErrorString(e));
+    return TATVA_EHIP;
+  }
+  double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+  for (int64_t i = 0; i < N; ++i)
+    for (int j = 0; j < 2; ++j) {
+      lo[j] = std::min(lo[j], X[(size_t)i * 2 + j]);
+      hi[j] = std::max(hi[j], X[(size_t)i * 2 + j]);
+    }
+  const double w0 = hi[0] - lo[0], w1 = hi[1] - lo[1];
+  const double cells = std::max<double>(1.0, 0.5 * (double)E);
+  const double aspect = (w0 > 0.0 && w1 > 0.0) ? w0 / w1 : 1.0;
+  PointGrid g{};
+  g.n[0] = (int)std::min(8192.0, std::max(1.0, std::ceil(std::sqrt(cells * aspect))));
+  g.n[1] = (int)std::min(8192.0, std::max(1.0, std::ceil(cells / g.n[0])));
+  g.lo[0] = lo[0];
+  g.lo[1] = lo[1];
+  g.h[0] = w0 > 0.0 ? w0 / g.n[0] : 1.0;
+  g.h[1] = w1 > 0.0 ? w1 / g.n[1] : 1.0;
+  const int64_t n_cells = (int64_t)g.n[0] * g.n[1];
+  std::vector<int64_t> ptr((size_t)n_cells + 1, 0);
+  auto cell_range = [&](int64_t el, int c0[2], int c1[2]) {
+    double a[2] = {1e300, 1e300}, b[2] = {-1e300, -1e300};
+    for (int v = 0; v < nne; ++v)
+      for (int j = 0; j < 2; ++j) {
+        const double x = X[(size_t)C[(size_t)(el * nne + v)] * 2 + j];
+        a[j] = std::min(a[j], x);
+        b[j] = std::max(b[j], x);
+      }
+    for (int j = 0; j < 2; ++j) {
+      c0[j] = grid_cell(a[j], g.lo[j], g.h[j], g.n[j]);
+      c1[j] = grid_cell(b[j], g.lo[j], g.h[j], g.n[j]);
+    }
+  };
+  for (int64_t el = 0; el < E; ++el) {
+    int c0[2], c1[2];
+    cell_range(el, c0, c1);
+    for (int cy = c0[1]; cy <= c1[1]; ++cy)
+      for (int cx = c0[0]; cx <= c1[0]; ++cx) ptr[(size_t)((int64_t)cy * g.n[0] + cx) + 1]++;
+  }
+  for (int64_t c = 0; c < n_cells; ++c) ptr[(size_t)c + 1] += ptr[(size_t)c];
+  if (ptr.back() >= (int64_t(1) << 31)) {
+    set_error("point grid: too many cell entries");
+    return TATVA_EINVAL;
+  }
+  std::vector<int32_t> ptr32(ptr.begin(), ptr.end()), list((size_t)ptr.back());
+  for (int64_t el = 0; el < E; ++el) {
+    int c0[2], c1[2];
+    cell_range(el, c0, c1);
+    for (int cy = c0[1]; cy <= c1[1]; ++cy)
+      for (int cx = c0[0]; cx <= c1[0]; ++cx)
+        list[(size_t)ptr[(size_t)((int64_t)cy * g.n[0] + cx)]++] = (int32_t)el;
+  }
+  int32_t *d_ptr = nullptr, *d_list = nullptr;
+  e = hipMalloc(&d_ptr, ptr32.size() * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&d_list, std::max<size_t>(1, list.size()) * sizeof(int32_t));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_ptr, ptr32.data(), ptr32.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && !list.empty())
+    e = hipMemcpyAsync(d_list, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the host arrays must outlive the copies
+  if (e != hipSuccess) {
+    if (d_ptr) (void)hipFree(d_ptr);
+    if (d_list) (void)hipFree(d_list);
+    set_error(std::string("point grid: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? TATVA_ENOMEM : TATVA_EHIP;
+  }
+  m->loc = g;
+  m->loc_ptr = d_ptr;
+  m->loc_elems = d_list;
+  return TATVA_OK;
+}
+
 extern "C" int tatva_interpolate(tatva_mesh *m, int64_t n_points, const double *points, int32_t nc,
                                  const double *nodal, double *out, int32_t *n_outside,
                                  void *stream) {
@@ -1454,11 +1553,16 @@ extern "C" int tatva_interpolate(tatva_mesh *m, int64_t n_points, const double *
   *n_outside = 0;
   if (n_points == 0) return TATVA_OK;
   hipStream_t s = (hipStream_t)stream;
+  {
+    const int rc = build_point_grid(m, s);
+    if (rc) return rc;
+  }
   int *d_out = reinterpret_cast<int *>(m->partials);  // the mesh's reduction scratch
   hipError_t e = hipMemsetAsync(d_out, 0, sizeof(int), s);
   if (e == hipSuccess) {
     k_interpolate<TATVA_TRI3><<<grid_for(n_points), kBlock, 0, s>>>(
-        n_points, points, m->n_elements, m->coords, m->conn, m->quad, nc, nodal, out, d_out);
+        n_points, points, m->loc, m->loc_ptr, m->loc_elems, m->coords, m->conn, m->quad, nc, nodal,
+        out, d_out);
     e = hipGetLastError();
   }
   int h = 0;

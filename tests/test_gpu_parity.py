@@ -585,3 +585,32 @@ def test_interpolate_matches_oracle():
                                atol=1e-12)
     with pytest.raises(RuntimeError):
         op.interpolate(nodal, np.array([[1.2, 0.5]]))
+
+
+def test_interpolate_grid_location_on_a_graded_mesh():
+    """Point location through the mesh's cell grid on a 3 x 1 Tri3 mesh with
+    graded (x -> 3 (x/3)^1.7) and jittered nodes, so elements differ in size
+    and many span several cells: random points, every vertex and edge
+    midpoints (shared by several triangles) give the oracle's values, found by
+    the first-element rule of operator.py:399-468 over all elements."""
+    m0 = ta.Mesh.rectangle((0.0, 3.0), (0.0, 1.0), 60, 24)
+    X = m0.coords.copy()
+    X[:, 0] = 3.0 * (X[:, 0] / 3.0) ** 1.7
+    rng = np.random.default_rng(12)
+    inner = ~np.isclose(m0.coords[:, 0], 0.0) & ~np.isclose(m0.coords[:, 0], 3.0) & \
+        ~np.isclose(m0.coords[:, 1], 0.0) & ~np.isclose(m0.coords[:, 1], 1.0)
+    X[:, 1] += inner * rng.uniform(-0.25, 0.25, X.shape[0]) / 24
+    m = ta.Mesh(X, m0.elements)
+    op = ta.Operator(m, ta.Tri3())
+    mids = 0.5 * (X[m.elements[::2, 0]] + X[m.elements[::2, 2]])
+    pts = np.vstack([rng.uniform([0.0, 0.0], [3.0, 1.0], (2500, 2)), X, mids])
+    nodal = rng.standard_normal((m.n_nodes, 2))
+    got = to_np(op.interpolate(nodal, pts))
+    want = fem.interpolate(fem.Tri3(), X, m.elements, nodal, pts, batched=True)
+    assert np.isfinite(want).all()
+    assert rel_err(got, want) < 1e-10
+    lin = 0.5 + X @ np.array([1.5, -2.0])
+    np.testing.assert_allclose(to_np(op.interpolate(lin, pts)), 0.5 + pts @ np.array([1.5, -2.0]),
+                               atol=1e-11)
+    with pytest.raises(RuntimeError):
+        op.interpolate(nodal, np.vstack([pts[:10], [[3.2, 0.5]], [[1.0, -0.1]]]))

@@ -489,3 +489,19 @@ def test_interpolate_oracle_linear_fields_and_location():
     np.testing.assert_array_equal(idx, [0, 1])
     out = fem.interpolate(el, X, E, nodal, np.array([[1.5, 0.5]]))
     assert np.isnan(out).all()
+
+
+def test_batched_polygon_location_equals_the_loop():
+    """find_containing_polygons_batched returns the same (first) polygon as the
+    per-pair loop, for interior, vertex, edge-midpoint and outside points of a
+    jittered mesh whose vertices and edges are shared by several triangles."""
+    X, E = fem.unit_square_tri3(7, 5)
+    rng = np.random.default_rng(11)
+    inner = ~np.isclose(X, 0.0).any(axis=1) & ~np.isclose(X, 1.0).any(axis=1)
+    X = X + inner[:, None] * rng.uniform(-0.04, 0.04, X.shape)
+    mids = 0.5 * (X[E[:, 0]] + X[E[:, 1]])
+    pts = np.vstack([rng.uniform(-0.05, 1.05, (120, 2)), X, mids[::3], [[0.3, 1.0], [1.0, 1.0]]])
+    want = fem.find_containing_polygons(pts, X[E])
+    got = fem.find_containing_polygons_batched(pts, X[E], chunk=17)
+    np.testing.assert_array_equal(got, want)
+    assert (want == -1).any() and (want >= 0).sum() > 150
