@@ -190,7 +190,11 @@ int tatva_project_rhs(tatva_mesh *mesh, int32_t nc, const double *quad_values, d
  * interpolated at points (n_points x 2); the containing element is the first
  * whose vertex polygon holds the point (edges included), mapped to reference
  * coordinates by one Newton step from the first quadrature point.  Points in no
- * element get NaN; their number is returned in *n_outside (synchronises stream). */
+ * element get NaN; their number is returned in *n_outside (synchronises stream).
+ * The first call on a mesh builds its point-location grid (elements binned by
+ * bounding box, kept with the mesh until tatva_mesh_destroy); each point then
+ * tests only the elements of its grid cell, in ascending order, which finds
+ * the same element as the reference's scan over all of them. */
 int tatva_interpolate(tatva_mesh *mesh, int64_t n_points, const double *points, int32_t nc,
                       const double *nodal, double *out, int32_t *n_outside, void *stream);
 
