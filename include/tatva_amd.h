@@ -245,6 +245,21 @@ int tatva_pcg_matfree_f32(tatva_mesh *mesh, const tatva_density *density, const 
                           float *x, double tol, int32_t maxiter, int32_t chunk, void *workspace,
                           int64_t workspace_bytes, tatva_pcg_stats *stats, void *stream);
 
+/* The same solve when constraints tie several full dofs to one reduced dof
+ * (Periodic, lifter/constraints.py:184-220): the operator is P^T H(u) P with P
+ * the linear part of the lift (Lifter.lift_from_zeros, lifter/base.py:201-229)
+ * and P^T its adjoint (Lifter.reduce_adjoint, lifter/base.py:235-251), as the
+ * reference's jax.jvp of the reduced residual computes it.  read_map gives each
+ * full dof the reduced index it reads (-1: constraint value); rows_ptr/rows_list
+ * (n + 1, CSR) list the full dofs summed into each reduced index, ascending. */
+int64_t tatva_pcg_matfree_mapped_workspace(tatva_mesh *mesh, int64_t n);
+int tatva_pcg_matfree_f32_mapped(tatva_mesh *mesh, const tatva_density *density, const float *u,
+                                 const int32_t *read_map, const int32_t *rows_ptr,
+                                 const int32_t *rows_list, int64_t n, const float *diag,
+                                 const float *b, float *x, double tol, int32_t maxiter,
+                                 int32_t chunk, void *workspace, int64_t workspace_bytes,
+                                 tatva_pcg_stats *stats, void *stream);
+
 /* ------------------------------------------------------- batched PCG ------ */
 
 /* A batch runs many matrix-free FP32 Jacobi-PCG solves (each with its own

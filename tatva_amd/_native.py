@@ -84,6 +84,11 @@ _SIGS = {
     "tatva_pcg_matfree_f32": (c_int, [c_void, ctypes.POINTER(Density), c_void, c_void, c_i64, c_void,
                                       c_void, c_void, c_dbl, c_i32, c_i32, c_void, c_i64,
                                       ctypes.POINTER(PcgStats), c_void]),
+    "tatva_pcg_matfree_mapped_workspace": (c_i64, [c_void, c_i64]),
+    "tatva_pcg_matfree_f32_mapped": (c_int, [c_void, ctypes.POINTER(Density), c_void, c_void,
+                                             c_void, c_void, c_i64, c_void, c_void, c_void, c_dbl,
+                                             c_i32, c_i32, c_void, c_i64,
+                                             ctypes.POINTER(PcgStats), c_void]),
     "tatva_batch_create": (c_int, [c_int, c_i32, ctypes.POINTER(c_void)]),
     "tatva_batch_destroy": (c_int, [c_void]),
     "tatva_batch_slot_workspace": (c_i64, [c_void, c_i32, c_i64]),

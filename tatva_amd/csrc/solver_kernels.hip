@@ -1272,6 +1272,161 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   return rc;
 }
 
+// ------------------------------------- matrix-free PCG through a lift map ----
+//
+// Constraints that tie several full dofs to one reduced dof (Periodic) make the
+// reduced operator P^T H P with P the linear part of the lift: the reduced
+// vector is spread to full dofs through the lift's read map, H runs on the
+// full dofs, and each reduced entry sums the rows that read it.
+
+namespace tatva {
+
+// out[j] = in[map[j]], 0 where map[j] < 0 (constraint values)
+__global__ __launch_bounds__(kBlock) void k_lift_map_f32(int64_t n, const int32_t *__restrict__ map,
+                                                         const float *__restrict__ in,
+                                                         float *__restrict__ out,
+                                                         const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int32_t r = map[i];
+    out[i] = r >= 0 ? in[r] : 0.0f;
+  }
+}
+
+// w[j] = sum of y[list[k]] over k in [ptr[j], ptr[j+1]) in ascending k; with
+// part, the block's share of z.w (FP32 products summed in FP64).
+__global__ __launch_bounds__(kBlock) void k_reduce_rows_dot_f32(int64_t n, const int32_t *__restrict__ ptr,
+                                                                const int32_t *__restrict__ list,
+                                                                const float *__restrict__ y,
+                                                                const float *__restrict__ z,
+                                                                float *__restrict__ w,
+                                                                double *__restrict__ part,
+                                                                const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  double dot = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    float sum = 0.0f;
+    for (int32_t k = ptr[j]; k < ptr[j + 1]; ++k) sum += y[list[k]];
+    w[j] = sum;
+    dot += (double)z[j] * (double)sum;
+  }
+  if (part) {
+    dot = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = dot;
+  }
+}
+
+}  // namespace tatva
+
+extern "C" int64_t tatva_pcg_matfree_mapped_workspace(tatva_mesh *m, int64_t n) {
+  if (!m || n <= 0) return -1;
+  const int64_t n_full = m->n_nodes * m->dim;
+  return (int64_t)(matfree_u_offset(n) + 3 * align256((size_t)n_full * sizeof(float)));
+}
+
+extern "C" int tatva_pcg_matfree_f32_mapped(tatva_mesh *m, const tatva_density *d, const float *u_in,
+                                            const int32_t *read_map, const int32_t *rows_ptr,
+                                            const int32_t *rows_list, int64_t n,
+                                            const float *diag_in, const float *b_in, float *x_io,
+                                            double tol, int32_t maxiter, int32_t chunk,
+                                            void *workspace, int64_t ws_bytes,
+                                            tatva_pcg_stats *stats, void *stream) {
+  TATVA_REQUIRE(m && d && u_in && read_map && rows_ptr && rows_list && diag_in && b_in && x_io &&
+                    n > 0,
+                "invalid arguments");
+  TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
+  const WsLayout L = ws_layout(n, kMatfreeVecs, sizeof(float));
+  TATVA_REQUIRE(workspace && ws_bytes >= tatva_pcg_matfree_mapped_workspace(m, n),
+                "PCG workspace too small");
+  const int64_t n_full = m->n_nodes * density_ncomp(m, d);
+  chunk += chunk & 1;
+  TATVA_TRY_HIP(hipSetDevice(m->device));
+  DeviceRes *res = nullptr;
+  int rc = get_res(m->device, &res);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(res->mu);
+  char *wsb = (char *)workspace;
+  float *z = ws_vec<float>(wsb, L, 0);
+  float *w = ws_vec<float>(wsb, L, 1);
+  float *p = ws_vec<float>(wsb, L, 2);
+  float *sv = ws_vec<float>(wsb, L, 3);
+  float *r = ws_vec<float>(wsb, L, 4);
+  float *dinv = ws_vec<float>(wsb, L, 5);
+  float *x = ws_vec<float>(wsb, L, 6);
+  float *b = ws_vec<float>(wsb, L, 7);
+  float *diag = ws_vec<float>(wsb, L, 8);
+  const size_t full_bytes = align256((size_t)m->n_nodes * m->dim * sizeof(float));
+  float *u = reinterpret_cast<float *>(wsb + matfree_u_offset(n));
+  float *v_full = reinterpret_cast<float *>(wsb + matfree_u_offset(n) + full_bytes);
+  float *y_full = reinterpret_cast<float *>(wsb + matfree_u_offset(n) + 2 * full_bytes);
+  const PcgParts P = parts_at(wsb, L);
+  PcgState *st = ws_state(wsb, L);
+  hipStream_t caller = (hipStream_t)stream;
+  hipStream_t s = res->cap;
+  TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
+  TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
+  TATVA_TRY_HIP(hipMemcpyAsync(x, x_io, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  TATVA_TRY_HIP(hipMemcpyAsync(b, b_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  TATVA_TRY_HIP(hipMemcpyAsync(diag, diag_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  TATVA_TRY_HIP(hipMemcpyAsync(u, u_in, n_full * sizeof(float), hipMemcpyDeviceToDevice, s));
+  const int gv = step_grid(n, step_prefetch<float>());
+  const int g_red = vec_grid(n);
+  const bool profiling = g_profiling.load() != 0;
+  // y = P^T H(u) P v, with the shares of v.y when part is given
+  const auto mapped_op = [&](const float *v, float *y, double *part, const int *skip) -> int {
+    k_lift_map_f32<<<grid_for(n_full), kBlock, 0, s>>>(n_full, read_map, v, v_full, skip);
+    const int e = launch_hvp_f32(m, d, u, v_full, nullptr, y_full, s, skip);
+    if (e) return e;
+    k_reduce_rows_dot_f32<<<g_red, kBlock, 0, s>>>(n, rows_ptr, rows_list, y_full, v, y, part, skip);
+    TATVA_CHECK_LAUNCH();
+    return TATVA_OK;
+  };
+  // r = b - H x, z = D^-1 r, shares of r.z, r.r, b.b (w is scratch here)
+  k_state_reset<<<1, 64, 0, s>>>(st);
+  k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
+  rc = mapped_op(x, w, nullptr, nullptr);
+  if (rc) return rc;
+  k_sub<float><<<grid_for(n), kBlock, 0, s>>>(n, b, w, r);
+  k_pcg_init<float><<<gv, kBlock, 0, s>>>(n, r, dinv, b, z, P.rz(1), P.rr(1), P.bb);
+  k_pcg_init_state<<<1, kBlock, 0, s>>>(P.rr(1), P.bb, gv, tol, maxiter, st);
+  TATVA_CHECK_LAUNCH();
+  // KA: w = P^T H P z with the shares of z.w
+  const std::function<int()> apply_op = [&]() -> int { return mapped_op(z, w, P.pap, &st->done); };
+  const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
+    if (with_op) {
+      const int e = apply_op();
+      if (e) return e;
+    }
+    k_cg_step<float><<<gv, kBlock, 0, s>>>(n, x, r, z, p, sv, w, dinv, P.pap, g_red, P.rz(q ^ 1),
+                                           P.rr(q ^ 1), P.rz(q), P.rr(q), q, st);
+    return TATVA_OK;
+  };
+  GraphKey key{};
+  key.ws = workspace;
+  key.a = m;
+  key.b = read_map;
+  key.c = rows_ptr;
+  key.d = rows_list;
+  key.n = n;
+  key.m = n_full;
+  key.i0 = d->kind;
+  key.i1 = chunk;
+  key.i2 = profiling ? 1 : 0;
+  key.f0 = d->p0;
+  key.f1 = d->p1;
+  key.f2 = d->p2;
+  rc = chunk_graph(s, chunk, step, profiling, key, res->matfree);
+  if (rc == TATVA_OK)
+    rc = run_chunks(res, res->matfree.exec, st, maxiter, chunk, profiling, apply_op, stats);
+  if (rc == TATVA_OK)
+    TATVA_TRY_HIP(hipMemcpyAsync(x_io, x, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
+  TATVA_TRY_HIP(hipStreamWaitEvent(caller, res->ev_out, 0));
+  return rc;
+}
+
 // ------------------------------------------------------- vector kernels ----
 
 namespace tatva {

@@ -105,18 +105,33 @@ def pcg_matfree(energy: ReducedEnergy, u_full32, b32, x0=None, *, diag32=None, t
     x = t.zeros(n, dtype=t.float32, device=b.device) if x0 is None else \
         nat.to_device(x0, t.float32).reshape(-1).clone()
     u = nat.to_device(u_full32, t.float32).reshape(-1).contiguous()
+    lf = energy.lifter
     if diag32 is None:
-        diag32 = energy.energy.hessian_diag(u, dof_map=energy.lifter.dof_map_device(), n_out=n,
-                                            dtype=t.float32)
+        if lf.has_periodic:
+            diag32 = lf.reduce_adjoint(energy.energy.hessian_diag(u)).to(t.float32)
+        else:
+            diag32 = energy.energy.hessian_diag(u, dof_map=lf.dof_map_device(), n_out=n,
+                                                dtype=t.float32)
     maxiter = 10 * n if maxiter is None else int(maxiter)
     L = nat.lib()
     op = energy.op
-    nbytes = L.tatva_pcg_matfree_workspace(op.handle, n)
-    ws = (workspace or _Workspace()).get(nbytes)
     st = nat.PcgStats()
     desc = energy.energy._desc
+    if lf.has_periodic:
+        # P^T H P: spread through the lift's read map, sum the rows of each reduced dof
+        nbytes = L.tatva_pcg_matfree_mapped_workspace(op.handle, n)
+        ws = (workspace or _Workspace()).get(nbytes)
+        ptr, lst = lf.reduce_rows_device()
+        nat.check(L.tatva_pcg_matfree_f32_mapped(
+            op.handle, ctypes.byref(desc), nat.dptr(u), nat.dptr(lf.read_map_device()),
+            nat.dptr(ptr), nat.dptr(lst), n, nat.dptr(diag32), nat.dptr(b), nat.dptr(x),
+            float(tol), maxiter, int(chunk), nat.dptr(ws), ws.numel(), ctypes.byref(st),
+            nat.stream()))
+        return x, _stats_to_info(st)
+    nbytes = L.tatva_pcg_matfree_workspace(op.handle, n)
+    ws = (workspace or _Workspace()).get(nbytes)
     nat.check(L.tatva_pcg_matfree_f32(op.handle, ctypes.byref(desc), nat.dptr(u),
-                                      nat.dptr(energy.lifter.dof_map_device()), n,
+                                      nat.dptr(lf.dof_map_device()), n,
                                       nat.dptr(diag32), nat.dptr(b), nat.dptr(x), float(tol),
                                       maxiter, int(chunk), nat.dptr(ws), ws.numel(),
                                       ctypes.byref(st), nat.stream()))
@@ -192,8 +207,6 @@ class Newton:
         if lifter is not None:
             self.red.lifter = lifter
         lf = self.red.lifter
-        if self.mode == "mixed" and lf.has_periodic:
-            raise NotImplementedError("mode='mixed' supports Fixed constraints only")
         lf.apply_constraints(u_full)
         dof_map = lf.dof_map_device()
         free = lf.free_device()
@@ -255,6 +268,8 @@ class Newton:
                     break
                 nat.check(L.tatva_axpy_free_f32(n, nat.dptr(free), 1.0, nat.dptr(du),
                                                 nat.dptr(u_full), nat.stream()))
+                if lf.has_periodic:
+                    lf.apply_constraints(u_full)  # slaves follow their masters
         t.cuda.synchronize()
         stats.seconds = time.perf_counter() - t0
         return stats

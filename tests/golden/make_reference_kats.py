@@ -22,6 +22,20 @@ KATS = {
         "quad_value": 4.0,
         "quad_integral_expected": [2.0, 2.0],
     },
+    "interpolate_two_triangles": {
+        "source": "tests/test_operator.py:146-167 (on the mesh of :14-32)",
+        "nodal": "sum of the node coordinates",
+        "points": [[0.25, 0.25], [0.75, 0.25], [0.25, 0.75], [0.5, 0.5]],
+        "expected": [0.5, 1.0, 1.0, 1.0],
+        "outside_points": [[1.5, 0.5]],
+    },
+    "containing_polygons_boundary": {
+        "source": "tests/test_mesh.py:7-24",
+        "polygons": [[[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]],
+                     [[1.0, 0.0], [2.0, 0.0], [2.0, 1.0], [1.0, 1.0]]],
+        "points": [[0.5, 0.5], [1.0, 0.5], [1.5, 0.5]],
+        "expected": [0, 0, 1],
+    },
     "lifter_fixed_periodic": {
         "source": "tests/test_lifter.py:26-41",
         "size": 6,

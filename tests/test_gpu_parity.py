@@ -503,6 +503,28 @@ def test_periodic_residual_hvp_hessian_newton(density):
     assert np.abs(un - uo).max() <= 1e-8 * np.abs(uo).max()
 
 
+@pytest.mark.parametrize("density", [energies.LinearElasticity(0.4, 0.6),
+                                     energies.NeoHookean(1.0, 1.0)],
+                         ids=["LinearElasticity", "NeoHookean"])
+def test_periodic_mixed_newton_matches_oracle(density):
+    """Mixed Newton (FP32 matrix-free PCG on P^T H P, FP64 residual) under
+    Periodic + Fixed constraints: the FP32 reduced operator applied as a whole
+    (tatva_pcg_matfree_f32_mapped) reaches the oracle's Newton solution, and
+    every slave equals its master."""
+    m, L, Lo, E, prob, slaves, masters = _periodic_case(6, 5, density)
+    S = L.adapt_sparsity(sparse.pattern_from_mesh(m, 2))
+    cm = sparse.ColoredMatrix.from_csr(S)
+    So = Lo.adapt_sparsity(fem.pattern_from_mesh(m.elements, m.n_nodes, 2))
+    uo, *_ = fem.newton(prob, Lo, Lo.lift_from_zeros(np.zeros(Lo.size_reduced)), So,
+                        to_np(cm.colors), tol=1e-10, lin_tol=1e-12)
+    u = t_zeros(m.n_nodes * 2)
+    st = solvers.Newton(E, L, mode="mixed", tol=1e-10).solve(u)
+    assert st.converged and st.cg_iterations > 0
+    un = to_np(u)
+    np.testing.assert_array_equal(un[slaves], un[masters])
+    assert np.abs(un - uo).max() <= 1e-7 * np.abs(uo).max()
+
+
 def test_periodic_hessian_when_slave_and_master_share_elements():
     """One cell across: every element holds a slave and its master (the serial
     accumulation path of the mapped colored Hessian)."""
@@ -585,6 +607,19 @@ def test_interpolate_matches_oracle():
                                atol=1e-12)
     with pytest.raises(RuntimeError):
         op.interpolate(nodal, np.array([[1.2, 0.5]]))
+
+
+def test_interpolate_reference_kats():
+    """tests/test_operator.py:146-167: the sum of the coordinates interpolated at
+    four points of the two-triangle mesh (one on the shared diagonal); a point
+    outside raises RuntimeError."""
+    k, m = GOLDEN["interpolate_two_triangles"], GOLDEN["operator_two_triangles"]
+    X = np.array(m["nodes"])
+    op = ta.Operator(ta.Mesh(X, np.array(m["elements"])), ta.Tri3())
+    got = to_np(op.interpolate(X.sum(axis=1), np.array(k["points"])))
+    np.testing.assert_allclose(got, k["expected"], rtol=1e-12)
+    with pytest.raises(RuntimeError, match="outside the mesh"):
+        op.interpolate(np.ones(X.shape[0]), np.array(k["outside_points"]))
 
 
 def test_interpolate_grid_location_on_a_graded_mesh():

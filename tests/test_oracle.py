@@ -491,6 +491,15 @@ def test_interpolate_oracle_linear_fields_and_location():
     assert np.isnan(out).all()
 
 
+def test_containing_polygons_boundary_kat():
+    """tests/test_mesh.py:7-24: a point on the edge two squares share belongs to
+    the first of them."""
+    k = GOLDEN["containing_polygons_boundary"]
+    polys, pts = np.array(k["polygons"]), np.array(k["points"])
+    np.testing.assert_array_equal(fem.find_containing_polygons(pts, polys), k["expected"])
+    np.testing.assert_array_equal(fem.find_containing_polygons_batched(pts, polys), k["expected"])
+
+
 def test_batched_polygon_location_equals_the_loop():
     """find_containing_polygons_batched returns the same (first) polygon as the
     per-pair loop, for interior, vertex, edge-midpoint and outside points of a
