@@ -12,7 +12,7 @@ timed, as in the reference's setup/compile/run split
 
 metric value: Newton DOF-iterations/s = n_free * (CG iterations) / wall seconds
 of the timed steps.  The roofline object describes the dominant kernel (the
-fused CSR SpMV + p.Ap of the PCG), timed with HIP events on its own stream.
+node-block CSR SpMV of the PCG), timed with HIP events on its own stream.
 """
 
 from __future__ import annotations
@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the node-block SpMV on the 64^3 line, from the two
 # rocprofv3 PMC passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
 # plain 512 MiB copy) recorded in profiles/k_pcg_bspmv_hex8_le_64cubed.md.
-SPMV_TRAFFIC_64 = {"fetch_kib": 273681.4, "fetch_factor": 2.000, "write_kib": 7401.9,
+SPMV_TRAFFIC_64 = {"fetch_kib": 273681.8, "fetch_factor": 2.000, "write_kib": 6347.5,
                    "write_factor": 1.000}
 
 
@@ -433,7 +433,7 @@ def main(argv=None):
     if blocked:
         # node-block view: 8 B per value, one int32 per 3x3 block, two indptr
         # reads per node row, p read once and Ap written once
-        kernel = "k_pcg_bspmv<3> (node-block CSR SpMV fused with p.Ap)"
+        kernel = "k_pcg_bspmv<3> (node-block CSR SpMV; p.Ap shares by k_dot_shares)"
         spmv_bytes = 8 * nnz + 4 * (nnz // 9) + 8 * (n_free // 3) + 16 * n_free
     else:
         kernel = "k_sell_spmv (sliced ELL copy of the CSR, fused with p.Ap)"

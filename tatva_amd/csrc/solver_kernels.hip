@@ -8,9 +8,9 @@
 //   p_i = u_i + beta_i p_{i-1}, s_i = w_i + beta_i s_{i-1}  (s_i = A p_i),
 //   x_{i+1} = x_i + alpha_i p_i, r_{i+1} = r_i - alpha_i s_i,
 // which equals standard PCG in exact arithmetic and needs one cross-block sum
-// per iteration.  One iteration i = two kernels (three with the matrix-free
-// operator):
-//   KA  w = A u; each block writes its share of w.u            -> part_delta
+// per iteration.  One iteration i = two or three kernels:
+//   KA  w = A u and the blocks' shares of w.u (written by the operator's last
+//       kernel, or by k_dot_shares after the node-block SpMV)  -> part_delta
 //   KB  every block re-sums part_delta and the shares of r.u, r.r of the
 //       previous KB (fixed order) to get the scalars, tests convergence on
 //       r_i, updates p, s, x, r, u and writes the shares of r.u and r.r of
@@ -22,7 +22,7 @@
 // by the other parity, so no block reads a value block 0 of the same kernel
 // writes.  Each kernel returns at once when the state says done, so a chunk
 // may overrun convergence harmlessly.  The batched solver (batch_kernels.hip)
-// keeps the three-kernel standard form (update, then p update).
+// runs the same form for every slot of a launch.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -141,38 +141,44 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(int64_t nbr, const int32_t *__
   }
 }
 
-// K1 (node-block view): Ap = A p and the block's share of p.Ap.
+// K1 (node-block view): Ap = A p.  The shares of p.Ap come from k_dot_shares:
+// folding the dot into this kernel cost 7-9 us per launch at 64^3 Hex8 (every
+// variant measured), more than a separate pass over p and Ap.
 template <int B>
 __global__ __launch_bounds__(kBlock) void k_pcg_bspmv(int64_t nbr, const int32_t *__restrict__ indptr,
                                                       const int32_t *__restrict__ bcol,
                                                       const double *__restrict__ data,
                                                       const double *__restrict__ p,
                                                       double *__restrict__ Ap,
-                                                      double *__restrict__ part_pap,
                                                       const PcgState *__restrict__ st) {
   if (st->done) return;
   constexpr int G = kBlock / kBLanes;
   const int lane = threadIdx.x % kBLanes;
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const int64_t stride = (int64_t)gridDim.x * G;
-  double dot = 0.0;
   for (int64_t I = lb * G + threadIdx.x / kBLanes; I < nbr; I += stride) {
-    // the row's own p entries, loaded with the row so the dot needs no extra round trip
-    double pv[B];
-#pragma unroll
-    for (int c = 0; c < B; ++c) pv[c] = p[B * I + c];  // same address in every lane
     double acc[B];
     bcsr_block_row<B>(I, lane, indptr, bcol, data, p, acc);
     if (lane == 0) {
 #pragma unroll
-      for (int c = 0; c < B; ++c) {
-        Ap[B * I + c] = acc[c];
-        dot += acc[c] * pv[c];
-      }
+      for (int c = 0; c < B; ++c) Ap[B * I + c] = acc[c];
     }
   }
-  dot = block_sum(dot);
-  if (threadIdx.x == 0) part_pap[blockIdx.x] = dot;
+}
+
+// Shares of p.Ap for the CG step: block k sums its grid-stride range of
+// products in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_dot_shares(int64_t n, const double *__restrict__ p,
+                                                       const double *__restrict__ Ap,
+                                                       double *__restrict__ part,
+                                                       const PcgState *__restrict__ st) {
+  if (st->done) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc = 0.0;
+#pragma unroll 4
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) acc += p[i] * Ap[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 // Derives the block column list; flags *bad if the pattern is not made of
@@ -1082,7 +1088,9 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   const bool big = g_op > kMaxReduceBlocks;
   TATVA_REQUIRE(g_op <= csr_op_parts(n), "SpMV grid exceeds the workspace shares");
   double *op_part = big ? reinterpret_cast<double *>(wsb + L.op_parts) : P.pap;
-  const int n_delta = big ? kShareBlocks : g_op;
+  // node-block operator: its shares come from k_dot_shares over g_dot blocks
+  const int g_dot = step_grid(n, 4);
+  const int n_delta = nblk > 1 ? g_dot : big ? kShareBlocks : g_op;
   auto apply_op = [&](const double *xin, const double *bin, double *yout) -> int {
     if (nblk == 3)
       k_bspmv<3><<<g_op, kBlock, 0, s>>>(n / 3, indptr, block_cols, data, xin, bin, yout);
@@ -1094,12 +1102,13 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   };
-  // KA: w = A u and the shares of w.u (summed to P.pap[0] when big)
+  // KA: w = A u and the shares of w.u (node blocks: by k_dot_shares; sliced
+  // ELL: fused, summed to kShareBlocks partials when big)
   auto spmv_dot = [&]() -> int {
     if (nblk == 3)
-      k_pcg_bspmv<3><<<g_op, kBlock, 0, s>>>(n / 3, indptr, block_cols, data, u, w, op_part, st);
+      k_pcg_bspmv<3><<<g_op, kBlock, 0, s>>>(n / 3, indptr, block_cols, data, u, w, st);
     else if (nblk == 2)
-      k_pcg_bspmv<2><<<g_op, kBlock, 0, s>>>(n / 2, indptr, block_cols, data, u, w, op_part, st);
+      k_pcg_bspmv<2><<<g_op, kBlock, 0, s>>>(n / 2, indptr, block_cols, data, u, w, st);
     else
       k_sell_spmv<<<g_op, kBlock, 0, s>>>(n, sv_ell.nsl, sv_ell.soff, sv_ell.cols, sv_ell.vals, u,
                                           nullptr, w, op_part, st);
@@ -1107,7 +1116,10 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
     return TATVA_OK;
   };
   const std::function<int()> sum_shares = [&]() -> int {
-    if (big) k_sum_shares<<<kShareBlocks, kBlock, 0, s>>>(op_part, g_op, P.pap, st);
+    if (nblk > 1)
+      k_dot_shares<<<g_dot, kBlock, 0, s>>>(n, u, w, P.pap, st);
+    else if (big)
+      k_sum_shares<<<kShareBlocks, kBlock, 0, s>>>(op_part, g_op, P.pap, st);
     TATVA_CHECK_LAUNCH();
     return TATVA_OK;
   };
