@@ -385,8 +385,9 @@ class Server:
         self.max_newton = int(max_newton)
         self.maxiter_factor = int(maxiter_factor)
         self.keep_solutions = keep_solutions
+        self._device = nat.device()
         h = ctypes.c_void_p()
-        nat.check(nat.lib().tatva_batch_create(nat.device().index or 0, self.max_slots,
+        nat.check(nat.lib().tatva_batch_create(self._device.index or 0, self.max_slots,
                                                ctypes.byref(h)))
         self._batch = h.value
         self._stats = (nat.PcgStats * self.max_slots)()
@@ -395,15 +396,21 @@ class Server:
 
     def _prepare(self, req: SolveRequest):
         """Worker thread: the request's _Job, built on the thread's own stream,
-        and an event the serving stream waits for before using it."""
+        and an event the serving stream waits for before using it.  (A new
+        thread starts on device 0, so the server's device is entered first.
+        A job's tensors are freed only after the serving stream has been
+        synchronised past its last use (the batch poll or its last residual
+        norm), so their reuse on this stream cannot overtake work queued
+        there.)"""
         t = nat.torch()
-        st = getattr(self._local, "stream", None)
-        if st is None:
-            st = self._local.stream = t.cuda.Stream(device=nat.device())
-        with t.cuda.stream(st):
-            job = _Job(req, self)
-            ready = t.cuda.Event()
-            ready.record(st)
+        with t.cuda.device(self._device):
+            st = getattr(self._local, "stream", None)
+            if st is None:
+                st = self._local.stream = t.cuda.Stream(device=self._device)
+            with t.cuda.stream(st):
+                job = _Job(req, self)
+                ready = t.cuda.Event()
+                ready.record(st)
         return job, ready
 
     def close(self):
