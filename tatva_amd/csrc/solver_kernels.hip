@@ -146,8 +146,8 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(int64_t nbr, const int32_t *__
 // variant measured), more than a separate pass over p and Ap.  With rev the
 // blocks take the node rows from the end: alternating the direction between
 // iterations, each pass starts on the matrix lines the previous pass read
-// last, which the 256 MB last-level cache still holds (64^3 Hex8: 96 instead
-// of 108 us per launch).
+// last, which the 256 MB last-level cache still holds (64^3 Hex8, in the
+// solve: 91 instead of 108 us per launch).
 template <int B>
 __global__ __launch_bounds__(kBlock) void k_pcg_bspmv(int64_t nbr, const int32_t *__restrict__ indptr,
                                                       const int32_t *__restrict__ bcol,
@@ -437,7 +437,7 @@ constexpr int step_prefetch() {
 // KB of the single-reduction form (see the file comment).  u is D^-1 r.  In
 // FP64, r is not stored: u_{i+1} = u_i - alpha D^-1 s_i, and r_{i+1} =
 // u_{i+1} / D^-1 is formed for the shares of r.u and r.r (ten vector streams
-// per element instead of twelve; same iteration counts).  In FP32 that
+// per element instead of twelve; iteration counts within 1 %).  In FP32 that
 // recurrence took 6-8 % more iterations, so r is kept there (r unused and
 // may be NULL in FP64).
 template <class T>
