@@ -567,11 +567,16 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
                                                          const T *__restrict__ v,
                                                          const int32_t *__restrict__ dof_map,
                                                          TileSet ts, T *__restrict__ partial,
-                                                         const int *__restrict__ skip) {
+                                                         const int *__restrict__ skip, int rev) {
   if (skip && *skip) return;
   extern __shared__ unsigned char lds_raw[];
-  tile_assemble<KIND, Dens, T, MODE, AFFINE>((int)xcd_block(blockIdx.x, gridDim.x), coords, quad, dens,
-                                             u, v, dof_map, ts, partial, lds_raw);
+  // rev: tiles from the end (a solver alternates the order between iterations
+  // so that each pass starts on the data the last-level cache kept from the
+  // previous one; every tile writes its own slots, so the result is the same)
+  int tile = (int)xcd_block(blockIdx.x, gridDim.x);
+  if (rev) tile = (int)gridDim.x - 1 - tile;
+  tile_assemble<KIND, Dens, T, MODE, AFFINE>(tile, coords, quad, dens, u, v, dof_map, ts, partial,
+                                             lds_raw);
 }
 
 // Sum of node i's tile slots (ascending tile): the tiles wrote them to the

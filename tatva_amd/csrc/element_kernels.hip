@@ -1218,7 +1218,8 @@ namespace tatva {
 template <class T, int MODE>
 static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, const T *v,
                            const int32_t *dof_map, const double *f_ext, T *out, hipStream_t s,
-                           const int *skip, double *part_dot = nullptr, int gather_blocks = 0) {
+                           const int *skip, double *part_dot = nullptr, int gather_blocks = 0,
+                           int rev = 0) {
   T *partial = (T *)m->scratch;
   return with_kind(m->kind, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
@@ -1232,10 +1233,10 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
       }
       if (K == TATVA_HEX8 && m->hex_affine)
         k_tile_assemble<K, Dens, T, MODE, true><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev);
       else
         k_tile_assemble<K, Dens, T, MODE, false><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip);
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev);
       if (part_dot)
         k_slot_gather_dot<NC, T><<<gather_blocks, kBlock, 0, s>>>(
             m->n_nodes, m->t_slot_ptr, partial, dof_map, v, out, part_dot, skip);
@@ -1265,9 +1266,9 @@ int hvp_gather_blocks(const tatva_mesh *m) {
 
 int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                        const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
-                       const int *skip) {
+                       const int *skip, int rev) {
   return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip, part_dot,
-                                          hvp_gather_blocks(m));
+                                          hvp_gather_blocks(m), rev);
 }
 
 }  // namespace tatva

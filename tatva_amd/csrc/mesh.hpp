@@ -57,10 +57,11 @@ int launch_hvp_f64(tatva_mesh *m, const tatva_density *d, const double *u, const
 int launch_hvp_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                    const int32_t *dof_map, float *y, hipStream_t s, const int *skip = nullptr);
 // y = H(u) v (FP32) with the block shares of v.y in part_dot, over
-// hvp_gather_blocks(m) blocks (v, y reduced through dof_map).
+// hvp_gather_blocks(m) blocks (v, y reduced through dof_map); rev: tiles in
+// reverse order (same result).
 int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                        const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
-                       const int *skip = nullptr);
+                       const int *skip = nullptr, int rev = 0);
 int hvp_gather_blocks(const tatva_mesh *m);
 int density_ncomp(const tatva_mesh *m, const tatva_density *d);
 // Drops cached solver graphs that captured this mesh's arrays.

@@ -1282,13 +1282,15 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
                          gv, tol, maxiter, st, s);
   if (rc) return rc;
   const int g_op = hvp_gather_blocks(m);
-  // KA: w = H z with the shares of z.w
-  const std::function<int()> apply_op = [&]() -> int {
-    return launch_hvp_dot_f32(m, d, u, z, dof_map, w, P.pap, s, &st->done);
+  // KA: w = H z with the shares of z.w; the tiles run in alternating order
+  // (iteration parity), as the CSR SpMV streams its matrix
+  const std::function<int(int)> apply_op = [&](int q) -> int {
+    return launch_hvp_dot_f32(m, d, u, z, dof_map, w, P.pap, s, &st->done, q);
   };
+  const std::function<int()> sampled_op = [&]() -> int { return apply_op(0); };
   const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
     if (with_op) {
-      const int e = apply_op();
+      const int e = apply_op(q);
       if (e) return e;
     }
     k_cg_step<float><<<gv, kBlock, 0, s>>>(n, x, r, z, p, sv, w, dinv, P.pap, g_op, P.rz(q ^ 1),
@@ -1309,7 +1311,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   key.f2 = d->p2;
   rc = chunk_graph(s, chunk, step, profiling, key, res->matfree);
   if (rc == TATVA_OK)
-    rc = run_chunks(res, res->matfree.exec, st, maxiter, chunk, profiling, apply_op, stats);
+    rc = run_chunks(res, res->matfree.exec, st, maxiter, chunk, profiling, sampled_op, stats);
   if (rc == TATVA_OK)
     TATVA_TRY_HIP(hipMemcpyAsync(x_io, x, n * sizeof(float), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
