@@ -527,6 +527,10 @@ def main(argv=None):
             "algorithmic_bytes": spmv_bytes,
             "avg_launch_us": spmv_s * 1e6 if op_samples else None,
             "samples": op_samples,
+            "note": "achieved = algorithmic bytes / timed launch; the PCG streams the matrix "
+                    "in alternating directions between iterations, so part of those bytes "
+                    "come from the 256 MB last-level cache rather than HBM (the PMC traffic "
+                    "counts every L2 miss)",
         },
         "cpu_baseline": cpu,
         "newton_iterations": newton_total,
