@@ -202,9 +202,25 @@ class Newton:
     def colored_matrix(self):
         return self._cm
 
+    def _check_layout(self, lifter) -> None:
+        """The reduced sizes, CSR pattern and kernel output lengths were fixed by the
+        constructor's lifter: a lifter for ``solve`` may change constraint values
+        (``lifter.at[key].set``), not which dofs are constrained or tied."""
+        cur = self.red.lifter
+        if lifter is cur or lifter._layout is cur._layout:
+            return
+        import numpy as np  # noqa: PLC0415
+
+        if (lifter.size != cur.size or lifter.size_reduced != cur.size_reduced
+                or not np.array_equal(lifter.read_map, cur.read_map)
+                or not np.array_equal(lifter.dof_map, cur.dof_map)):
+            raise ValueError("Newton.solve: the lifter constrains other dofs than the one the "
+                             "solver was built with; build a new Newton for it")
+
     def solve(self, u_full, lifter=None) -> NewtonStats:
         t = nat.torch()
         if lifter is not None:
+            self._check_layout(lifter)
             self.red.lifter = lifter
         lf = self.red.lifter
         lf.apply_constraints(u_full)

@@ -287,3 +287,24 @@ def test_jittered_box_keeps_every_element_positive():
         assert np.isclose(_min_corner_volume(plain.coords, plain.elements, et), (1 / 6) ** 3)
     with pytest.raises(ValueError):
         ta.Mesh.box(8, type="tetrahedron", jitter=0.9, seed=0)
+
+
+def test_newton_rejects_a_lifter_with_another_layout():
+    """Newton.solve keeps the reduced sizes and pattern of its constructor's
+    lifter: new constraint values pass, other constrained dofs raise before any
+    kernel could read or write past the end of a reduced buffer."""
+    import types
+
+    from tatva_amd import solvers
+
+    L = lifter.Lifter(12, lifter.Fixed([0, 1, 2, 9], lifter.RuntimeValue("s")))
+    nw = solvers.Newton.__new__(solvers.Newton)
+    nw.red = types.SimpleNamespace(lifter=L.at["s"].set(np.zeros(4)))
+    nw._check_layout(L.at["s"].set(np.ones(4)))                     # same layout object
+    nw._check_layout(lifter.Lifter(12, lifter.Fixed([0, 1, 2, 9], 0.5)))  # equal index sets
+    for other in (lifter.Lifter(12, lifter.Fixed([0, 1, 2], 0.0)),
+                  lifter.Lifter(12, lifter.Fixed([0, 1, 2, 10], 0.0)),
+                  lifter.Lifter(12, lifter.Fixed([0, 1, 2], 0.0), lifter.Periodic([9], [3])),
+                  lifter.Lifter(15, lifter.Fixed([0, 1, 2, 9], 0.0))):
+        with pytest.raises(ValueError, match="constrains other dofs"):
+            nw._check_layout(other)
