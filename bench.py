@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark of the element-assembly / colored-Hessian / Jacobi-PCG path on one MI355X.
+"""Benchmark of the element-assembly / matrix-free HVP / Jacobi-PCG Newton path on one MI355X.
 
-Workload (default): 3-D linear elasticity on a structured 64^3 Hex8 unit cube
-(262,144 elements, 274,625 nodes), E = 1, nu = 0.3, 2x2x2 Gauss quadrature,
-clamped at x = 0, uniform traction (1e-3, 0, 0) on x = 1 as consistent nodal
-loads t h^2 {1/4 corner, 1/2 edge, 1 interior}.  One step = one Newton solve
-from u = 0 in FP64: residual, colored Hessian -> CSR (int32), Jacobi-PCG to
-1e-10, update, residual check.  Setup (mesh upload, pattern, coloring) is not
-timed, as in the reference's setup/compile/run split
+Headline workload (default): compressible Neo-Hookean on a structured 160^3 Hex8
+unit cube (4,096,000 elements, 4,173,281 nodes, 12,442,080 free dofs), 2x2x2
+Gauss, mu = 1, lambda = 10, x = 0 clamped, u_x prescribed on x = 1 in 5 load
+steps to a stretch of 1.3 (u_x = 0.3).  Solver: mixed-precision Newton (FP64
+residual, energy and outer update; FP32 matrix-free HVP and Jacobi-PCG to 1e-3
+for each correction), Newton to 1e-8 relative residual per load step.
+
+One step = one load step solved to 1e-8 (homogeneous-stretch predictor, new
+u_x on x = 1, Newton); steps cycle through the 5 load steps and restart from
+u = 0 after the last.  Setup (mesh upload, tiling, constraints) is not timed,
+as in the reference's setup/compile/run split
 (tests/test_colored_matrix_benchmark.py:97-113).
 
-metric value: Newton DOF-iterations/s = n_free * (CG iterations) / wall seconds
-of the timed steps.  The roofline object describes the dominant kernel (the
-node-block CSR SpMV of the PCG), timed with HIP events on its own stream.
+metric value: Newton DOF-iterations/s = n_free * (inner CG iterations) / wall
+seconds of the timed steps.  Alongside: the plain FP64 GPU path (colored FP64
+Hessian -> CSR, FP64 Jacobi-PCG to 1e-10, Newton to 1e-8) on the same five load
+steps, the FP64 oracle's residual at both final states, device memory of both,
+the roofline of the dominant kernel of the mixed path (the FP32 HVP tile
+kernel), timed with HIP events on its own stream, and a CPU baseline.
 """
 
 from __future__ import annotations
@@ -260,6 +267,23 @@ def cpu_baseline(K_host, b_host, seconds=12.0, max_iters=400):
     return A.shape[0] * it / dt, it, dt
 
 
+def chunked_oracle_residuals(elem, coords, elements, dens, vecs, chunk=262144):
+    """The oracle's FP64 residuals of several states of a large mesh, accumulated
+    over element chunks (the chunk's geometry is computed once for all states)."""
+    import numpy as np
+
+    from oracle import fem  # the checker only
+
+    out = [np.zeros(v.size) for v in vecs]
+    for a in range(0, elements.shape[0], chunk):
+        prob = fem.Problem(elem, coords, elements[a:a + chunk], dens)
+        for r, v in zip(out, vecs):
+            r += prob.residual(v)
+        print(f"[oracle] residuals: {min(a + chunk, elements.shape[0])}/{elements.shape[0]} "
+              "elements", file=sys.stderr, flush=True)
+    return out
+
+
 def chunked_oracle_residual(elem, coords, elements, dens, u, chunk=262144):
     """The oracle's FP64 residual of a large mesh, accumulated over element
     chunks (residuals are sums of per-element contributions)."""
@@ -332,21 +356,362 @@ def large_mixed(workload, oracle_check=True, upto=None):
     return res
 
 
+class LoadStepper:
+    """Steps of the headline: load step k (1..n_steps) = predictor u_x += dx * x,
+    u_x = k * dx on x = 1, Newton to the solver's tolerance.  After the last load
+    step the next step restarts from u = 0."""
+
+    def __init__(self, P, solver, n_steps, stretch):
+        import torch
+
+        mesh = P["mesh"]
+        self.P, self.solver, self.n_steps, self.stretch = P, solver, n_steps, stretch
+        self.ramp = torch.zeros(mesh.n_nodes * 3, dtype=torch.float64, device="cuda")
+        self.ramp[0::3] = torch.as_tensor(mesh.coords[:, 0], device="cuda")
+        self.u = torch.zeros(mesh.n_nodes * 3, dtype=torch.float64, device="cuda")
+        self.k = 0          # load steps done in the current pass
+
+    def load(self, k):
+        return self.stretch * k / self.n_steps
+
+    def step(self):
+        if self.k == self.n_steps:
+            self.u.zero_()
+            self.k = 0
+        self.k += 1
+        self.u.add_(self.ramp, alpha=self.stretch / self.n_steps)
+        L = self.P["L"].at["load"].set(self.P["values"](self.load(self.k)))
+        return self.solver.solve(self.u, L)
+
+
+def device_gb_in_use():
+    import torch
+
+    free, total = torch.cuda.mem_get_info()
+    return (total - free) / 1e9
+
+
+def hvp_tile_bytes(op, dens_ncomp=3, cached_lin=0):
+    """Algorithmic bytes of one FP32 HVP tile-kernel launch (k_tile_assemble,
+    matrix-free PCG operator), from the mesh's device layout: per element its
+    nne uint16 local node numbers (+ cached linearisation floats per quadrature
+    point, if any), per (tile, local node) slot its global node (int32), the
+    node's FP64 coordinates, FP32 state and tangent, the tangent's dof-map
+    entries (int32) and the slot's FP32 partial sums written and the int32
+    position it is written to, plus per tile its color offsets and slot range.
+    Coordinates are counted per slot because tiles sharing a node each load it."""
+    lay = op.device_layout()
+    E, S, T = lay["n_elements"], lay["n_slots"], lay["n_tiles"]
+    nne = 8 if op.element.kind == 2 else (4 if op.element.kind == 1 else 3)
+    d = 3 if nne != 3 else 2
+    nc = dens_ncomp
+    b = 2 * nne * E + 4 * cached_lin * lay["n_quad"] * E
+    b += S * (4 + 8 * d + 4 * nc * (0 if cached_lin else 1) + (4 + 4) * nc + 4 * nc + 4)
+    b += T * (2 * (lay["max_colors"] + 1) + 8)
+    return int(b), lay
+
+
+def headline(args):
+    """C5: mixed Newton on the Hex8 Neo-Hookean 160^3 cube (the timed line),
+    then the FP64 colored-CSR Newton on the same load steps, oracle residuals."""
+    import gc
+
+    import numpy as np
+    import torch
+
+    from tatva_amd import solvers
+
+    n, steps_per_pass, stretch = args.n, 5, 0.3
+    t0 = time.perf_counter()
+    P = stretch_problem(n, kind="hex8")
+    L0 = P["L"].at["load"].set(P["values"](0.0))
+    mesh, op = P["mesh"], P["op"]
+    gb0 = device_gb_in_use()
+    mixed = solvers.Newton(P["E"], L0, mode="mixed", tol=1e-8, inner_tol=1e-3, chunk=64)
+    runner = LoadStepper(P, mixed, steps_per_pass, stretch)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    n_free = mixed.red.size
+    print(f"[headline] {mesh.n_elements} elements, n_free {n_free}, setup {setup_s:.1f} s",
+          file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        st = runner.step()
+        print(f"[headline] warm-up {w + 1}/{args.warmup}: load step {runner.k}, "
+              f"{st.newton_iterations} Newton, {st.cg_iterations} CG, {st.seconds:.2f} s",
+              file=sys.stderr, flush=True)
+    solvers.set_profiling(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = []
+    for i in range(args.steps):
+        stats.append(runner.step())
+        print(f"[headline] step {i + 1}/{args.steps}: load step {runner.k}, "
+              f"{stats[-1].newton_iterations} Newton, {stats[-1].cg_iterations} CG, "
+              f"{stats[-1].seconds:.2f} s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    solvers.set_profiling(False)
+    gb_mixed = device_gb_in_use() - gb0
+    cg_total = sum(s.cg_iterations for s in stats)
+    value = n_free * cg_total / elapsed
+    op_n = sum(s.op_samples for s in stats)
+    tile_us = sum(s.op_main_ms for s in stats) / op_n * 1e3 if op_n else None
+    op_us = sum(s.op_ms for s in stats) / op_n * 1e3 if op_n else None
+    u_mixed = runner.u.clone()
+    k_mixed = runner.k
+    mixed_full = None
+    if args.steps < steps_per_pass or args.steps % steps_per_pass:
+        # the comparison needs one whole pass over the load steps: run one untimed
+        runner.k = steps_per_pass
+        t2 = time.perf_counter()
+        full = [runner.step() for _ in range(steps_per_pass)]
+        torch.cuda.synchronize()
+        mixed_full = (time.perf_counter() - t2, full)
+        u_mixed, k_mixed = runner.u.clone(), runner.k
+    else:
+        passes = args.steps // steps_per_pass
+        mixed_full = (elapsed / passes, stats[-steps_per_pass:])
+    res = {"mixed": {
+        "seconds_per_pass": mixed_full[0],
+        "newton_iterations": sum(s.newton_iterations for s in mixed_full[1]),
+        "cg_iterations": sum(s.cg_iterations for s in mixed_full[1]),
+        "converged": all(s.converged for s in mixed_full[1]),
+        "worst_rel_residual": max(s.residual_history[-1] / s.residual_history[0]
+                                  for s in mixed_full[1]),
+        "device_gb_in_use": gb_mixed,
+    }}
+    res["mixed"]["dof_iters_per_s"] = n_free * res["mixed"]["cg_iterations"] / mixed_full[0]
+    del runner, mixed
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if not args.no_fp64:
+        gb1 = device_gb_in_use()
+        t3 = time.perf_counter()
+        csr = solvers.Newton(P["E"], L0, mode="csr", tol=1e-8, lin_tol=1e-10, chunk=64)
+        torch.cuda.synchronize()
+        csr_setup = time.perf_counter() - t3
+        print(f"[fp64] colored CSR set up in {csr_setup:.1f} s (nnz {csr.colored_matrix.nnz}, "
+              f"{csr.colored_matrix.n_colors} colors)", file=sys.stderr, flush=True)
+        P["progress"] = True
+        solvers.set_profiling(True)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        u_csr, cstats = solve_load_steps(P, csr, steps_per_pass, stretch, upto=k_mixed)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t4
+        solvers.set_profiling(False)
+        P["progress"] = False
+        cg = sum(s.cg_iterations for s in cstats)
+        on = sum(s.op_samples for s in cstats)
+        blocked = csr.colored_matrix.block_view(3) is not None
+        res["fp64_csr"] = {
+            "seconds_per_pass": dt, "setup_s": csr_setup,
+            "newton_iterations": sum(s.newton_iterations for s in cstats), "cg_iterations": cg,
+            "dof_iters_per_s": n_free * cg / dt,
+            "converged": all(s.converged for s in cstats),
+            "worst_rel_residual": max(s.residual_history[-1] / s.residual_history[0]
+                                      for s in cstats),
+            "spmv_us": sum(s.op_main_ms for s in cstats) / on * 1e3 if on else None,
+            "spmv_layout": "node-block CSR" if blocked else "sliced ELL copy of the CSR",
+            "nnz": csr.colored_matrix.nnz, "n_colors": csr.colored_matrix.n_colors,
+            "device_gb_in_use": device_gb_in_use() - gb1,
+        }
+        res["dof_iters_speedup"] = res["mixed"]["dof_iters_per_s"] / res["fp64_csr"]["dof_iters_per_s"]
+        res["time_to_solution_speedup"] = dt / res["mixed"]["seconds_per_pass"]
+        res["mixed_vs_fp64_rel_error"] = float((u_mixed - u_csr).abs().max() / u_csr.abs().max())
+        del csr
+        gc.collect()
+        torch.cuda.empty_cache()
+    else:
+        u_csr = None
+    if not args.no_oracle:
+        from oracle import fem  # the checker only
+
+        L_end = P["L"].at["load"].set(P["values"](stretch * k_mixed / steps_per_pass))
+        free = L_end.free_dofs
+        u_load = L_end.lift_from_zeros(np.zeros(free.size)).cpu().numpy()
+        vecs = [u_load, u_mixed.cpu().numpy()] + ([u_csr.cpu().numpy()] if u_csr is not None else [])
+        t5 = time.perf_counter()
+        rs = chunked_oracle_residuals(fem.Hex8(), mesh.coords, mesh.elements,
+                                      fem.NeoHookean(1.0, 10.0), vecs)
+        r_load = float(np.linalg.norm(rs[0][free]))
+        res["oracle_load_residual_norm"] = r_load
+        res["oracle_seconds"] = time.perf_counter() - t5
+        for key, r in zip(("mixed", "fp64_csr"), rs[1:]):
+            res[key]["oracle_rel_residual"] = float(np.linalg.norm(r[free]) / r_load)
+    tile_bytes, lay = hvp_tile_bytes(op)
+    achieved = tile_bytes / (tile_us * 1e-6) / 1e9 if tile_us else None
+    out = {
+        "metric": "Newton DOF-iters/s + HBM GB/s (mixed vs FP64 @1e-8 rel); serve p50/p99 ms",
+        "value": value,
+        "unit": "DOF-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / max(args.steps, 1) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 HVP/CG + f64 residual/Newton",
+        "data": "synthetic (structured mesh, prescribed boundary displacements)",
+        "config": {
+            "workload": f"hex8 Neo-Hookean {n}^3, mu=1, lambda=10, x=0 clamped, u_x on x=1 in "
+                        f"{steps_per_pass} load steps to {stretch}; one step = one load step, "
+                        "mixed Newton (FP32 matrix-free Jacobi-PCG to 1e-3) to 1e-8 relative",
+            "elements": mesh.n_elements, "nodes": mesh.n_nodes, "n_free": n_free,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "k_tile_assemble<HEX8, NeoHookean<3,float>, float, hvp, affine> "
+                      "(FP32 HVP tile kernel of the matrix-free PCG operator)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "algorithmic_bytes": tile_bytes,
+            "avg_launch_us": tile_us,
+            "operator_us": op_us,
+            "samples": op_n,
+            "layout": lay,
+        },
+        "cpu_baseline": None,
+        "newton_iterations": sum(s.newton_iterations for s in stats),
+        "cg_iterations": cg_total,
+        "converged": all(s.converged for s in stats),
+        "setup_s": setup_s,
+        **res,
+    }
+    return out
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    capped by OMP_NUM_THREADS when the host sets it (the GPU host's share)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.strip().isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
+def cpu_baseline_headline(args):
+    """The oracle's FP64 element kernels in C over OpenMP threads
+    (oracle/fem_omp.c), driving the same inexact-Newton step on the headline
+    mesh: residual assembly, Jacobi diagonal, then matrix-free Jacobi-PCG
+    iterations (one HVP each; NumPy vector updates) for a bounded time.  The
+    state is the first load step's start (u_x = 0.06 x, constraints set)."""
+    import numpy as np
+
+    from oracle import fem  # the CPU baseline only
+
+    n = args.n
+    threads = cpu_threads()
+    X, el = fem.cube_hex8(n)
+    x0 = np.where(np.isclose(X[:, 0], 0.0))[0]
+    x1 = np.where(np.isclose(X[:, 0], 1.0))[0]
+    fixed = np.concatenate([(x0[:, None] * 3 + np.arange(3)).ravel(), x1 * 3])
+    size = X.shape[0] * 3
+    free = np.setdiff1d(np.arange(size), fixed)
+    u = np.zeros(size)
+    u[0::3] = 0.06 * X[:, 0]
+    u[(x0[:, None] * 3 + np.arange(3)).ravel()] = 0.0
+    Q = fem.ProblemOMP(fem.Hex8(), X, el, fem.NeoHookean(1.0, 10.0), threads=threads)
+    try:  # BLAS threads of the NumPy dot products would compete with the OpenMP ones
+        from threadpoolctl import threadpool_limits
+        limit = threadpool_limits(1, user_api="blas")
+    except ImportError:
+        limit = None
+    t0 = time.perf_counter()
+    b = -Q.residual(u)[free]
+    dinv = 1.0 / Q.hessian_diag(u)[free]
+    t_asm = time.perf_counter() - t0
+    x = np.zeros(free.size)
+    r = b.copy()
+    z = dinv * r
+    p = z.copy()
+    rz = r @ z
+    bn = np.linalg.norm(b)
+    full = np.zeros(size)
+    its = 0
+    while time.perf_counter() - t0 < args.cpu_seconds and np.linalg.norm(r) > 1e-3 * bn:
+        full[free] = p
+        Ap = Q.hvp(u, full)[free]
+        alpha = rz / (p @ Ap)
+        x += alpha * p
+        r -= alpha * Ap
+        z = dinv * r
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+        its += 1
+    dt = time.perf_counter() - t0
+    if limit is not None:
+        limit.restore_original_limits()
+    return {"value": free.size * its / dt, "unit": "DOF-iters/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle FP64 Hex8 Neo-Hookean {n}^3 kernels in C (oracle/fem_omp.c, "
+                      f"OpenMP, {threads} threads): residual + Jacobi diagonal assembly "
+                      f"({t_asm:.1f} s) then {its} matrix-free Jacobi-PCG iterations of the first "
+                      f"Newton step on {free.size} free dofs, {dt:.1f} s in total (assembly "
+                      "included)"}
+
+
+def c2_extra():
+    """C2 (Hex8 linear elasticity 64^3, traction 1e-3 on x = 1): FP64 colored-CSR
+    Newton vs mixed Newton on the same problem, with the oracle's FP64 residual
+    of both solutions relative to the load's."""
+    import numpy as np
+    import torch
+
+    from oracle import fem  # the checker only
+
+    P = build_problem(64)
+    nd = P["mesh"].n_nodes * 3
+
+    def run_c2(sv):
+        uu = torch.zeros(nd, dtype=torch.float64, device="cuda")
+        return uu, [sv.solve(uu)]
+
+    r = mixed_vs_fp64(P["E"], P["L"], run_c2, colored_matrix=P["cm"], keep_solutions=True)
+    mesh, dn = P["mesh"], P["dens"]
+    prob = fem.Problem(fem.Hex8(), mesh.coords, mesh.elements,
+                       fem.LinearElasticity(dn.mu, dn.lmbda), f_ext=P["f"])
+    free = P["L"].free_dofs
+    r0 = np.linalg.norm(prob.residual(np.zeros(nd))[free])
+    for mode in ("mixed", "csr"):
+        uh = r.pop(f"_u_{mode}").cpu().numpy()
+        r[mode]["oracle_rel_residual"] = float(np.linalg.norm(prob.residual(uh)[free]) / r0)
+    r["workload"] = ("hex8 linear elasticity 64^3, E=1, nu=0.3, x=0 clamped, traction 1e-3 "
+                     "on x=1; FP64 CSR PCG to 1e-10 vs mixed Newton to 1e-8")
+    return r
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=64, help="cells per side of the Hex8 cube")
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=None,
+                    help="cells per side of the Hex8 cube (headline 160, c2 64)")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="headline: skip the FP64 colored-CSR comparison")
+    ap.add_argument("--no-oracle", action="store_true",
+                    help="headline: skip the oracle residuals of the final states")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-vs-FP64 and C1 latency sections")
     ap.add_argument("--nh-n", type=int, default=48, help="cells per side, Neo-Hookean comparison")
     ap.add_argument("--nh-steps", type=int, default=2, help="load steps, Neo-Hookean comparison")
     ap.add_argument("--nh-stretch", type=float, default=0.1, help="final u_x on x = 1")
-    ap.add_argument("--workload", choices=("c2", "c2mixed", "c3", "c4", "c5", "nh48"), default="c2",
-                    help="c2: the default line; c2mixed / nh48: only the mixed-vs-FP64 "
+    ap.add_argument("--workload", choices=("headline", "c2", "c2mixed", "c3", "c4", "c5", "nh48"),
+                    default="headline",
+                    help="headline: the default line (Hex8 Neo-Hookean 160^3 mixed Newton, "
+                         "FP64 path alongside); c2: the FP64 Hex8 elasticity 64^3 line; "
+                         "c2mixed / nh48: only the mixed-vs-FP64 "
                          "comparison on the 64^3 elasticity / 48^3 Neo-Hookean cube; "
                          "c3/c5: large mixed-vs-FP64 Neo-Hookean runs; c4: the serving "
                          "stream only (--serve-requests, --serve-qps)")
@@ -367,6 +732,22 @@ def main(argv=None):
     from tatva_amd import solvers
 
     torch.cuda.set_device(0)
+    if args.workload == "headline":
+        if args.n is None:
+            args.n = 160
+        out = headline(args)
+        if not args.no_extras:
+            out["c2_mixed_vs_fp64"] = c2_extra()
+            out["c1_poisson_tri3_64"] = c1_latency()
+            if args.serve_requests > 0:
+                out["serve_c4"] = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk)
+                                   for q in args.serve_qps]
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline_headline(args)
+        print(json.dumps(out), flush=True)
+        return 0
+    if args.n is None:
+        args.n = 64
     if args.workload in ("c3", "c5"):
         r = large_mixed(args.workload, oracle_check=not args.no_cpu_baseline,
                         upto=args.large_steps)

@@ -63,6 +63,8 @@ typedef struct {
   double op_ms;           /* with profiling on: summed duration of the sampled operator launches */
   int32_t op_samples;     /* number of sampled operator launches (0 with profiling off) */
   int32_t reserved;
+  double op_main_ms;      /* with profiling on: the sampled operators' main kernel alone (the
+                             SpMV; the tile kernel of a matrix-free HVP, without its gather) */
 } tatva_pcg_stats;
 
 const char *tatva_last_error(void);
@@ -75,6 +77,15 @@ int tatva_device_count(int *out);
  * tatva_pcg_stats.op_ms / op_samples. */
 int tatva_set_profiling(int enable);
 
+/* Process-wide options (no counterpart in the reference).
+ * TATVA_OPT_MATFREE_LIN_CACHE (default 1): tatva_pcg_matfree_f32 linearises the
+ * density once per solve at its fixed state (Neo-Hookean: F^-1 and
+ * mu - lambda ln J per quadrature point, FP32, in the workspace) and every HVP
+ * of the solve reads it instead of recomputing it; 0 recomputes it in every
+ * HVP (same arithmetic, same result). */
+enum { TATVA_OPT_MATFREE_LIN_CACHE = 1 };
+int tatva_set_option(int32_t option, int32_t value);
+
 /* ---------------------------------------------------------------- mesh ---- */
 
 /* Uploads a mesh (host arrays) and builds its node->element incidence.
@@ -86,6 +97,12 @@ int tatva_mesh_create(int device, int elem_kind, const double *coords, int64_t n
                       const double *quad_points, const double *quad_weights,
                       tatva_mesh **out);
 int tatva_mesh_destroy(tatva_mesh *mesh);
+
+/* Sizes of a mesh handle's device layout (no counterpart in the reference; for
+ * byte counts of the kernels): out[0..n) of {n_nodes, n_elements, n_tiles,
+ * n_slots (tile, local node) pairs, largest local node count of a tile, largest
+ * element color count of a tile, affine Hex8 flag, quadrature points}. */
+int tatva_mesh_info(const tatva_mesh *mesh, int64_t *out, int32_t n);
 
 /* Integration weights detJ*w_q, out (E, Q).
  * Replaces Operator.get_integration_weights (tatva/operator.py:172-192). */
@@ -228,9 +245,12 @@ int tatva_spmv_bcsr(int64_t n, const int32_t *indptr, const int32_t *block_cols,
  * x holds x0 on entry.  Iterations run in hipGraph-captured chunks of `chunk`.
  * With block_cols (from tatva_csr_block_view) the operator reads the CSR data
  * through the node-block view; without, the call first copies the matrix into
- * a sliced-ELL layout (64-row slices) held in a device buffer the library owns
- * and grows as needed (one host synchronisation per call). */
-int64_t tatva_pcg_csr_workspace(int64_t n);
+ * a sliced-ELL layout (64-row slices) in the caller's workspace (one host
+ * synchronisation per call).  The workspace query takes the HOST copy of
+ * indptr when the solve will run without a block view (NULL: with one), and
+ * then includes the sliced-ELL copy.  The library keeps no buffer of the
+ * solve after it returns. */
+int64_t tatva_pcg_csr_workspace(int64_t n, const int32_t *host_indptr);
 int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
                   const double *data, const int32_t *block_cols, int32_t block_size,
                   const double *b, double *x, double tol, int32_t maxiter, int32_t chunk,
@@ -238,8 +258,10 @@ int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *
                   void *stream);
 
 /* Jacobi-PCG with the matrix-free FP32 Hessian-vector product of (mesh, density)
- * at the FP32 state u (full vector); b, x, diag are reduced FP32 vectors. */
-int64_t tatva_pcg_matfree_workspace(tatva_mesh *mesh, int64_t n);
+ * at the FP32 state u (full vector); b, x, diag are reduced FP32 vectors.  The
+ * workspace holds the CG vectors, a copy of u and, for a Neo-Hookean density,
+ * the cache of its linearisation at u (TATVA_OPT_MATFREE_LIN_CACHE). */
+int64_t tatva_pcg_matfree_workspace(tatva_mesh *mesh, const tatva_density *density, int64_t n);
 int tatva_pcg_matfree_f32(tatva_mesh *mesh, const tatva_density *density, const float *u,
                           const int32_t *dof_map, int64_t n, const float *diag, const float *b,
                           float *x, double tol, int32_t maxiter, int32_t chunk, void *workspace,
@@ -322,8 +344,12 @@ int tatva_axpy_free(int64_t n_red, const int32_t *free_dofs, double alpha, const
 /* y = x cast to FP32 / FP64, scaled by alpha. */
 int tatva_cast_f64_f32(int64_t n, double alpha, const double *x, float *y, void *stream);
 int tatva_cast_f32_f64(int64_t n, double alpha, const float *x, double *y, void *stream);
-/* out[0] = x . y (deterministic reduction, FP64). */
-int tatva_dot(int64_t n, const double *x, const double *y, double *out, void *stream);
+/* out[0] = x . y (deterministic reduction, FP64), the per-block partial sums in
+ * the caller's workspace of tatva_dot_workspace(n) bytes (calls on different
+ * streams need different workspaces). */
+int64_t tatva_dot_workspace(int64_t n);
+int tatva_dot(int64_t n, const double *x, const double *y, double *out, void *workspace,
+              int64_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------ host-side setup (no GPU) -- */
 
@@ -358,6 +384,14 @@ void tatva_csr_host_free(tatva_csr_host *csr);
  * tatva/sparse/base.py:59-74).  Writes colors (n) and the number of colors. */
 int tatva_distance2_colors(int64_t n, const int32_t *indptr, const int32_t *indices,
                            int32_t *colors, int32_t *n_colors);
+
+/* Checks that colors (n, host) are a distance-2 coloring of a CSR pattern (host
+ * arrays): no row has two columns of one color, which the colored
+ * decompression of tatva_colored_hessian needs (ColoredMatrix.from_csr with
+ * caller colors, tatva/sparse/base.py:59-74).  *bad_row = the first offending
+ * row, or -1 when the coloring is valid. */
+int tatva_check_distance2_colors(int64_t n, const int32_t *indptr, const int32_t *indices,
+                                 const int32_t *colors, int64_t *bad_row);
 
 #ifdef __cplusplus
 }

@@ -575,6 +575,37 @@ def greedy_coloring(adjacency):
     return colors
 
 
+def greedy_coloring_ldf(adjacency):
+    """``greedy_coloring_ldf`` (_coloring.py:156-186): vertices in the order of
+    ``np.argsort(degrees)[::-1]`` (degree = size of the A^2 row), each the
+    smallest color its A^2 neighbours do not hold."""
+    n = len(adjacency)
+    degrees = np.array([len(a) for a in adjacency], dtype=np.int32)
+    colors = -np.ones(n, dtype=np.int32)
+    for i in np.argsort(degrees)[::-1]:
+        used = {c for c in colors[adjacency[i]] if c >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        colors[i] = c
+    return colors
+
+
+def largest_degree_first_distance2_colors(indptr, indices, n):
+    """``largest_degree_first_distance2_colors`` (_coloring.py:286-300)."""
+    return greedy_coloring_ldf(distance2_adjacency(n, np.asarray(indptr), np.asarray(indices)))
+
+
+def is_distance2_coloring(indptr, indices, colors):
+    """True when no row of the pattern has two columns of one color (what the
+    colored decompression needs; tests/test_sparse.py:76-77 checks its result)."""
+    for r in range(len(indptr) - 1):
+        c = colors[indices[indptr[r]:indptr[r + 1]]]
+        if np.unique(c).size != c.size:
+            return False
+    return True
+
+
 def distance2_colors_py(indptr, indices, n):
     """``distance2_colors`` (_coloring.py:270-283), pure Python (small n only)."""
     return greedy_coloring(distance2_adjacency(n, np.asarray(indptr), np.asarray(indices)))
@@ -594,6 +625,13 @@ def _clib():
         lib.oracle_distance2_greedy.restype = ctypes.c_int
         lib.oracle_distance2_greedy.argtypes = [
             ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.oracle_assemble.restype = ctypes.c_int
+        lib.oracle_assemble.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+            ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+            ctypes.c_int]
+        lib.oracle_max_threads.restype = ctypes.c_int
         _CLIB = lib
     return _CLIB
 
@@ -607,6 +645,57 @@ def distance2_colors(indptr, indices, n):
     if rc < 0:
         raise RuntimeError("oracle coloring failed")
     return out
+
+
+class ProblemOMP:
+    """``Problem.residual`` / ``hvp`` / ``hessian_diag`` of a Tet4 or Hex8 mesh with
+    linear elasticity or Neo-Hookean, in plain C over OpenMP threads
+    (oracle/fem_omp.c: the same nodal element algebra, elements in parallel,
+    atomic scatter into nodes).  For the CPU baseline at sizes where the NumPy
+    Problem's per-element arrays do not fit; pinned against Problem in the
+    oracle's tests."""
+
+    _KIND = {"Tet4": 1, "Hex8": 2}
+
+    def __init__(self, elem, coords, elements, density, threads=0):
+        kind = self._KIND.get(type(elem).__name__)
+        if kind is None or not isinstance(density, (LinearElasticity, NeoHookean)):
+            raise NotImplementedError("ProblemOMP: Tet4/Hex8 with LinearElasticity/NeoHookean")
+        self.kind = kind
+        self.dens = 2 if isinstance(density, LinearElasticity) else 3
+        self.mu, self.lmbda = float(density.mu), float(density.lmbda)
+        self.X = np.ascontiguousarray(coords, dtype=np.float64)
+        self.conn = np.ascontiguousarray(elements, dtype=np.int32)
+        self.qp = np.ascontiguousarray(elem.quad_points, dtype=np.float64)
+        self.qw = np.ascontiguousarray(elem.quad_weights, dtype=np.float64)
+        self.n = self.X.shape[0] * 3
+        self.threads = int(threads)
+
+    def _run(self, mode, u, v=None):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty(self.n)
+        rc = _clib().oracle_assemble(
+            self.kind, self.dens, self.mu, self.lmbda, self.X.ctypes.data, self.X.shape[0],
+            self.conn.ctypes.data, self.conn.shape[0], self.qw.shape[0], self.qp.ctypes.data,
+            self.qw.ctypes.data, u.ctypes.data, None if vv is None else vv.ctypes.data, mode,
+            out.ctypes.data, self.threads)
+        if rc != 0:
+            raise RuntimeError("oracle_assemble failed")
+        return out
+
+    def residual(self, u):
+        return self._run(0, u)
+
+    def hvp(self, u, v):
+        return self._run(1, u, v)
+
+    def hessian_diag(self, u):
+        return self._run(2, u)
+
+
+def omp_max_threads() -> int:
+    return int(_clib().oracle_max_threads())
 
 
 # --------------------------------------------------------------------------

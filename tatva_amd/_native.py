@@ -32,7 +32,7 @@ class PcgStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("status", ctypes.c_int32),
                 ("rel_residual", ctypes.c_double), ("b_norm", ctypes.c_double),
                 ("op_ms", ctypes.c_double), ("op_samples", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("op_main_ms", ctypes.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -41,6 +41,8 @@ _SIGS = {
     "tatva_version": (c_int, []),
     "tatva_device_count": (c_int, [ctypes.POINTER(c_int)]),
     "tatva_set_profiling": (c_int, [c_int]),
+    "tatva_set_option": (c_int, [c_i32, c_i32]),
+    "tatva_mesh_info": (c_int, [c_void, ctypes.POINTER(c_i64), c_int]),
     "tatva_mesh_create": (c_int, [c_int, c_int, c_void, c_i64, c_int, c_void, c_i64, c_int, c_int,
                                   c_void, c_void, ctypes.POINTER(c_void)]),
     "tatva_mesh_destroy": (c_int, [c_void]),
@@ -73,14 +75,14 @@ _SIGS = {
                                   c_void]),
     "tatva_rows_cols": (c_int, [c_i64, c_void, c_void, c_void, c_void, c_void, c_void]),
     "tatva_spmv_csr": (c_int, [c_i64, c_i64, c_void, c_void, c_void, c_void, c_void, c_void]),
-    "tatva_pcg_csr_workspace": (c_i64, [c_i64]),
+    "tatva_pcg_csr_workspace": (c_i64, [c_i64, c_void]),
     "tatva_pcg_csr": (c_int, [c_i64, c_i64, c_void, c_void, c_void, c_void, c_i32, c_void, c_void,
                               c_dbl, c_i32, c_i32, c_void, c_i64, ctypes.POINTER(PcgStats),
                               c_void]),
     "tatva_csr_block_view": (c_int, [c_i64, c_i64, c_void, c_void, c_i32, c_void,
                                      ctypes.POINTER(c_i32), c_void]),
     "tatva_spmv_bcsr": (c_int, [c_i64, c_void, c_void, c_i32, c_void, c_void, c_void, c_void]),
-    "tatva_pcg_matfree_workspace": (c_i64, [c_void, c_i64]),
+    "tatva_pcg_matfree_workspace": (c_i64, [c_void, ctypes.POINTER(Density), c_i64]),
     "tatva_pcg_matfree_f32": (c_int, [c_void, ctypes.POINTER(Density), c_void, c_void, c_i64, c_void,
                                       c_void, c_void, c_dbl, c_i32, c_i32, c_void, c_i64,
                                       ctypes.POINTER(PcgStats), c_void]),
@@ -107,7 +109,10 @@ _SIGS = {
     "tatva_axpy_free": (c_int, [c_i64, c_void, c_dbl, c_void, c_void, c_void]),
     "tatva_cast_f64_f32": (c_int, [c_i64, c_dbl, c_void, c_void, c_void]),
     "tatva_cast_f32_f64": (c_int, [c_i64, c_dbl, c_void, c_void, c_void]),
-    "tatva_dot": (c_int, [c_i64, c_void, c_void, c_void, c_void]),
+    "tatva_dot_workspace": (c_i64, [c_i64]),
+    "tatva_check_distance2_colors": (c_int, [c_i64, c_void, c_void, c_void,
+                                             ctypes.POINTER(c_i64)]),
+    "tatva_dot": (c_int, [c_i64, c_void, c_void, c_void, c_void, c_i64, c_void]),
     "tatva_pattern_from_mesh": (c_int, [c_void, c_i64, c_int, c_i64, c_int, c_void, c_i64,
                                         ctypes.POINTER(c_void)]),
     "tatva_pattern_from_element_dofs": (c_int, [c_void, c_i64, c_int, c_i64, c_void, c_i64,

@@ -48,6 +48,17 @@ struct TileSet {
   const uint16_t *inc_list;  // (n_tiles * kTile * nne)
 };
 
+// Element work of a tile kernel.  kModeLinearize stores the density's
+// linearisation (Dens::kLin values per quadrature point, from the state u) in a
+// cache and writes no output; kModeHvpLin is kModeHvp reading that cache instead
+// of re-linearising at every call (same arithmetic, same result), for solvers
+// whose state is fixed over many products.
+enum { kModeResidual = 0, kModeHvp = 1, kModeDiag = 2, kModeLinearize = 3, kModeHvpLin = 4 };
+
+// Cache layout: value k of quadrature point q of the element in tile slot t of
+// tile b at lin[((b * nq + q) * kLin + k) * kTile + t] (a tile's threads read
+// consecutive floats).
+
 // Per-tile sum by the incidence lists instead of element colors.
 template <int KIND>
 constexpr bool tile_gather() {
@@ -101,12 +112,16 @@ __device__ __forceinline__ void elem_residual(const double (&X)[Elem<KIND>::NNE]
 }
 
 // y_e = sum_q w B^T dP(G(u_e); G(v_e)); geometry in FP64, physics in T.
-template <int KIND, class Dens, class T>
+// MODE kModeLinearize stores the linearisation at G(u_e) to lin (point q's
+// values from lin + q * kLin * kTile, stride kTile) and leaves ye zero;
+// kModeHvpLin reads it from there instead of linearising at G(u_e).
+template <int KIND, class Dens, class T, int MODE = kModeHvp>
 __device__ __forceinline__ void elem_hvp(const double (&X)[Elem<KIND>::NNE][Elem<KIND>::D],
                                          const T (&ue)[Elem<KIND>::NNE][Dens::NC],
                                          const T (&ve)[Elem<KIND>::NNE][Dens::NC],
                                          const QuadRule &quad, const Dens &dens,
-                                         T (&ye)[Elem<KIND>::NNE][Dens::NC]) {
+                                         T (&ye)[Elem<KIND>::NNE][Dens::NC],
+                                         T *lin = nullptr) {
   using El = Elem<KIND>;
   constexpr int D = El::D, NNE = El::NNE, NC = Dens::NC;
 #pragma unroll
@@ -129,14 +144,24 @@ __device__ __forceinline__ void elem_hvp(const double (&X)[Elem<KIND>::NNE][Elem
         T g = T(0), dg = T(0);
 #pragma unroll
         for (int a = 0; a < NNE; ++a) {
-          g += B[j][a] * ue[a][c];
-          dg += B[j][a] * ve[a][c];
+          if constexpr (MODE != kModeHvpLin) g += B[j][a] * ue[a][c];
+          if constexpr (MODE != kModeLinearize) dg += B[j][a] * ve[a][c];
         }
         G[c][j] = g;
         dG[c][j] = dg;
       }
-    const auto L = dens.linearize(G);
-    dens.dP(L, dG, dP);
+    T *lq = (MODE == kModeLinearize || MODE == kModeHvpLin)
+                ? lin + (int64_t)q * Dens::kLin * kTile : nullptr;
+    if constexpr (MODE == kModeLinearize) {
+      dens.store_lin(dens.linearize(G), lq, kTile);
+      continue;
+    }
+    if constexpr (MODE == kModeHvpLin) {
+      dens.dP(dens.load_lin(lq, kTile), dG, dP);
+    } else {
+      const auto L = dens.linearize(G);
+      dens.dP(L, dG, dP);
+    }
 #pragma unroll
     for (int a = 0; a < NNE; ++a)
 #pragma unroll
@@ -198,7 +223,6 @@ __device__ __forceinline__ void elem_diag(const double (&X)[Elem<KIND>::NNE][Ele
   }
 }
 
-enum { kModeResidual = 0, kModeHvp = 1, kModeDiag = 2 };
 
 // ------------------------------------------------- Hex8 in modal form -----
 //
@@ -273,16 +297,20 @@ __device__ __forceinline__ void hex_dmodes(const S (&M)[8], S x, S y, S z, S (&d
 // Hex8 residual (MODE = kModeResidual: S = P(G), plus the Poisson source) or
 // Hessian-vector product (S = dP(G(u); G(v))): ye = sum_q w_q B_q^T S_q.
 // Geometry in FP64, physics in T.
+// MODE kModeLinearize / kModeHvpLin: as elem_hvp (cache at lin, stride kTile).
 template <class Dens, class T, int MODE, bool AFFINE>
 __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&ue)[8][Dens::NC],
                                              const T (&ve)[8][Dens::NC], const QuadRule &quad,
-                                             const Dens &dens, T (&ye)[8][Dens::NC]) {
+                                             const Dens &dens, T (&ye)[8][Dens::NC],
+                                             T *lin = nullptr) {
   constexpr int NC = Dens::NC;
+  constexpr bool kUsesU = MODE != kModeHvpLin;
+  constexpr bool kUsesV = MODE == kModeHvp || MODE == kModeHvpLin;
   double XM[3][8];
   hex_modes<double, 3>(X, XM);
   T UM[NC][8], VM[NC][8];
-  hex_modes<T, NC>(ue, UM);
-  hex_modes<T, NC>(ve, VM);
+  if constexpr (kUsesU) hex_modes<T, NC>(ue, UM);
+  if constexpr (kUsesV) hex_modes<T, NC>(ve, VM);
   T A[NC][8];
 #pragma unroll
   for (int c = 0; c < NC; ++c)
@@ -329,12 +357,20 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
     const T xt = T(x), yt = T(y), zt = T(z);
     // G[c][j] = sum_k H[c][k] inv[j][k], H = natural derivatives
     T G[NC][3], S[NC][3];
+    if constexpr (kUsesU) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      T h[3];
-      hex_dmodes<T>(UM[c], xt, yt, zt, h);
+      for (int c = 0; c < NC; ++c) {
+        T h[3];
+        hex_dmodes<T>(UM[c], xt, yt, zt, h);
 #pragma unroll
-      for (int j = 0; j < 3; ++j) G[c][j] = h[0] * iv[j][0] + h[1] * iv[j][1] + h[2] * iv[j][2];
+        for (int j = 0; j < 3; ++j) G[c][j] = h[0] * iv[j][0] + h[1] * iv[j][1] + h[2] * iv[j][2];
+      }
+    }
+    T *lq = (MODE == kModeLinearize || MODE == kModeHvpLin)
+                ? lin + (int64_t)q * Dens::kLin * kTile : nullptr;
+    if constexpr (MODE == kModeLinearize) {
+      dens.store_lin(dens.linearize(G), lq, kTile);
+      continue;
     }
     if constexpr (MODE == kModeResidual) {
       dens.P(G, S);
@@ -347,8 +383,12 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
 #pragma unroll
         for (int j = 0; j < 3; ++j) dG[c][j] = h[0] * iv[j][0] + h[1] * iv[j][1] + h[2] * iv[j][2];
       }
-      const auto L = dens.linearize(G);
-      dens.dP(L, dG, S);
+      if constexpr (MODE == kModeHvpLin) {
+        dens.dP(dens.load_lin(lq, kTile), dG, S);
+      } else {
+        const auto L = dens.linearize(G);
+        dens.dP(L, dG, S);
+      }
     }
     // Q[c][k] = w sum_j S[c][j] inv[j][k], accumulated into the derivative modes
 #pragma unroll
@@ -414,7 +454,8 @@ inline bool hex_is_affine_host(const double (&X)[8][3]) {
 // space, and the tile's incidence lists follow it.
 constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 constexpr size_t tile_stage_bytes(int max_local, int D, int NC, size_t tsize, int mode) {
-  return (size_t)max_local * (D * sizeof(double) + (mode == kModeHvp ? 3 : 2) * NC * tsize);
+  return (size_t)max_local *
+         (D * sizeof(double) + ((mode == kModeHvp || mode == kModeHvpLin) ? 3 : 2) * NC * tsize);
 }
 constexpr size_t tile_union_bytes(int max_local, int D, int nne, int NC, size_t tsize, int mode) {
   const size_t staged = tile_stage_bytes(max_local, D, NC, tsize, mode);
@@ -440,7 +481,10 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
                                               const T *__restrict__ u, const T *__restrict__ v,
                                               const int32_t *__restrict__ dof_map,
                                               const TileSet &ts, T *__restrict__ partial,
-                                              unsigned char *__restrict__ lds) {
+                                              unsigned char *__restrict__ lds,
+                                              T *__restrict__ lin = nullptr) {
+  constexpr bool kUsesU = MODE != kModeHvpLin;
+  constexpr bool kUsesV = MODE == kModeHvp || MODE == kModeHvpLin;
   using El = Elem<KIND>;
   constexpr int D = El::D, NNE = El::NNE, NC = Dens::NC;
   const int32_t nb = ts.node_ptr[tile];
@@ -457,8 +501,8 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     for (int c = 0; c < NC; ++c) {
       const int64_t dof = g * NC + c;
       acc[j * NC + c] = T(0);
-      us[j * NC + c] = u[dof];
-      if constexpr (MODE == kModeHvp) {
+      if constexpr (kUsesU) us[j * NC + c] = u[dof];
+      if constexpr (kUsesV) {
         if (dof_map) {
           const int32_t r = dof_map[dof];
           vs[j * NC + c] = r >= 0 ? v[r] : T(0);
@@ -493,30 +537,31 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
   __syncthreads();
   if (real) {
     double X[NNE][D];
-    T ue[NNE][NC];
+    T ue[NNE][NC], ve[NNE][NC];
 #pragma unroll
     for (int a = 0; a < NNE; ++a) {
 #pragma unroll
       for (int k = 0; k < D; ++k) X[a][k] = Xs[l[a] * D + k];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) ue[a][c] = us[l[a] * NC + c];
+      for (int c = 0; c < NC; ++c) {
+        ue[a][c] = kUsesU ? us[l[a] * NC + c] : T(0);
+        ve[a][c] = kUsesV ? vs[l[a] * NC + c] : T(0);
+      }
     }
+    // this thread's cached linearisation (kModeLinearize / kModeHvpLin)
+    T *lp = lin ? lin + (int64_t)tile * quad.nq * Dens::kLin * kTile + threadIdx.x : nullptr;
     if constexpr (KIND == TATVA_HEX8 && MODE == kModeResidual) {
       hex8_element<Dens, T, kModeResidual, AFFINE>(X, ue, ue, quad, dens, ye);
     } else if constexpr (MODE == kModeResidual) {
       elem_residual<KIND, Dens>(X, ue, quad, dens, ye);
-    } else if constexpr (MODE == kModeHvp) {
-      T ve[NNE][NC];
-#pragma unroll
-      for (int a = 0; a < NNE; ++a)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) ve[a][c] = vs[l[a] * NC + c];
-      if constexpr (KIND == TATVA_HEX8) hex8_element<Dens, T, kModeHvp, AFFINE>(X, ue, ve, quad, dens, ye);
-      else elem_hvp<KIND, Dens, T>(X, ue, ve, quad, dens, ye);
+    } else if constexpr (MODE == kModeHvp || MODE == kModeHvpLin || MODE == kModeLinearize) {
+      if constexpr (KIND == TATVA_HEX8) hex8_element<Dens, T, MODE, AFFINE>(X, ue, ve, quad, dens, ye, lp);
+      else elem_hvp<KIND, Dens, T, MODE>(X, ue, ve, quad, dens, ye, lp);
     } else {
       elem_diag<KIND, Dens, T>(X, ue, quad, dens, ye);
     }
   }
+  if constexpr (MODE == kModeLinearize) return;  // the cache is the output
   if constexpr (tile_gather<KIND>()) {
     __syncthreads();  // every element has read its nodes: the LDS is free
     T *ys = reinterpret_cast<T *>(lds);
@@ -567,7 +612,8 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
                                                          const T *__restrict__ v,
                                                          const int32_t *__restrict__ dof_map,
                                                          TileSet ts, T *__restrict__ partial,
-                                                         const int *__restrict__ skip, int rev) {
+                                                         const int *__restrict__ skip, int rev,
+                                                         T *__restrict__ lin) {
   if (skip && *skip) return;
   extern __shared__ unsigned char lds_raw[];
   // rev: tiles from the end (a solver alternates the order between iterations
@@ -576,7 +622,7 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
   int tile = (int)xcd_block(blockIdx.x, gridDim.x);
   if (rev) tile = (int)gridDim.x - 1 - tile;
   tile_assemble<KIND, Dens, T, MODE, AFFINE>(tile, coords, quad, dens, u, v, dof_map, ts, partial,
-                                             lds_raw);
+                                             lds_raw, lin);
 }
 
 // Sum of node i's tile slots (ascending tile): the tiles wrote them to the

@@ -22,6 +22,9 @@ struct Poisson {
   static constexpr bool kSource = true;
   T f;
   struct Lin {};
+  static constexpr int kLin = 0;  // values of a cached linearisation per point
+  __device__ __forceinline__ void store_lin(const Lin &, T *, int) const {}
+  __device__ __forceinline__ Lin load_lin(const T *, int) const { return Lin{}; }
   __device__ __forceinline__ T psi(const T G[NC][D], T uq) const {
     T s = T(0);
 #pragma unroll
@@ -45,6 +48,9 @@ struct LinearElastic {
   static constexpr bool kSource = false;
   T mu, lam;
   struct Lin {};
+  static constexpr int kLin = 0;
+  __device__ __forceinline__ void store_lin(const Lin &, T *, int) const {}
+  __device__ __forceinline__ Lin load_lin(const T *, int) const { return Lin{}; }
   __device__ __forceinline__ void stress(const T G[NC][D], T s[NC][D]) const {
     T tr = T(0);
 #pragma unroll
@@ -98,6 +104,26 @@ struct NeoHookean {
     const T J = det_inv(F, L.Fi);
     L.lnJ = tlog<T>(J);
     L.c1 = mu - lam * L.lnJ;
+    return L;
+  }
+  // dP needs F^-1 and c1 only: a cached linearisation holds those D*D + 1
+  // values per point, value k of a point at dst[k * stride]
+  static constexpr int kLin = D * D + 1;
+  __device__ __forceinline__ void store_lin(const Lin &L, T *dst, int stride) const {
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) dst[(i * D + j) * stride] = L.Fi[i][j];
+    dst[D * D * stride] = L.c1;
+  }
+  __device__ __forceinline__ Lin load_lin(const T *src, int stride) const {
+    Lin L;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) L.Fi[i][j] = src[(i * D + j) * stride];
+    L.c1 = src[D * D * stride];
+    L.lnJ = T(0);  // not used by dP
     return L;
   }
   __device__ __forceinline__ T psi(const T G[NC][D], T) const {

@@ -1090,6 +1090,14 @@ extern "C" int tatva_mesh_destroy(tatva_mesh *m) {
   return TATVA_OK;
 }
 
+extern "C" int tatva_mesh_info(const tatva_mesh *m, int64_t *out, int32_t n) {
+  TATVA_REQUIRE(m && out && n >= 0, "invalid arguments");
+  const int64_t v[8] = {m->n_nodes,       m->n_elements,           m->tiles.n_tiles, m->n_slots,
+                        m->tiles.max_local, m->tiles.max_colors, m->hex_affine,    m->quad.nq};
+  for (int32_t i = 0; i < n && i < 8; ++i) out[i] = v[i];
+  return TATVA_OK;
+}
+
 #define MESH_PRELUDE(m)                                   \
   TATVA_REQUIRE((m) != nullptr, "mesh is NULL");          \
   {                                                       \
@@ -1219,13 +1227,17 @@ template <class T, int MODE>
 static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, const T *v,
                            const int32_t *dof_map, const double *f_ext, T *out, hipStream_t s,
                            const int *skip, double *part_dot = nullptr, int gather_blocks = 0,
-                           int rev = 0) {
+                           int rev = 0, hipEvent_t between = nullptr, T *lin = nullptr) {
   T *partial = (T *)m->scratch;
   return with_kind(m->kind, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
     return with_density<Elem<K>::D, T>(d, [&](auto dens) -> int {
       using Dens = decltype(dens);
       constexpr int NC = Dens::NC;
+      if constexpr ((MODE == kModeLinearize || MODE == kModeHvpLin) && Dens::kLin == 0) {
+        set_error("this density has no linearisation to cache");
+        return TATVA_EINVAL;
+      } else {
       const size_t lds = tile_lds_bytes(K, m->tiles.max_local, Elem<K>::D, Elem<K>::NNE, NC, sizeof(T), MODE);
       if (lds > kTileLdsMax) {
         set_error("mesh tiles need more LDS than a workgroup holds");
@@ -1233,10 +1245,16 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
       }
       if (K == TATVA_HEX8 && m->hex_affine)
         k_tile_assemble<K, Dens, T, MODE, true><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev);
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin);
       else
         k_tile_assemble<K, Dens, T, MODE, false><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev);
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin);
+      if constexpr (MODE == kModeLinearize) {
+        TATVA_CHECK_LAUNCH();
+        return TATVA_OK;  // the cache is the output: no gather
+      }
+      // optional event between the tile kernel and the gather (kernel timing)
+      if (between) TATVA_TRY_HIP(hipEventRecord(between, s));
       if (part_dot)
         k_slot_gather_dot<NC, T><<<gather_blocks, kBlock, 0, s>>>(
             m->n_nodes, m->t_slot_ptr, partial, dof_map, v, out, part_dot, skip);
@@ -1245,6 +1263,7 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
             m->n_nodes, m->t_slot_ptr, partial, f_ext, dof_map, out, skip);
       TATVA_CHECK_LAUNCH();
       return TATVA_OK;
+      }
     });
   });
 }
@@ -1255,8 +1274,23 @@ int launch_hvp_f64(tatva_mesh *m, const tatva_density *d, const double *u, const
 }
 
 int launch_hvp_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
-                   const int32_t *dof_map, float *y, hipStream_t s, const int *skip) {
+                   const int32_t *dof_map, float *y, hipStream_t s, const int *skip,
+                   const float *lin) {
+  if (lin)
+    return launch_assembly<float, kModeHvpLin>(m, d, u, v, dof_map, nullptr, y, s, skip, nullptr,
+                                               0, 0, nullptr, const_cast<float *>(lin));
   return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip);
+}
+
+int64_t lin_cache_floats(const tatva_mesh *m, const tatva_density *d) {
+  if (!m || !d || d->kind != TATVA_DENSITY_NEO_HOOKEAN) return 0;
+  return (int64_t)m->tiles.n_tiles * kTile * m->quad.nq * (m->dim * m->dim + 1);
+}
+
+int launch_linearize_f32(tatva_mesh *m, const tatva_density *d, const float *u, float *lin,
+                         hipStream_t s) {
+  return launch_assembly<float, kModeLinearize>(m, d, u, nullptr, nullptr, nullptr, nullptr, s,
+                                                nullptr, nullptr, 0, 0, nullptr, lin);
 }
 
 int hvp_gather_blocks(const tatva_mesh *m) {
@@ -1266,9 +1300,13 @@ int hvp_gather_blocks(const tatva_mesh *m) {
 
 int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                        const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
-                       const int *skip, int rev) {
+                       const int *skip, int rev, hipEvent_t between, const float *lin) {
+  if (lin)
+    return launch_assembly<float, kModeHvpLin>(m, d, u, v, dof_map, nullptr, y, s, skip, part_dot,
+                                               hvp_gather_blocks(m), rev, between,
+                                               const_cast<float *>(lin));
   return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip, part_dot,
-                                          hvp_gather_blocks(m), rev);
+                                          hvp_gather_blocks(m), rev, between);
 }
 
 }  // namespace tatva

@@ -416,3 +416,42 @@ extern "C" int tatva_distance2_colors(int64_t n, const int32_t *indptr, const in
   if (n_colors_out) *n_colors_out = max_color + 1;
   return TATVA_OK;
 }
+
+// Distance-2 validity of a coloring for the colored decompression: every row's
+// columns carry distinct colors (then J_compressed[row, color] holds exactly
+// one entry of the row).  Host arrays; *bad_row = the first row that has two
+// columns of one color, or -1.  O(nnz) with one color stamp per color.
+extern "C" int tatva_check_distance2_colors(int64_t n, const int32_t *indptr,
+                                            const int32_t *indices, const int32_t *colors,
+                                            int64_t *bad_row) {
+  if (n < 0 || !indptr || !colors || !bad_row || (n > 0 && !indices)) {
+    set_error("invalid arguments");
+    return TATVA_EINVAL;
+  }
+  *bad_row = -1;
+  int32_t max_color = -1;
+  for (int64_t i = 0; i < n; ++i) {
+    if (colors[i] < 0) {
+      set_error("colors must be non-negative");
+      return TATVA_EINVAL;
+    }
+    max_color = std::max(max_color, colors[i]);
+  }
+  std::vector<int64_t> stamp((size_t)max_color + 1, -1);
+  for (int64_t r = 0; r < n; ++r) {
+    for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+      const int32_t j = indices[k];
+      if (j < 0 || j >= n) {
+        set_error("column index out of range");
+        return TATVA_EINVAL;
+      }
+      int64_t &st = stamp[(size_t)colors[j]];
+      if (st == r) {
+        *bad_row = r;
+        return TATVA_OK;
+      }
+      st = r;
+    }
+  }
+  return TATVA_OK;
+}

@@ -711,21 +711,19 @@ struct CachedGraph {
 
 struct DeviceRes {
   std::mutex mu;
-  bool init = false;
+  std::atomic<bool> init{false};
   hipStream_t cap = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipEvent_t prof_a = nullptr, prof_b = nullptr;  // timing events around a sampled operator
+  hipEvent_t prof_m = nullptr;  // after the sampled operator's first kernel
   PcgState *host_state = nullptr;
   CachedGraph csr, matfree;
-  // sliced ELL copy of the CSR operator (grown on demand, kept until exit)
-  int64_t *sell_soff = nullptr;
-  size_t sell_soff_bytes = 0;
-  void *sell_data = nullptr;
-  size_t sell_data_bytes = 0;
 };
 
 static DeviceRes g_res[64];
+static std::mutex g_res_init_mu;  // serialises the first use of a device's resources
 static std::atomic<int> g_profiling{0};
+static std::atomic<int> g_lin_cache{1};  // TATVA_OPT_MATFREE_LIN_CACHE
 
 // Drops cached solver graphs that captured this mesh's arrays (called before
 // the mesh's device memory is freed).
@@ -743,30 +741,19 @@ static int get_res(int device, DeviceRes **out) {
     return TATVA_EINVAL;
   }
   DeviceRes &r = g_res[device];
-  if (!r.init) {
+  std::lock_guard<std::mutex> lock(g_res_init_mu);
+  if (!r.init.load()) {
     TATVA_TRY_HIP(hipStreamCreateWithFlags(&r.cap, hipStreamNonBlocking));
     TATVA_TRY_HIP(hipEventCreateWithFlags(&r.ev_in, hipEventDisableTiming));
     TATVA_TRY_HIP(hipEventCreateWithFlags(&r.ev_out, hipEventDisableTiming));
     TATVA_TRY_HIP(hipEventCreate(&r.prof_a));
     TATVA_TRY_HIP(hipEventCreate(&r.prof_b));
+    TATVA_TRY_HIP(hipEventCreate(&r.prof_m));
     TATVA_TRY_HIP(hipHostMalloc((void **)&r.host_state, sizeof(PcgState), hipHostMallocDefault));
-    r.init = true;
+    r.init.store(true);
   }
   *out = &r;
   return TATVA_OK;
-}
-
-static hipError_t grow(void **buf, size_t *have, size_t need) {
-  if (*have >= need) return hipSuccess;
-  if (*buf) {
-    const hipError_t e = hipFree(*buf);
-    *buf = nullptr;
-    *have = 0;
-    if (e != hipSuccess) return e;
-  }
-  const hipError_t e = hipMalloc(buf, need);
-  if (e == hipSuccess) *have = need;
-  return e;
 }
 
 struct SellView {
@@ -776,28 +763,31 @@ struct SellView {
   const double *vals = nullptr;
 };
 
-// Lays the CSR matrix out as sliced ELL in the device's buffers (one host
-// synchronisation: the padded size decides the buffer size).
-static int build_sell(DeviceRes *res, int64_t n, const int32_t *indptr, const int32_t *indices,
-                      const double *data, hipStream_t s, SellView &V) {
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Lays the CSR matrix out as sliced ELL in the caller's workspace from byte
+// offset `at` (slice offsets, values, columns); one host synchronisation reads
+// the padded size, which must fit the workspace.
+static int build_sell(int64_t n, const int32_t *indptr, const int32_t *indices,
+                      const double *data, char *ws, size_t at, size_t ws_bytes, hipStream_t s,
+                      SellView &V) {
   V.nsl = sell_slices(n);
-  TATVA_TRY_HIP(grow((void **)&res->sell_soff, &res->sell_soff_bytes,
-                     (size_t)(V.nsl + 1) * sizeof(int64_t)));
-  k_sell_len<<<grid_for(V.nsl), kBlock, 0, s>>>(n, V.nsl, indptr, res->sell_soff);
-  k_scan_inplace<<<1, kBlock, 0, s>>>(res->sell_soff, V.nsl + 1);
+  int64_t *soff = reinterpret_cast<int64_t *>(ws + at);
+  const size_t vals_at = align256(at + (size_t)(V.nsl + 1) * sizeof(int64_t));
+  TATVA_REQUIRE(vals_at <= ws_bytes, "PCG workspace too small for the sliced ELL copy");
+  k_sell_len<<<grid_for(V.nsl), kBlock, 0, s>>>(n, V.nsl, indptr, soff);
+  k_scan_inplace<<<1, kBlock, 0, s>>>(soff, V.nsl + 1);
   TATVA_CHECK_LAUNCH();
-  TATVA_TRY_HIP(hipMemcpyAsync(&V.steps, res->sell_soff + V.nsl, sizeof(int64_t),
-                               hipMemcpyDeviceToHost, s));
+  TATVA_TRY_HIP(hipMemcpyAsync(&V.steps, soff + V.nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   TATVA_TRY_HIP(hipStreamSynchronize(s));
   const size_t entries = (size_t)V.steps * kSellRows;
-  TATVA_TRY_HIP(grow(&res->sell_data, &res->sell_data_bytes,
-                     std::max<size_t>(1, entries * (sizeof(double) + sizeof(int32_t)))));
-  double *vals = (double *)res->sell_data;
-  int32_t *cols = (int32_t *)(vals + entries);
-  k_sell_fill<<<sell_grid(n), kBlock, 0, s>>>(n, V.nsl, indptr, indices, data, res->sell_soff,
-                                              cols, vals);
+  TATVA_REQUIRE(vals_at + entries * (sizeof(double) + sizeof(int32_t)) <= ws_bytes,
+                "PCG workspace too small for the sliced ELL copy");
+  double *vals = reinterpret_cast<double *>(ws + vals_at);
+  int32_t *cols = reinterpret_cast<int32_t *>(vals + entries);
+  k_sell_fill<<<sell_grid(n), kBlock, 0, s>>>(n, V.nsl, indptr, indices, data, soff, cols, vals);
   TATVA_CHECK_LAUNCH();
-  V.soff = res->sell_soff;
+  V.soff = soff;
   V.cols = cols;
   V.vals = vals;
   return TATVA_OK;
@@ -847,7 +837,6 @@ struct WsLayout {
   size_t total;
 };
 
-static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // nvec vectors of n elements of size esz, 6 partial buffers, the state record.
 static WsLayout ws_layout(int64_t n, int nvec, size_t esz, int64_t n_op_parts = 0) {
@@ -904,12 +893,15 @@ static PcgParts parts_at(char *w, const WsLayout &L) {
 // With profiling, the graph holds the chunk minus its first operator
 // application, which `sampled_op` launches eagerly on the same stream between
 // prof_a and prof_b; its duration is read after each chunk.
+// With mid_sampled, sampled_op also records prof_m after its first kernel,
+// whose duration (prof_a to prof_m) is summed into op_main_ms.
 static int run_chunks(DeviceRes *res, hipGraphExec_t exec, PcgState *st, int maxiter, int chunk,
                       bool profiling, const std::function<int()> &sampled_op,
                       tatva_pcg_stats *stats,
-                      const std::function<int()> &after_sample = std::function<int()>()) {
+                      const std::function<int()> &after_sample = std::function<int()>(),
+                      bool mid_sampled = false) {
   const int max_launch = (maxiter + chunk - 1) / chunk + 1;
-  double op_ms = 0.0;
+  double op_ms = 0.0, main_ms = 0.0;
   int samples = 0;
   bool launched = false;
   for (int l = 0; l <= max_launch + 1; ++l) {
@@ -920,6 +912,10 @@ static int run_chunks(DeviceRes *res, hipGraphExec_t exec, PcgState *st, int max
       float ms = 0.0f;
       TATVA_TRY_HIP(hipEventElapsedTime(&ms, res->prof_a, res->prof_b));
       op_ms += ms;
+      if (mid_sampled) {
+        TATVA_TRY_HIP(hipEventElapsedTime(&ms, res->prof_a, res->prof_m));
+      }
+      main_ms += ms;
       ++samples;
     }
     if (res->host_state->done) break;
@@ -945,6 +941,7 @@ static int run_chunks(DeviceRes *res, hipGraphExec_t exec, PcgState *st, int max
     stats->op_ms = op_ms;
     stats->op_samples = samples;
     stats->reserved = 0;
+    stats->op_main_ms = main_ms;
   }
   return TATVA_OK;
 }
@@ -986,6 +983,13 @@ using namespace tatva;
 extern "C" int tatva_set_profiling(int enable) {
   g_profiling.store(enable ? 1 : 0);
   return TATVA_OK;
+}
+
+extern "C" int tatva_set_option(int32_t option, int32_t value) {
+  switch (option) {
+    case TATVA_OPT_MATFREE_LIN_CACHE: g_lin_cache.store(value ? 1 : 0); return TATVA_OK;
+    default: set_error("unknown option"); return TATVA_EINVAL;
+  }
 }
 
 extern "C" int tatva_spmv_csr(int64_t n, int64_t nnz, const int32_t *indptr,
@@ -1061,8 +1065,23 @@ constexpr int kCsrVecs = 8;
 // most one block per 4 rows (64 lanes per row).
 static int64_t csr_op_parts(int64_t n) { return (n + 3) / 4; }
 
-extern "C" int64_t tatva_pcg_csr_workspace(int64_t n) {
-  return (int64_t)ws_layout(n, kCsrVecs, sizeof(double), csr_op_parts(n)).total;
+extern "C" int64_t tatva_pcg_csr_workspace(int64_t n, const int32_t *host_indptr) {
+  if (n <= 0) return -1;
+  size_t total = ws_layout(n, kCsrVecs, sizeof(double), csr_op_parts(n)).total;
+  if (host_indptr) {
+    // the sliced ELL copy: slice offsets, then every slice padded to its longest row
+    const int64_t nsl = sell_slices(n);
+    int64_t steps = 0;
+    for (int64_t sl = 0; sl < nsl; ++sl) {
+      int64_t longest = 0;
+      for (int64_t r = sl * kSellRows; r < std::min<int64_t>(n, (sl + 1) * kSellRows); ++r)
+        longest = std::max<int64_t>(longest, (int64_t)host_indptr[r + 1] - host_indptr[r]);
+      steps += longest;
+    }
+    total = align256(total + (size_t)(nsl + 1) * sizeof(int64_t));
+    total += (size_t)steps * kSellRows * (sizeof(double) + sizeof(int32_t));
+  }
+  return (int64_t)total;
 }
 
 extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
@@ -1109,7 +1128,7 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   // operator: node-block view when given, else a sliced ELL copy of the CSR
   SellView sv_ell;
   if (nblk == 1) {
-    rc = build_sell(res, n, indptr, indices, data, s, sv_ell);
+    rc = build_sell(n, indptr, indices, data, wsb, L.total, (size_t)ws_bytes, s, sv_ell);
     if (rc) return rc;
   }
   const int g_op = nblk > 1 ? spmv_grid_full(n / nblk, kBLanes) : sell_grid(n);
@@ -1210,11 +1229,11 @@ int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
                       const int32_t *dof_map, int64_t n, const float *diag, const float *b,
                       const float *x, float *r, float *p, float *Ap, float *dinv,
                       double *part_rz1, double *part_rr1, double *part_bb, int gv, double tol,
-                      int maxiter, PcgState *st, hipStream_t s) {
+                      int maxiter, PcgState *st, hipStream_t s, const float *lin) {
   k_state_reset<<<1, 64, 0, s>>>(st);
   k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
   // r = b - H x
-  const int rc = launch_hvp_f32(m, d, u, x, dof_map, Ap, s);
+  const int rc = launch_hvp_f32(m, d, u, x, dof_map, Ap, s, nullptr, lin);
   if (rc) return rc;
   k_sub<float><<<grid_for(n), kBlock, 0, s>>>(n, b, Ap, r);
   k_pcg_init<float><<<gv, kBlock, 0, s>>>(n, r, dinv, b, p, part_rz1, part_rr1, part_bb);
@@ -1226,15 +1245,20 @@ int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
 }  // namespace tatva
 
 // workspace: reduced FP32 vectors z (= D^-1 r), w, p, s, r, D^-1, x, b, D, the
-// partials and the state, then the full FP32 state u
+// partials and the state, then the full FP32 state u, then (densities with a
+// linearisation) the cache of the density's linearisation at u
 constexpr int kMatfreeVecs = 9;
 
 static size_t matfree_u_offset(int64_t n) { return ws_layout(n, kMatfreeVecs, sizeof(float)).total; }
 
-extern "C" int64_t tatva_pcg_matfree_workspace(tatva_mesh *m, int64_t n) {
-  if (!m || n <= 0) return -1;
-  const int64_t n_full = m->n_nodes * m->dim;
-  return (int64_t)(matfree_u_offset(n) + align256((size_t)n_full * sizeof(float)));
+static size_t matfree_lin_offset(const tatva_mesh *m, int64_t n) {
+  return matfree_u_offset(n) + align256((size_t)(m->n_nodes * m->dim) * sizeof(float));
+}
+
+extern "C" int64_t tatva_pcg_matfree_workspace(tatva_mesh *m, const tatva_density *d, int64_t n) {
+  if (!m || !d || n <= 0) return -1;
+  return (int64_t)(matfree_lin_offset(m, n) +
+                   align256((size_t)lin_cache_floats(m, d) * sizeof(float)));
 }
 
 extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, const float *u_in,
@@ -1245,7 +1269,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   TATVA_REQUIRE(m && d && u_in && diag_in && b_in && x_io && n > 0, "invalid arguments");
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
   const WsLayout L = ws_layout(n, kMatfreeVecs, sizeof(float));
-  TATVA_REQUIRE(workspace && ws_bytes >= tatva_pcg_matfree_workspace(m, n),
+  TATVA_REQUIRE(workspace && ws_bytes >= tatva_pcg_matfree_workspace(m, d, n),
                 "PCG workspace too small");
   const int64_t n_full = m->n_nodes * density_ncomp(m, d);
   chunk += chunk & 1;
@@ -1265,6 +1289,10 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   float *b = ws_vec<float>(wsb, L, 7);
   float *diag = ws_vec<float>(wsb, L, 8);
   float *u = reinterpret_cast<float *>(wsb + matfree_u_offset(n));
+  // the state is fixed during the solve: the density is linearised once
+  float *lin = (g_lin_cache.load() && lin_cache_floats(m, d) > 0)
+                   ? reinterpret_cast<float *>(wsb + matfree_lin_offset(m, n))
+                   : nullptr;
   const PcgParts P = parts_at(wsb, L);
   PcgState *st = ws_state(wsb, L);
   hipStream_t caller = (hipStream_t)stream;
@@ -1278,16 +1306,23 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   const int gv = step_grid(n, step_prefetch<float>());
   const bool profiling = g_profiling.load() != 0;
   // r = b - H x, z = D^-1 r, shares of r.z, r.r, b.b (w is scratch here)
+  if (lin) {
+    rc = launch_linearize_f32(m, d, u, lin, s);
+    if (rc) return rc;
+  }
   rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, r, z, w, dinv, P.rz(1), P.rr(1), P.bb,
-                         gv, tol, maxiter, st, s);
+                         gv, tol, maxiter, st, s, lin);
   if (rc) return rc;
   const int g_op = hvp_gather_blocks(m);
   // KA: w = H z with the shares of z.w; the tiles run in alternating order
   // (iteration parity), as the CSR SpMV streams its matrix
   const std::function<int(int)> apply_op = [&](int q) -> int {
-    return launch_hvp_dot_f32(m, d, u, z, dof_map, w, P.pap, s, &st->done, q);
+    return launch_hvp_dot_f32(m, d, u, z, dof_map, w, P.pap, s, &st->done, q, nullptr, lin);
   };
-  const std::function<int()> sampled_op = [&]() -> int { return apply_op(0); };
+  // the sampled operator also times its tile kernel alone (prof_a to prof_m)
+  const std::function<int()> sampled_op = [&]() -> int {
+    return launch_hvp_dot_f32(m, d, u, z, dof_map, w, P.pap, s, &st->done, 0, res->prof_m, lin);
+  };
   const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
     if (with_op) {
       const int e = apply_op(q);
@@ -1306,12 +1341,14 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   key.i0 = d->kind;
   key.i1 = chunk;
   key.i2 = profiling ? 1 : 0;
+  key.i3 = lin ? 1 : 0;
   key.f0 = d->p0;
   key.f1 = d->p1;
   key.f2 = d->p2;
   rc = chunk_graph(s, chunk, step, profiling, key, res->matfree);
   if (rc == TATVA_OK)
-    rc = run_chunks(res, res->matfree.exec, st, maxiter, chunk, profiling, sampled_op, stats);
+    rc = run_chunks(res, res->matfree.exec, st, maxiter, chunk, profiling, sampled_op, stats,
+                    std::function<int()>(), true);
   if (rc == TATVA_OK)
     TATVA_TRY_HIP(hipMemcpyAsync(x_io, x, n * sizeof(float), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
@@ -1640,24 +1677,20 @@ extern "C" int tatva_cast_f32_f64(int64_t n, double alpha, const float *x, doubl
   return TATVA_OK;
 }
 
-extern "C" int tatva_dot(int64_t n, const double *x, const double *y, double *out, void *stream) {
+extern "C" int64_t tatva_dot_workspace(int64_t n) {
+  if (n <= 0) return -1;
+  return (int64_t)reduce_grid(n) * (int64_t)sizeof(double);
+}
+
+extern "C" int tatva_dot(int64_t n, const double *x, const double *y, double *out,
+                         void *workspace, int64_t workspace_bytes, void *stream) {
   TATVA_REQUIRE(n > 0 && x && y && out, "invalid arguments");
-  // The partials live in a small per-device buffer owned by the library
-  // (allocated on the first call on a device, kept until the process ends).
-  static double *partials[64] = {nullptr};
-  static std::mutex mu;
-  int device = 0;
-  TATVA_TRY_HIP(hipGetDevice(&device));
-  TATVA_REQUIRE(device >= 0 && device < 64, "device id out of range");
-  {
-    std::lock_guard<std::mutex> lock(mu);
-    if (!partials[device])
-      TATVA_TRY_HIP(hipMalloc(&partials[device], kMaxReduceBlocks * sizeof(double)));
-  }
+  TATVA_REQUIRE(workspace && workspace_bytes >= tatva_dot_workspace(n), "dot workspace too small");
   hipStream_t s = (hipStream_t)stream;
   const int g = reduce_grid(n);
-  k_dot_part<<<g, kBlock, 0, s>>>(n, x, y, partials[device]);
-  k_finalize_sum<double><<<1, kBlock, 0, s>>>(partials[device], g, out, 0);
+  double *partials = static_cast<double *>(workspace);
+  k_dot_part<<<g, kBlock, 0, s>>>(n, x, y, partials);
+  k_finalize_sum<double><<<1, kBlock, 0, s>>>(partials, g, out, 0);
   TATVA_CHECK_LAUNCH();
   return TATVA_OK;
 }

@@ -102,6 +102,15 @@ class Operator:
             object.__setattr__(self, "_dev", _DeviceMesh(self.mesh, self.element))
         return self._dev.handle
 
+    def device_layout(self) -> dict:
+        """Sizes of the mesh's device layout (tiles of elements and their
+        (tile, local node) slots), for the kernels' byte counts."""
+        out = (ctypes.c_int64 * 8)()
+        nat.check(nat.lib().tatva_mesh_info(self.handle, out, 8))
+        keys = ("n_nodes", "n_elements", "n_tiles", "n_slots", "max_local", "max_colors",
+                "hex_affine", "n_quad")
+        return {k: int(v) for k, v in zip(keys, out)}
+
     @property
     def n_elements(self) -> int:
         return self.mesh.n_elements

@@ -307,10 +307,9 @@ class _Job:
 
 
 def _norm(x) -> float:
-    t = nat.torch()
-    out = t.empty((), dtype=t.float64, device=x.device)
-    nat.check(nat.lib().tatva_dot(x.numel(), nat.dptr(x), nat.dptr(x), nat.dptr(out), nat.stream()))
-    return math.sqrt(max(float(out.item()), 0.0))
+    from .solvers import norm  # noqa: PLC0415
+
+    return norm(x)
 
 
 @dataclass

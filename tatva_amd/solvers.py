@@ -22,6 +22,8 @@ import math
 import time
 from dataclasses import dataclass, field, replace
 
+import numpy as np
+
 from . import _native as nat
 from .energies import Energy, ReducedEnergy
 from .sparse import ColoredMatrix, _HessianAssembler
@@ -35,6 +37,7 @@ class PcgInfo:
     b_norm: float
     op_ms: float = 0.0        # profiling: summed duration of sampled operator launches
     op_samples: int = 0
+    op_main_ms: float = 0.0   # profiling: the same launches' main kernel (SpMV / HVP tile kernel)
 
     @property
     def converged(self) -> bool:
@@ -43,7 +46,7 @@ class PcgInfo:
 
 def _stats_to_info(st: nat.PcgStats) -> PcgInfo:
     return PcgInfo(int(st.iterations), float(st.rel_residual), int(st.status), float(st.b_norm),
-                   float(st.op_ms), int(st.op_samples))
+                   float(st.op_ms), int(st.op_samples), float(st.op_main_ms))
 
 
 class _Workspace:
@@ -76,11 +79,17 @@ def pcg(A: ColoredMatrix, b, x0=None, *, tol: float = 1e-10, maxiter: int | None
         nat.to_device(x0).reshape(-1).clone()
     maxiter = 10 * n if maxiter is None else int(maxiter)
     L = nat.lib()
-    nbytes = L.tatva_pcg_csr_workspace(n)
-    ws = (workspace or _WS_CSR).get(nbytes)
-    st = nat.PcgStats()
     view = A.block_view(block_size) if block_size else None
     bc, bs = (view[0], view[1]) if view is not None else (None, 1)
+    key = ("pcg_csr_workspace", view is None)
+    nbytes = A._cache.get(key)
+    if nbytes is None:
+        # without a block view the workspace also holds the sliced-ELL copy
+        hip = None if view is not None else np.ascontiguousarray(A.host_indptr(), dtype=np.int32)
+        nbytes = int(L.tatva_pcg_csr_workspace(n, None if hip is None else hip.ctypes.data))
+        A._cache[key] = nbytes
+    ws = (workspace or _WS_CSR).get(nbytes)
+    st = nat.PcgStats()
     nat.check(L.tatva_pcg_csr(n, A.nnz, nat.dptr(A.indptr), nat.dptr(A.indices), nat.dptr(A.data),
                               nat.dptr(bc), bs, nat.dptr(bv), nat.dptr(x), float(tol), maxiter,
                               int(chunk), nat.dptr(ws), ws.numel(), ctypes.byref(st),
@@ -128,7 +137,7 @@ def pcg_matfree(energy: ReducedEnergy, u_full32, b32, x0=None, *, diag32=None, t
             float(tol), maxiter, int(chunk), nat.dptr(ws), ws.numel(), ctypes.byref(st),
             nat.stream()))
         return x, _stats_to_info(st)
-    nbytes = L.tatva_pcg_matfree_workspace(op.handle, n)
+    nbytes = L.tatva_pcg_matfree_workspace(op.handle, ctypes.byref(desc), n)
     ws = (workspace or _Workspace()).get(nbytes)
     nat.check(L.tatva_pcg_matfree_f32(op.handle, ctypes.byref(desc), nat.dptr(u),
                                       nat.dptr(lf.dof_map_device()), n,
@@ -143,12 +152,31 @@ def set_profiling(enable: bool) -> None:
     nat.check(nat.lib().tatva_set_profiling(1 if enable else 0))
 
 
+OPT_MATFREE_LIN_CACHE = 1
+
+
+def set_linearization_cache(enable: bool) -> None:
+    """Matrix-free FP32 PCG: linearise the density once per solve (default) or
+    in every Hessian-vector product (same result)."""
+    nat.check(nat.lib().tatva_set_option(OPT_MATFREE_LIN_CACHE, 1 if enable else 0))
+
+
+def dot(x, y):
+    """x . y (FP64 device scalar) with the deterministic reduction kernel; its
+    partial sums go to a workspace allocated on the current stream."""
+    t = nat.torch()
+    L = nat.lib()
+    n = x.numel()
+    out = t.empty((), dtype=t.float64, device=x.device)
+    ws = t.empty(int(L.tatva_dot_workspace(n)), dtype=t.uint8, device=x.device)
+    nat.check(L.tatva_dot(n, nat.dptr(x), nat.dptr(y), nat.dptr(out), nat.dptr(ws), ws.numel(),
+                          nat.stream()))
+    return out
+
+
 def norm(x) -> float:
     """Euclidean norm with the deterministic FP64 reduction kernel."""
-    t = nat.torch()
-    out = t.empty((), dtype=t.float64, device=x.device)
-    nat.check(nat.lib().tatva_dot(x.numel(), nat.dptr(x), nat.dptr(x), nat.dptr(out), nat.stream()))
-    return math.sqrt(max(float(out.item()), 0.0))
+    return math.sqrt(max(float(dot(x, x).item()), 0.0))
 
 
 @dataclass
@@ -162,6 +190,98 @@ class NewtonStats:
     assembly_seconds: float = 0.0
     op_ms: float = 0.0
     op_samples: int = 0
+    op_main_ms: float = 0.0
+
+
+def padded_node_layout(mesh, lifter, nc):
+    """Host part of the whole-node layout (``_PaddedNodes``): the lifter that
+    removes only nodes with no free dof, its pattern S (node-blocked, sorted),
+    each reduced dof's index in it (red_to_pad), the positions of S's entries
+    in rows or columns of constrained dofs (off the diagonal: zeroed) and of
+    their diagonal entries (set to 1).  None when the reduced pattern is
+    node-blocked already."""
+    from .lifter import Fixed, Lifter  # noqa: PLC0415
+    from .sparse import pattern_from_mesh  # noqa: PLC0415
+
+    free = lifter.free_dofs
+    nodes = np.unique(free // nc)
+    pad_dofs = (nodes[:, None] * nc + np.arange(nc)).ravel()
+    if pad_dofs.size == free.size:
+        return None
+    mask = np.ones(lifter.size, dtype=bool)
+    mask[pad_dofs] = False
+    L_pad = Lifter(lifter.size, Fixed(np.nonzero(mask)[0], 0.0))
+    S = L_pad.adapt_sparsity(pattern_from_mesh(mesh, nc))
+    S.sort_indices()
+    ip, ix = S.indptr.astype(np.int64), S.indices
+    extra = np.nonzero(~np.isin(pad_dofs, free))[0]
+    lens = ip[extra + 1] - ip[extra]
+    row = np.repeat(extra, lens)                                      # entries (d, i)
+    pos = np.repeat(ip[extra] - np.cumsum(np.r_[0, lens[:-1]]), lens) + np.arange(lens.sum())
+    col = ix[pos].astype(np.int64)
+    on = col == row
+    diag_pos = pos[on]
+    # the symmetric entry (i, d) of each off-diagonal (d, i): bisection in row i
+    i, d = col[~on], row[~on]
+    lo, hi = ip[i], ip[i + 1]
+    while np.any(lo < hi):
+        mid = (lo + hi) // 2
+        go = (lo < hi) & (ix[np.minimum(mid, ix.size - 1)] < d)
+        lo, hi = np.where(go, mid + 1, lo), np.where(go | (lo >= hi), hi, mid)
+    zero_pos = np.unique(np.concatenate([pos[~on], lo]))
+    red_to_pad = np.searchsorted(pad_dofs, free)
+    return L_pad, S, red_to_pad, zero_pos, diag_pos
+
+
+class _PaddedNodes:
+    """Whole-node layout of the FP64 Newton system under Fixed constraints on
+    single node components (e.g. u_x prescribed on a face: those nodes keep
+    their other components).  The reduced Hessian S[free][:, free]
+    (lifter/base.py:322-331) then has rows of 2 and 3 dofs per node and no node
+    blocks.  Here the system is solved on every dof of each node that has a
+    free one: the Hessian on that whole-node pattern, with the rows and columns
+    of the constrained dofs among them zeroed and a unit diagonal,
+
+        [K_ff  0] [du_f]   [r_f]
+        [ 0    I] [du_c] = [ 0 ]
+
+    so du_c = 0 and du_f solves K_ff du_f = r_f, the reduced Newton system.  CG
+    on it builds the same Krylov space (the constrained entries of every vector
+    stay exactly 0), and the node-block SpMV streams 8 bytes per value plus one
+    int32 per 3x3 block instead of the 12 bytes per value of a sliced-ELL copy
+    of the reduced matrix."""
+
+    def __init__(self, lifter, cm, red_to_pad, zero_pos, diag_pos):
+        self.lifter, self.cm = lifter, cm
+        self.red_to_pad, self.zero_pos, self.diag_pos = red_to_pad, zero_pos, diag_pos
+
+    @classmethod
+    def build(cls, energy, lifter, nc):
+        from .sparse import ColoredMatrix  # noqa: PLC0415
+
+        lay = padded_node_layout(energy.op.mesh, lifter, nc)
+        if lay is None:
+            return None          # the reduced matrix is node-blocked already
+        L_pad, S, red_to_pad, zero_pos, diag_pos = lay
+        t = nat.torch()
+        cm = ColoredMatrix.from_csr(S)
+        return cls(L_pad, cm, nat.to_device(red_to_pad, t.int64),
+                   nat.to_device(zero_pos, t.int64), nat.to_device(diag_pos, t.int64))
+
+    def assemble(self, asm, u_full, data):
+        data = asm.assemble_full_state(u_full, self.lifter, data)
+        data.index_fill_(0, self.zero_pos, 0.0)
+        data.index_fill_(0, self.diag_pos, 1.0)
+        return data
+
+    def spread(self, r):
+        t = nat.torch()
+        out = t.zeros(self.cm.shape[0], dtype=r.dtype, device=r.device)
+        out.index_copy_(0, self.red_to_pad, r)
+        return out
+
+    def gather(self, x):
+        return x.index_select(0, self.red_to_pad)
 
 
 class Newton:
@@ -190,16 +310,28 @@ class Newton:
         self._cm = colored_matrix
         self._asm = None
         self._ws = _Workspace()
+        self._pad = None
         if mode == "csr":
-            if colored_matrix is None:
-                from .sparse import ColoredMatrix, pattern_from_mesh  # noqa: PLC0415
-                pat = lifter.adapt_sparsity(pattern_from_mesh(energy.op.mesh, energy.ncomp))
-                colored_matrix = ColoredMatrix.from_csr(pat)
-            self._cm = colored_matrix
-            self._asm = _HessianAssembler(self.red.grad, colored_matrix)
+            if colored_matrix is None and self.block_size and not lifter.has_periodic:
+                # constraints on single components of a node would break the
+                # node blocks of the reduced matrix: solve on whole nodes
+                self._pad = _PaddedNodes.build(energy, lifter, self.block_size)
+            if self._pad is not None:
+                self._cm = self._pad.cm
+                self._asm = _HessianAssembler(energy.reduced(self._pad.lifter).grad, self._pad.cm)
+            else:
+                if colored_matrix is None:
+                    from .sparse import ColoredMatrix, pattern_from_mesh  # noqa: PLC0415
+                    pat = lifter.adapt_sparsity(pattern_from_mesh(energy.op.mesh, energy.ncomp))
+                    colored_matrix = ColoredMatrix.from_csr(pat)
+                self._cm = colored_matrix
+                self._asm = _HessianAssembler(self.red.grad, colored_matrix)
 
     @property
     def colored_matrix(self):
+        """The matrix the FP64 PCG solves with: the reduced Hessian's pattern, or
+        with constrained single components the whole-node pattern (see
+        ``_PaddedNodes``)."""
         return self._cm
 
     def _check_layout(self, lifter) -> None:
@@ -209,8 +341,6 @@ class Newton:
         cur = self.red.lifter
         if lifter is cur or lifter._layout is cur._layout:
             return
-        import numpy as np  # noqa: PLC0415
-
         if (lifter.size != cur.size or lifter.size_reduced != cur.size_reduced
                 or not np.array_equal(lifter.read_map, cur.read_map)
                 or not np.array_equal(lifter.dof_map, cur.dof_map)):
@@ -249,18 +379,26 @@ class Newton:
             stats.newton_iterations += 1
             if self.mode == "csr":
                 ta = time.perf_counter()
-                data = self._asm.assemble_full_state(u_full, lf, data)
+                pad = self._pad
+                if pad is None:
+                    data = self._asm.assemble_full_state(u_full, lf, data)
+                else:
+                    data = pad.assemble(self._asm, u_full, data)
                 K = replace(self._cm, data=data)
                 t.cuda.synchronize()
                 tb = time.perf_counter()
                 stats.assembly_seconds += tb - ta
                 # K du = r, then u <- u - du (the Newton step -K^-1 r)
-                du, info = pcg(K, r, tol=self.lin_tol, maxiter=maxiter, chunk=self.chunk,
+                rhs = r if pad is None else pad.spread(r)
+                du, info = pcg(K, rhs, tol=self.lin_tol, maxiter=maxiter, chunk=self.chunk,
                                workspace=self._ws, block_size=self.block_size)
+                if pad is not None:
+                    du = pad.gather(du)
                 stats.cg_seconds += time.perf_counter() - tb
                 stats.cg_iterations += info.iterations
                 stats.op_ms += info.op_ms
                 stats.op_samples += info.op_samples
+                stats.op_main_ms += info.op_main_ms
                 if info.status == nat.EBREAKDOWN:
                     break
                 nat.check(L.tatva_axpy_free(n, nat.dptr(free), -1.0, nat.dptr(du),
@@ -280,6 +418,7 @@ class Newton:
                 stats.cg_iterations += info.iterations
                 stats.op_ms += info.op_ms
                 stats.op_samples += info.op_samples
+                stats.op_main_ms += info.op_main_ms
                 if info.status == nat.EBREAKDOWN:
                     break
                 nat.check(L.tatva_axpy_free_f32(n, nat.dptr(free), 1.0, nat.dptr(du),

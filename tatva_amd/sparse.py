@@ -146,6 +146,14 @@ class ColoredMatrix:
                 raise ValueError(f"colors must have {n} entries")
             if colors.size and colors.min() < 0:
                 raise ValueError("colors must be non-negative")
+            # the decompression data[k] = J_c[row, colors[indices[k]]] is exact only
+            # when no row has two columns of one color (a distance-2 coloring)
+            bad = ctypes.c_int64(-1)
+            nat.check(nat.lib().tatva_check_distance2_colors(
+                n, ip.ctypes.data, ix.ctypes.data, colors.ctypes.data, ctypes.byref(bad)))
+            if bad.value >= 0:
+                raise ValueError(f"colors are not a distance-2 coloring of the pattern: row "
+                                 f"{bad.value} has two columns of one color")
         t = nat.torch()
         return cls(
             data=nat.to_device(np.asarray(S.data, dtype=np.float64)),
@@ -162,6 +170,12 @@ class ColoredMatrix:
     def nnz(self) -> int:
         return int(self._host_indices.shape[0]) if self._host_indices is not None \
             else int(self.indices.shape[0])
+
+    def host_indptr(self) -> np.ndarray:
+        """indptr on the host (kept from construction, else copied once)."""
+        if self._host_indptr is None:
+            object.__setattr__(self, "_host_indptr", nat_host(self.indptr).astype(np.int32))
+        return self._host_indptr
 
     def to_csr(self):
         """SciPy CSR copy on the host (sparse/base.py:76-81)."""

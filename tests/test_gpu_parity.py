@@ -377,8 +377,10 @@ def test_pcg_graph_path_both_layouts(cells, n_expected):
     assert float(torch.linalg.norm(b2 - K2.matvec(x2)) / torch.linalg.norm(b2)) <= 2e-10
 
 
-def test_newton_poisson_tri3_matches_oracle():
-    n = 16
+@pytest.mark.parametrize("n", [16, 64], ids=["16x16", "c1-64x64"])
+def test_newton_poisson_tri3_matches_oracle(n):
+    """C1 (Tri3 Poisson, f = 1, u = 0 on the boundary) at 16^2 and at its full
+    64^2 size: one Newton step (quadratic energy), solution vs the oracle's."""
     m = ta.Mesh.unit_square(n, n)
     op = ta.Operator(m, ta.Tri3())
     E = op.energy(energies.Poisson(1.0))
@@ -398,8 +400,11 @@ def test_newton_poisson_tri3_matches_oracle():
 
 
 def test_newton_mixed_precision_neohookean():
-    """FP32 matrix-free CG inside FP64 Newton reaches 1e-8 relative residual and the
-    FP64 colored-CSR Newton solution within 1e-6 relative."""
+    """FP32 matrix-free CG inside FP64 Newton on a Hex8 Neo-Hookean cube, two load
+    steps set through lifter.at[key].set: the oracle's FP64 residual at the GPU
+    state is <= 1e-8 of the oracle residual at the step's start, and the state
+    is the oracle Newton solution of the same load to 1e-6 relative (both stop
+    at a relative residual; their states differ by the conditioning times it)."""
     t = __import__("torch")
     m = ta.Mesh.unit_cube(5)
     op = ta.Operator(m, ta.Hexahedron8())
@@ -409,16 +414,25 @@ def test_newton_mixed_precision_neohookean():
     fixed = np.concatenate([(x0n[:, None] * 3 + np.arange(3)).ravel(), x1n * 3])
     L = lifter.Lifter(m.n_nodes * 3, lifter.Fixed(fixed, lifter.RuntimeValue("s")))
     vals = lambda s: np.concatenate([np.zeros(x0n.size * 3), np.full(x1n.size, s)])  # noqa: E731
+    prob = fem.Problem(fem.Hex8(), m.coords, m.elements, fem.NeoHookean(1.0, 10.0))
+    So = fem.Lifter(m.n_nodes * 3, fem.Fixed(fixed)).adapt_sparsity(
+        fem.pattern_from_mesh(m.elements, m.n_nodes, 3))
+    co = fem.distance2_colors(So.indptr, So.indices, So.shape[0])
     um = t.zeros(m.n_nodes * 3, dtype=t.float64, device="cuda")
-    uc = um.clone()
     mixed = solvers.Newton(E, L.at["s"].set(vals(0.0)), mode="mixed", tol=1e-8, inner_tol=1e-3)
-    csr = solvers.Newton(E, L.at["s"].set(vals(0.0)), mode="csr", tol=1e-10, lin_tol=1e-12)
     for s in (0.05, 0.1):
+        Lo = fem.Lifter(m.n_nodes * 3, fem.Fixed(fixed, vals(s)))
+        start = to_np(um)
+        start = Lo.lift(Lo.reduce(start), start)
         st_m = mixed.solve(um, L.at["s"].set(vals(s)))
-        st_c = csr.solve(uc, L.at["s"].set(vals(s)))
-        assert st_m.converged and st_c.converged
+        assert st_m.converged
         assert st_m.residual_history[-1] <= 1e-8 * st_m.residual_history[0]
-    assert rel_err(to_np(um), to_np(uc)) < 1e-6
+        uo, _, _, hist = fem.newton(prob, Lo, start, So, co, tol=1e-11, lin_tol=1e-12)
+        assert hist[-1] <= 1e-10 * hist[0]
+        ug = to_np(um)
+        r0 = np.linalg.norm(Lo.reduce_adjoint(prob.residual(start)))
+        assert np.linalg.norm(Lo.reduce_adjoint(prob.residual(ug))) <= 1e-8 * r0
+        assert rel_err(ug, uo) < 1e-6
 
 
 # ------------------------------------------------------------- periodic ----

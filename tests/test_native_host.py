@@ -308,3 +308,76 @@ def test_newton_rejects_a_lifter_with_another_layout():
                   lifter.Lifter(15, lifter.Fixed([0, 1, 2, 9], 0.0))):
         with pytest.raises(ValueError, match="constrains other dofs"):
             nw._check_layout(other)
+
+
+def test_distance2_color_check_matches_the_oracle():
+    """tatva_check_distance2_colors (the validation of caller colors in
+    ColoredMatrix.from_csr) against the oracle's check: the greedy and the
+    largest-degree-first colorings pass, a coloring with one color changed to
+    that of another column of the same row is reported at the first such row,
+    and from_csr raises ValueError for it before any upload."""
+    import ctypes
+
+    X, el = fem.unit_square_tri3(6, 5)
+    S = fem.pattern_from_mesh(el, X.shape[0], 2)
+    n = S.shape[0]
+    ip = np.ascontiguousarray(S.indptr, dtype=np.int32)
+    ix = np.ascontiguousarray(S.indices, dtype=np.int32)
+
+    def check(colors):
+        c = np.ascontiguousarray(colors, dtype=np.int32)
+        bad = ctypes.c_int64(0)
+        nat.check(nat.lib().tatva_check_distance2_colors(n, ip.ctypes.data, ix.ctypes.data,
+                                                          c.ctypes.data, ctypes.byref(bad)))
+        return bad.value
+
+    greedy = fem.distance2_colors(S.indptr, S.indices, n)
+    ldf = fem.largest_degree_first_distance2_colors(S.indptr, S.indices, n)
+    assert check(greedy) == -1 and check(ldf) == -1
+    bad = greedy.copy()
+    r = 7
+    bad[S.indices[S.indptr[r] + 2]] = bad[S.indices[S.indptr[r]]]
+    first = next(i for i in range(n) if not fem.is_distance2_coloring(
+        S.indptr[i:i + 2] - S.indptr[i], S.indices[S.indptr[i]:S.indptr[i + 1]], bad))
+    assert check(bad) == first
+    with pytest.raises(ValueError, match="not a distance-2 coloring"):
+        sparse.ColoredMatrix.from_csr(S, colors=bad)
+
+
+def test_padded_node_layout_matches_brute_force():
+    """The whole-node layout of the FP64 Newton system (u_x prescribed on x = 1,
+    x = 0 clamped): its pattern is node-blocked and is the pattern of the dofs of
+    every node with a free dof; the zeroed positions are exactly the
+    off-diagonal entries in rows or columns of constrained dofs and the unit
+    ones their diagonal (brute force over the pattern's entries); each reduced
+    dof maps to its own dof."""
+    from tatva_amd.solvers import padded_node_layout
+
+    m = ta.Mesh.unit_cube(4, type="tetrahedron", jitter=0.2, seed=0)
+    x0 = np.where(np.isclose(m.coords[:, 0], 0))[0]
+    x1 = np.where(np.isclose(m.coords[:, 0], 1))[0]
+    dofs = np.concatenate([(x0[:, None] * 3 + np.arange(3)).ravel(), x1 * 3])
+    L = lifter.Lifter(m.n_nodes * 3, lifter.Fixed(dofs, 0.0))
+    L_pad, S, red_to_pad, zero_pos, diag_pos = padded_node_layout(m, L, 3)
+    pad_dofs = L_pad.free_dofs
+    np.testing.assert_array_equal(np.unique(pad_dofs // 3), np.unique(L.free_dofs // 3))
+    assert pad_dofs.size == 3 * np.unique(L.free_dofs // 3).size
+    np.testing.assert_array_equal(pad_dofs[red_to_pad], L.free_dofs)
+    ip, ix = S.indptr, S.indices
+    for k in range(S.shape[0] // 3):
+        for j in (1, 2):
+            np.testing.assert_array_equal(ix[ip[3 * k]:ip[3 * k + 1]],
+                                          ix[ip[3 * k + j]:ip[3 * k + j + 1]])
+    So = fem.Lifter(L.size, fem.Fixed(np.setdiff1d(np.arange(L.size), pad_dofs))).adapt_sparsity(
+        fem.pattern_from_mesh(m.elements, m.n_nodes, 3))
+    np.testing.assert_array_equal(ip, So.indptr)
+    np.testing.assert_array_equal(ix, So.indices)
+    con = set(np.nonzero(~np.isin(pad_dofs, L.free_dofs))[0].tolist())
+    rows = np.repeat(np.arange(S.shape[0]), np.diff(ip))
+    want_zero = [k for k, (r, c) in enumerate(zip(rows, ix)) if r != c and (r in con or c in con)]
+    want_diag = [k for k, (r, c) in enumerate(zip(rows, ix)) if r == c and r in con]
+    np.testing.assert_array_equal(zero_pos, want_zero)
+    np.testing.assert_array_equal(np.sort(diag_pos), want_diag)
+    # nothing to pad when whole nodes are constrained
+    assert padded_node_layout(m, lifter.Lifter(m.n_nodes * 3, lifter.Fixed(
+        (x0[:, None] * 3 + np.arange(3)).ravel(), 0.0)), 3) is None

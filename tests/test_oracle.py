@@ -514,3 +514,52 @@ def test_batched_polygon_location_equals_the_loop():
     got = fem.find_containing_polygons_batched(pts, X[E], chunk=17)
     np.testing.assert_array_equal(got, want)
     assert (want == -1).any() and (want >= 0).sum() > 150
+
+
+@pytest.mark.parametrize("kind", ["tet4", "hex8"])
+def test_openmp_assembly_matches_numpy_oracle(kind):
+    """oracle/fem_omp.c (the CPU baseline's element kernels, OpenMP over
+    elements) against Problem's residual, HVP and Hessian diagonal on jittered
+    meshes, both elastic densities, 1e-12 relative; the unsupported cases raise."""
+    if kind == "tet4":
+        X, el = fem.box_tet4(4, 3, 5, jitter=0.2, seed=1)
+        elem = fem.Tet4()
+    else:
+        X, el = fem.cube_hex8(4)
+        rng = np.random.default_rng(3)
+        inner = ~(np.isclose(X, 0) | np.isclose(X, 1)).any(1)
+        X = X + 0.05 * rng.uniform(-1, 1, X.shape) * inner[:, None]
+        elem = fem.Hex8()
+    for dens in (fem.LinearElasticity(0.4, 0.6), fem.NeoHookean(1.0, 10.0)):
+        P = fem.Problem(elem, X, el, dens)
+        Q = fem.ProblemOMP(elem, X, el, dens, threads=3)
+        rng = np.random.default_rng(0)
+        u = 0.02 * rng.standard_normal(P.n)
+        v = rng.standard_normal(P.n)
+        for a, b in ((P.residual(u), Q.residual(u)), (P.hvp(u, v), Q.hvp(u, v)),
+                     (P.hessian_diag(u), Q.hessian_diag(u))):
+            assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    with pytest.raises(NotImplementedError):
+        fem.ProblemOMP(fem.Tri3(), *fem.unit_square_tri3(2, 2), fem.Poisson(1.0))
+
+
+def test_ldf_coloring_is_a_valid_non_greedy_distance2_coloring():
+    """The restated largest-degree-first coloring (_coloring.py:156-186,286-300)
+    is a distance-2 coloring, differs from the natural-order greedy one, and the
+    validity check rejects a coloring with one color changed to a neighbour's."""
+    X, el = fem.unit_square_tri3(5, 4)
+    S = fem.pattern_from_mesh(el, X.shape[0], 2)
+    n = S.shape[0]
+    ldf = fem.largest_degree_first_distance2_colors(S.indptr, S.indices, n)
+    greedy = fem.distance2_colors_py(S.indptr, S.indices, n)
+    assert fem.is_distance2_coloring(S.indptr, S.indices, ldf)
+    assert fem.is_distance2_coloring(S.indptr, S.indices, greedy)
+    assert not np.array_equal(ldf, greedy)
+    # brute force: a distance-2 coloring gives different colors to any two columns of a row
+    A = S.toarray() != 0
+    for r in range(n):
+        cols = np.nonzero(A[r])[0]
+        assert len(set(ldf[cols])) == cols.size
+    bad = greedy.copy()
+    bad[S.indices[S.indptr[3] + 1]] = bad[S.indices[S.indptr[3]]]
+    assert not fem.is_distance2_coloring(S.indptr, S.indices, bad)
