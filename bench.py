@@ -210,7 +210,7 @@ def c1_latency(reps=20):
             "cg_iterations": its, "n_free": solver.red.size}
 
 
-def serve_stream(count, qps, seed=0, max_slots=64, chunk=8):
+def serve_stream(count, qps, seed=0, max_slots=64, chunk=8, hbm_budget=200e9):
     """Continuous-batching server on the mixed Hex8 LE / Neo-Hookean request
     stream (n log-uniform in [8, 48]); first an untimed warm-up request of
     each material at a size inside the stream's range, so that one-time
@@ -219,7 +219,7 @@ def serve_stream(count, qps, seed=0, max_slots=64, chunk=8):
     in neither path's first request."""
     from tatva_amd import serve
 
-    srv = serve.Server(max_slots=max_slots, chunk=chunk)
+    srv = serve.Server(max_slots=max_slots, chunk=chunk, hbm_budget=hbm_budget)
     srv.run([serve.SolveRequest(rid=-2, n=40, material="linear_elastic", E=1.0, nu=0.3),
              serve.SolveRequest(rid=-1, n=40, material="neo_hookean", E=1.0, nu=0.3,
                                 load_steps=2)], realtime=False)
@@ -228,7 +228,7 @@ def serve_stream(count, qps, seed=0, max_slots=64, chunk=8):
     srv.close()
     s = rep.summary()
     s.update({
-        "qps": qps, "max_slots": max_slots, "chunk": chunk,
+        "qps": qps, "max_slots": max_slots, "chunk": chunk, "hbm_budget_gb": hbm_budget / 1e9,
         "workload": f"{count} requests, Poisson arrivals at {qps}/s, hex8 cubes n in [8, 48] "
                     "log-uniform, E ~ LogNormal(0, 0.5), nu ~ U(0.2, 0.45), linear elastic "
                     "(traction 1e-3) / Neo-Hookean (2 load steps of u_x += 0.03) 50/50, "
@@ -236,7 +236,21 @@ def serve_stream(count, qps, seed=0, max_slots=64, chunk=8):
         "total_dofs": int(sum(r.n_dofs for r in reqs)),
         "all_converged": all(r.converged for r in rep.results),
     })
+    print(f"[serve] {count} requests at {qps}/s, budget {hbm_budget / 1e9:.2f} GB: "
+          f"p50 {s['p50_ms']:.1f} ms, p99 {s['p99_ms']:.1f} ms, {s['throughput_req_per_s']:.1f} "
+          f"req/s, peak {s['peak_admitted']} slots", file=sys.stderr, flush=True)
     return s
+
+
+def serve_sweep(args):
+    """The serving stream at each QPS of --serve-qps (--serve-requests requests
+    each, 200 GB budget), then once more at the highest QPS under the binding
+    budget --serve-budget-gb (admissions wait for memory)."""
+    out = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk) for q in args.serve_qps]
+    if args.serve_budget_gb > 0:
+        out.append(serve_stream(args.serve_requests, max(args.serve_qps), chunk=args.serve_chunk,
+                                hbm_budget=args.serve_budget_gb * 1e9))
+    return out
 
 
 def cpu_baseline(K_host, b_host, seconds=12.0, max_iters=400):
@@ -265,38 +279,6 @@ def cpu_baseline(K_host, b_host, seconds=12.0, max_iters=400):
         it += 1
     dt = time.perf_counter() - t0
     return A.shape[0] * it / dt, it, dt
-
-
-def chunked_oracle_residuals(elem, coords, elements, dens, vecs, chunk=262144):
-    """The oracle's FP64 residuals of several states of a large mesh, accumulated
-    over element chunks (the chunk's geometry is computed once for all states)."""
-    import numpy as np
-
-    from oracle import fem  # the checker only
-
-    out = [np.zeros(v.size) for v in vecs]
-    for a in range(0, elements.shape[0], chunk):
-        prob = fem.Problem(elem, coords, elements[a:a + chunk], dens)
-        for r, v in zip(out, vecs):
-            r += prob.residual(v)
-        print(f"[oracle] residuals: {min(a + chunk, elements.shape[0])}/{elements.shape[0]} "
-              "elements", file=sys.stderr, flush=True)
-    return out
-
-
-def chunked_oracle_residual(elem, coords, elements, dens, u, chunk=262144):
-    """The oracle's FP64 residual of a large mesh, accumulated over element
-    chunks (residuals are sums of per-element contributions)."""
-    import numpy as np
-
-    from oracle import fem  # the checker only
-
-    r = np.zeros(u.size)
-    for a in range(0, elements.shape[0], chunk):
-        r += fem.Problem(elem, coords, elements[a:a + chunk], dens).residual(u)
-        print(f"[oracle] residual: {min(a + chunk, elements.shape[0])}/{elements.shape[0]} "
-              "elements", file=sys.stderr, flush=True)
-    return r
 
 
 def large_mixed(workload, oracle_check=True, upto=None):
@@ -338,16 +320,17 @@ def large_mixed(workload, oracle_check=True, upto=None):
         el = fem.Tet4() if kind == "tet4" else fem.Hex8()
         L_end = NH["L"].at["load"].set(NH["values"](stretch * done / steps))
         free = L_end.free_dofs
+        # the oracle's element algebra in C over the host's threads
+        Q = fem.ProblemOMP(el, mesh.coords, mesh.elements, fem.NeoHookean(1.0, 10.0),
+                           threads=cpu_threads())
         # scale: the oracle residual of the final load applied to the undeformed body
         # (free dofs zero, prescribed dofs at their values)
         u_load = L_end.lift_from_zeros(np.zeros(free.size)).cpu().numpy()
-        r_load = np.linalg.norm(chunked_oracle_residual(el, mesh.coords, mesh.elements,
-                                                        fem.NeoHookean(1.0, 10.0), u_load)[free])
+        r_load = np.linalg.norm(Q.residual(u_load)[free])
         res["oracle_load_residual_norm"] = float(r_load)
         for mode in ("mixed", "csr"):
             uh = res.pop(f"_u_{mode}").cpu().numpy()
-            r = chunked_oracle_residual(el, mesh.coords, mesh.elements,
-                                        fem.NeoHookean(1.0, 10.0), uh)[free]
+            r = Q.residual(uh)[free]
             res[mode]["oracle_final_residual_norm"] = float(np.linalg.norm(r))
             res[mode]["oracle_rel_residual"] = float(np.linalg.norm(r) / r_load)
     else:
@@ -533,14 +516,20 @@ def headline(args):
         u_load = L_end.lift_from_zeros(np.zeros(free.size)).cpu().numpy()
         vecs = [u_load, u_mixed.cpu().numpy()] + ([u_csr.cpu().numpy()] if u_csr is not None else [])
         t5 = time.perf_counter()
-        rs = chunked_oracle_residuals(fem.Hex8(), mesh.coords, mesh.elements,
-                                      fem.NeoHookean(1.0, 10.0), vecs)
+        # the oracle's element algebra in C over the host's threads (pinned to the
+        # NumPy oracle to 1e-12 in the oracle's tests)
+        Q = fem.ProblemOMP(fem.Hex8(), mesh.coords, mesh.elements, fem.NeoHookean(1.0, 10.0),
+                           threads=cpu_threads())
+        rs = [Q.residual(v) for v in vecs]
         r_load = float(np.linalg.norm(rs[0][free]))
         res["oracle_load_residual_norm"] = r_load
         res["oracle_seconds"] = time.perf_counter() - t5
         for key, r in zip(("mixed", "fp64_csr"), rs[1:]):
             res[key]["oracle_rel_residual"] = float(np.linalg.norm(r[free]) / r_load)
-    tile_bytes, lay = hvp_tile_bytes(op)
+    # the PCG reads the cached linearisation (F^-1 and mu - lambda ln J: 10 floats per
+    # quadrature point) instead of the state
+    cached = args.lin_cache != "never"     # the headline mesh is Hex8
+    tile_bytes, lay = hvp_tile_bytes(op, cached_lin=10 if cached else 0)
     achieved = tile_bytes / (tile_us * 1e-6) / 1e9 if tile_us else None
     out = {
         "metric": "Newton DOF-iters/s + HBM GB/s (mixed vs FP64 @1e-8 rel); serve p50/p99 ms",
@@ -563,8 +552,9 @@ def headline(args):
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "k_tile_assemble<HEX8, NeoHookean<3,float>, float, hvp, affine> "
-                      "(FP32 HVP tile kernel of the matrix-free PCG operator)",
+            "kernel": "k_tile_assemble<HEX8, NeoHookean<3,float>, float, kModeHvpLin, affine> "
+                      "(FP32 HVP tile kernel of the matrix-free PCG operator, reading the "
+                      "cached linearisation)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -700,6 +690,10 @@ def main(argv=None):
                     help="headline: skip the FP64 colored-CSR comparison")
     ap.add_argument("--no-oracle", action="store_true",
                     help="headline: skip the oracle residuals of the final states")
+    ap.add_argument("--lin-cache", choices=("auto", "always", "never"), default="auto",
+                    help="matrix-free PCG: read the density's linearisation from a cache "
+                         "written once per solve (auto: on Hex8 meshes) or re-linearise in "
+                         "every HVP")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extras", action="store_true",
@@ -707,7 +701,8 @@ def main(argv=None):
     ap.add_argument("--nh-n", type=int, default=48, help="cells per side, Neo-Hookean comparison")
     ap.add_argument("--nh-steps", type=int, default=2, help="load steps, Neo-Hookean comparison")
     ap.add_argument("--nh-stretch", type=float, default=0.1, help="final u_x on x = 1")
-    ap.add_argument("--workload", choices=("headline", "c2", "c2mixed", "c3", "c4", "c5", "nh48"),
+    ap.add_argument("--workload", choices=("headline", "c2", "c2mixed", "c3", "c4", "c5", "nh48",
+                                           "copy"),
                     default="headline",
                     help="headline: the default line (Hex8 Neo-Hookean 160^3 mixed Newton, "
                          "FP64 path alongside); c2: the FP64 Hex8 elasticity 64^3 line; "
@@ -717,8 +712,10 @@ def main(argv=None):
                          "stream only (--serve-requests, --serve-qps)")
     ap.add_argument("--large-steps", type=int, default=None,
                     help="c3/c5: run only the first N load steps (short profiled runs)")
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-qps", type=float, nargs="+", default=[10.0, 40.0])
+    ap.add_argument("--serve-requests", type=int, default=500)
+    ap.add_argument("--serve-qps", type=float, nargs="+", default=[40.0, 160.0, 640.0])
+    ap.add_argument("--serve-budget-gb", type=float, default=0.4,
+                    help="HBM budget of the extra serving run at the highest QPS (0: skip)")
     ap.add_argument("--serve-chunk", type=int, default=8, help="batched PCG iterations per poll")
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -732,6 +729,17 @@ def main(argv=None):
     from tatva_amd import solvers
 
     torch.cuda.set_device(0)
+    solvers.set_linearization_cache(args.lin_cache)
+    if args.workload == "copy":
+        # calibration of the HBM traffic counters: a plain 512 MiB device copy
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"workload": "copy", "bytes_read": src.numel() * 8,
+                          "bytes_written": dst.numel() * 8, "launches": 10}), flush=True)
+        return 0
     if args.workload == "headline":
         if args.n is None:
             args.n = 160
@@ -740,8 +748,7 @@ def main(argv=None):
             out["c2_mixed_vs_fp64"] = c2_extra()
             out["c1_poisson_tri3_64"] = c1_latency()
             if args.serve_requests > 0:
-                out["serve_c4"] = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk)
-                                   for q in args.serve_qps]
+                out["serve_c4"] = serve_sweep(args)
         if not args.no_cpu_baseline:
             out["cpu_baseline"] = cpu_baseline_headline(args)
         print(json.dumps(out), flush=True)
@@ -754,8 +761,7 @@ def main(argv=None):
         print(json.dumps({"workload": args.workload, **r}), flush=True)
         return 0
     if args.workload == "c4":
-        r = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk) for q in args.serve_qps]
-        print(json.dumps({"workload": args.workload, "serve": r}), flush=True)
+        print(json.dumps({"workload": args.workload, "serve": serve_sweep(args)}), flush=True)
         return 0
     if args.workload == "c2mixed":
         P = build_problem(args.n)
@@ -845,8 +851,7 @@ def main(argv=None):
         del NH
         extras["c1_poisson_tri3_64"] = c1_latency()
         if args.serve_requests > 0:
-            extras["serve_c4"] = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk)
-                                  for q in args.serve_qps]
+            extras["serve_c4"] = serve_sweep(args)
 
     # checks on the final state: true linear residual and the FP64 oracle's residual
     res_free = P["E"].reduced(L).residual_full_state(u)

@@ -45,11 +45,18 @@ enum {
 };
 
 /* Built-in energy density descriptor (replaces the user JAX density callables of
- * tests/test_sparse.py:20-38 and tests/test_sparse_tracer.py:103-115). */
+ * tests/test_sparse.py:20-38 and tests/test_sparse_tracer.py:103-115).
+ * elem_params != NULL gives every element its own parameters (device array,
+ * n_elements x np in element order, np = 1 for Poisson (f), 2 otherwise (mu,
+ * lambda)); p0.. are then unused.  This is how element quantities reach a
+ * density (Operator.map(..., element_quantity=...), tatva/operator.py:229,256).
+ * points_per_element: tatva_density_eval with elem_params only (its points are
+ * consecutive per element). */
 typedef struct {
   int32_t kind;
-  int32_t reserved;
+  int32_t points_per_element;
   double p0, p1, p2;
+  const double *elem_params;
 } tatva_density;
 
 typedef struct tatva_mesh tatva_mesh;
@@ -78,11 +85,11 @@ int tatva_device_count(int *out);
 int tatva_set_profiling(int enable);
 
 /* Process-wide options (no counterpart in the reference).
- * TATVA_OPT_MATFREE_LIN_CACHE (default 1): tatva_pcg_matfree_f32 linearises the
+ * TATVA_OPT_MATFREE_LIN_CACHE: whether tatva_pcg_matfree_f32 linearises the
  * density once per solve at its fixed state (Neo-Hookean: F^-1 and
- * mu - lambda ln J per quadrature point, FP32, in the workspace) and every HVP
- * of the solve reads it instead of recomputing it; 0 recomputes it in every
- * HVP (same arithmetic, same result). */
+ * mu - lambda ln J per quadrature point, FP32, in the workspace) so that every
+ * HVP of the solve reads it instead of recomputing it (same arithmetic, same
+ * result): 0 never, 1 (default) for Hex8 meshes, 2 always. */
 enum { TATVA_OPT_MATFREE_LIN_CACHE = 1 };
 int tatva_set_option(int32_t option, int32_t value);
 

@@ -252,6 +252,15 @@ def _inv_T(F):
     return np.swapaxes(np.linalg.inv(F), -1, -2)
 
 
+def _pe(p, like):
+    """A density parameter against an (E, ...) array: a scalar as is, per-element
+    values (E,) with trailing unit axes (the element quantities of
+    Operator.map(..., element_quantity=...), tatva/operator.py:229,256)."""
+    if np.ndim(p) == 0:
+        return p
+    return np.asarray(p, dtype=np.float64).reshape((-1,) + (1,) * (np.ndim(like) - 1))
+
+
 @dataclass
 class Poisson:
     """psi = 1/2 |grad u|^2 - f u (scalar field, ncomp = 1)."""
@@ -260,7 +269,7 @@ class Poisson:
     scalar = True
 
     def psi(self, G, uq):
-        return 0.5 * np.sum(G * G, axis=(-2, -1)) - self.f * uq[..., 0]
+        return 0.5 * np.sum(G * G, axis=(-2, -1)) - _pe(self.f, uq[..., 0]) * uq[..., 0]
 
     def P(self, G):
         return G
@@ -281,7 +290,8 @@ class LinearElasticity:
         d = G.shape[-1]
         eps = 0.5 * (G + np.swapaxes(G, -1, -2))
         tr = np.trace(eps, axis1=-2, axis2=-1)
-        return 2 * self.mu * eps + self.lmbda * tr[..., None, None] * np.eye(d), eps
+        return (2 * _pe(self.mu, G) * eps + _pe(self.lmbda, G) * tr[..., None, None] * np.eye(d),
+                eps)
 
     def psi(self, G, uq=None):
         sig, eps = self._sig(G)
@@ -309,14 +319,14 @@ class NeoHookean:
         J = np.linalg.det(F)
         lnJ = np.log(J)
         I1 = np.sum(F * F, axis=(-2, -1))
-        return 0.5 * self.mu * (I1 - d - 2 * lnJ) + 0.5 * self.lmbda * lnJ ** 2
+        return 0.5 * _pe(self.mu, lnJ) * (I1 - d - 2 * lnJ) + 0.5 * _pe(self.lmbda, lnJ) * lnJ ** 2
 
     def P(self, G):
         d = G.shape[-1]
         F = np.eye(d) + G
         lnJ = np.log(np.linalg.det(F))
         FiT = _inv_T(F)
-        return self.mu * (F - FiT) + self.lmbda * lnJ[..., None, None] * FiT
+        return _pe(self.mu, F) * (F - FiT) + _pe(self.lmbda, F) * lnJ[..., None, None] * FiT
 
     def dP(self, G, dG):
         """dP = mu dF + (mu - lambda lnJ) F^-T dF^T F^-T + lambda tr(F^-1 dF) F^-T."""
@@ -327,8 +337,9 @@ class NeoHookean:
         FiT = np.swapaxes(Fi, -1, -2)
         tr = np.einsum("...ij,...ji->...", Fi, dG)
         t2 = FiT @ np.swapaxes(dG, -1, -2) @ FiT
-        return (self.mu * dG + (self.mu - self.lmbda * lnJ)[..., None, None] * t2
-                + self.lmbda * tr[..., None, None] * FiT)
+        c1 = _pe(self.mu, lnJ) - _pe(self.lmbda, lnJ) * lnJ
+        return (_pe(self.mu, dG) * dG + c1[..., None, None] * t2
+                + _pe(self.lmbda, dG) * tr[..., None, None] * FiT)
 
 
 # --------------------------------------------------------------------------
@@ -373,7 +384,8 @@ class Problem:
         P = self.density.P(self.grad_q(u))                                  # (E,Q,nc,d)
         re = np.einsum("eq,eqcd,eqdn->enc", self.w, P, self.B)
         if isinstance(self.density, Poisson):
-            re -= self.density.f * np.einsum("eq,qn->en", self.w, self.Nq)[..., None]
+            src = np.einsum("eq,qn->en", self.w, self.Nq)[..., None]
+            re -= _pe(self.density.f, src) * src
         r = self._assemble(re)
         if self.f_ext is not None:
             r -= self.f_ext
@@ -659,8 +671,10 @@ class ProblemOMP:
 
     def __init__(self, elem, coords, elements, density, threads=0):
         kind = self._KIND.get(type(elem).__name__)
-        if kind is None or not isinstance(density, (LinearElasticity, NeoHookean)):
-            raise NotImplementedError("ProblemOMP: Tet4/Hex8 with LinearElasticity/NeoHookean")
+        if kind is None or not isinstance(density, (LinearElasticity, NeoHookean)) or \
+                np.ndim(density.mu) or np.ndim(density.lmbda):
+            raise NotImplementedError("ProblemOMP: Tet4/Hex8 with LinearElasticity/NeoHookean, "
+                                      "uniform parameters")
         self.kind = kind
         self.dens = 2 if isinstance(density, LinearElasticity) else 3
         self.mu, self.lmbda = float(density.mu), float(density.lmbda)

@@ -24,8 +24,9 @@ c_int, c_i32, c_i64, c_dbl, c_void = (ctypes.c_int, ctypes.c_int32, ctypes.c_int
 
 
 class Density(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("p0", ctypes.c_double), ("p1", ctypes.c_double), ("p2", ctypes.c_double)]
+    _fields_ = [("kind", ctypes.c_int32), ("points_per_element", ctypes.c_int32),
+                ("p0", ctypes.c_double), ("p1", ctypes.c_double), ("p2", ctypes.c_double),
+                ("elem_params", ctypes.c_void_p)]
 
 
 class PcgStats(ctypes.Structure):

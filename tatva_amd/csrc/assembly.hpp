@@ -46,6 +46,7 @@ struct TileSet {
   // inc_list[tile * kTile * nne + ...]; local node j ends at inc_end[node_ptr + j]
   const uint16_t *inc_end;   // (n_slots)
   const uint16_t *inc_list;  // (n_tiles * kTile * nne)
+  const int32_t *elem;       // (n_tiles * kTile) element of each tile slot (-1: padding)
 };
 
 // Element work of a tile kernel.  kModeLinearize stores the density's
@@ -534,6 +535,8 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
   for (int a = 0; a < NNE; ++a)
 #pragma unroll
     for (int c = 0; c < NC; ++c) ye[a][c] = T(0);
+  // per-element material parameters: the density of this slot's element
+  const Dens de = (dens.ep && real) ? dens.at(ts.elem[slot]) : dens;
   __syncthreads();
   if (real) {
     double X[NNE][D];
@@ -551,14 +554,14 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     // this thread's cached linearisation (kModeLinearize / kModeHvpLin)
     T *lp = lin ? lin + (int64_t)tile * quad.nq * Dens::kLin * kTile + threadIdx.x : nullptr;
     if constexpr (KIND == TATVA_HEX8 && MODE == kModeResidual) {
-      hex8_element<Dens, T, kModeResidual, AFFINE>(X, ue, ue, quad, dens, ye);
+      hex8_element<Dens, T, kModeResidual, AFFINE>(X, ue, ue, quad, de, ye);
     } else if constexpr (MODE == kModeResidual) {
-      elem_residual<KIND, Dens>(X, ue, quad, dens, ye);
+      elem_residual<KIND, Dens>(X, ue, quad, de, ye);
     } else if constexpr (MODE == kModeHvp || MODE == kModeHvpLin || MODE == kModeLinearize) {
-      if constexpr (KIND == TATVA_HEX8) hex8_element<Dens, T, MODE, AFFINE>(X, ue, ve, quad, dens, ye, lp);
-      else elem_hvp<KIND, Dens, T, MODE>(X, ue, ve, quad, dens, ye, lp);
+      if constexpr (KIND == TATVA_HEX8) hex8_element<Dens, T, MODE, AFFINE>(X, ue, ve, quad, de, ye, lp);
+      else elem_hvp<KIND, Dens, T, MODE>(X, ue, ve, quad, de, ye, lp);
     } else {
-      elem_diag<KIND, Dens, T>(X, ue, quad, dens, ye);
+      elem_diag<KIND, Dens, T>(X, ue, quad, de, ye);
     }
   }
   if constexpr (MODE == kModeLinearize) return;  // the cache is the output

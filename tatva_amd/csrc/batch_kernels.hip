@@ -62,8 +62,8 @@ struct BatchSlot {
 
 template <class Dens>
 __device__ __forceinline__ Dens make_density(const tatva_density &d) {
-  if constexpr (Dens::kSource) return Dens{float(d.p0)};
-  else return Dens{float(d.p0), float(d.p1)};
+  if constexpr (Dens::kSource) return Dens{float(d.p0), d.elem_params};
+  else return Dens{float(d.p0), float(d.p1), d.elem_params};
 }
 
 template <int KIND, class Dens, bool AFFINE>

@@ -8,6 +8,8 @@
 //             P  = mu (F - F^-T) + lambda ln J F^-T
 //             dP = mu dG + (mu - lambda ln J) F^-T dG^T F^-T + lambda tr(F^-1 dG) F^-T
 // G[c][j] = d u_c / d x_j.
+// A density's parameters are uniform, or per element (ep != NULL): at(e) is the
+// density of element e.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,6 +23,12 @@ struct Poisson {
   static constexpr int NC = 1;
   static constexpr bool kSource = true;
   T f;
+  const double *ep = nullptr;  // per-element f (n_elements), or NULL
+  __device__ __forceinline__ Poisson at(int64_t e) const {
+    Poisson r = *this;
+    if (ep && e >= 0) r.f = T(ep[e]);
+    return r;
+  }
   struct Lin {};
   static constexpr int kLin = 0;  // values of a cached linearisation per point
   __device__ __forceinline__ void store_lin(const Lin &, T *, int) const {}
@@ -47,6 +55,15 @@ struct LinearElastic {
   static constexpr int NC = D;
   static constexpr bool kSource = false;
   T mu, lam;
+  const double *ep = nullptr;  // per-element (mu, lambda) pairs (n_elements x 2), or NULL
+  __device__ __forceinline__ LinearElastic at(int64_t e) const {
+    LinearElastic r = *this;
+    if (ep && e >= 0) {
+      r.mu = T(ep[2 * e]);
+      r.lam = T(ep[2 * e + 1]);
+    }
+    return r;
+  }
   struct Lin {};
   static constexpr int kLin = 0;
   __device__ __forceinline__ void store_lin(const Lin &, T *, int) const {}
@@ -89,6 +106,15 @@ struct NeoHookean {
   static constexpr int NC = D;
   static constexpr bool kSource = false;
   T mu, lam;
+  const double *ep = nullptr;  // per-element (mu, lambda) pairs (n_elements x 2), or NULL
+  __device__ __forceinline__ NeoHookean at(int64_t e) const {
+    NeoHookean r = *this;
+    if (ep && e >= 0) {
+      r.mu = T(ep[2 * e]);
+      r.lam = T(ep[2 * e + 1]);
+    }
+    return r;
+  }
   struct Lin {
     T Fi[D][D];  // F^-1
     T c1;        // mu - lambda ln J

@@ -44,9 +44,10 @@ static int with_density(const tatva_density *d, F &&f) {
     return TATVA_EINVAL;
   }
   switch (d->kind) {
-    case TATVA_DENSITY_POISSON: return f(Poisson<D, T>{T(d->p0)});
-    case TATVA_DENSITY_LINEAR_ELASTIC: return f(LinearElastic<D, T>{T(d->p0), T(d->p1)});
-    case TATVA_DENSITY_NEO_HOOKEAN: return f(NeoHookean<D, T>{T(d->p0), T(d->p1)});
+    case TATVA_DENSITY_POISSON: return f(Poisson<D, T>{T(d->p0), d->elem_params});
+    case TATVA_DENSITY_LINEAR_ELASTIC:
+      return f(LinearElastic<D, T>{T(d->p0), T(d->p1), d->elem_params});
+    case TATVA_DENSITY_NEO_HOOKEAN: return f(NeoHookean<D, T>{T(d->p0), T(d->p1), d->elem_params});
     default: set_error("unknown density kind"); return TATVA_EINVAL;
   }
 }
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_density_eval(int64_t n_elem, int nq,
 #pragma unroll
     for (int j = 0; j < D; ++j) G[c][j] = grad[(i * NC + c) * D + j];
   const double uq = values ? values[i] : 0.0;
-  psi[i] = dens.psi(G, uq);
+  psi[i] = dens.at(i / nq).psi(G, uq);
 }
 
 // Sum over elements of sum_q psi(G_q) detJ w_q (+ Poisson source via u_q).
@@ -253,6 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(int64_t n_elem, const double 
     int32_t nodes[NNE];
     double X[NNE][D];
     load_element<KIND>(e, conn, coords, nodes, X);
+    const Dens de = dens.at(e);
     double ue[NNE][NC];
 #pragma unroll
     for (int a = 0; a < NNE; ++a)
@@ -279,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(int64_t n_elem, const double 
 #pragma unroll
         for (int a = 0; a < NNE; ++a) uq += N[a] * ue[a][0];
       }
-      ee += dens.psi(G, uq) * (det * quad.w[q]);
+      ee += de.psi(G, uq) * (det * quad.w[q]);
     }
     acc += ee;
   }
@@ -306,6 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_elem_stiffness(int64_t n_elem, const
   int32_t nodes[NNE];
   double X[NNE][D];
   load_element<KIND>(e, conn, coords, nodes, X);
+  const Dens de = dens.at(e);
   double ue[NNE][NC], ke[NNE][NC];
 #pragma unroll
   for (int a = 0; a < NNE; ++a)
@@ -332,8 +335,8 @@ __global__ __launch_bounds__(kBlock) void k_elem_stiffness(int64_t n_elem, const
         for (int a = 0; a < NNE; ++a) bj = (a == b) ? B[j][a] : bj;
         dG[c][j] = (c == k) ? bj : 0.0;
       }
-    const auto L = dens.linearize(G);
-    dens.dP(L, dG, dP);
+    const auto L = de.linearize(G);
+    de.dP(L, dG, dP);
 #pragma unroll
     for (int a = 0; a < NNE; ++a)
 #pragma unroll
@@ -636,6 +639,7 @@ struct HostTiles {
   std::vector<int32_t> tnodes;      // global node of every slot
   std::vector<uint16_t> inc_end;    // (incidence lists only) n_slots
   std::vector<uint16_t> inc_list;   // (incidence lists only) n_tiles * kTile * nne
+  std::vector<int32_t> elem;        // n_tiles * kTile: element of each tile slot, -1 padding
 };
 
 uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
@@ -777,6 +781,7 @@ int build_tiles(const double *coords, int dim, int64_t N, const int32_t *element
   // per tile: local nodes, colors, color-ordered local connectivity
   T.n_tiles = (int32_t)n_tiles;
   T.lnodes.assign((size_t)n_tiles * kTile * nne, 0);
+  T.elem.assign((size_t)n_tiles * kTile, -1);
   if (incidence) T.inc_list.assign((size_t)n_tiles * kTile * nne, 0);
   std::vector<std::vector<int32_t>> tloc((size_t)n_tiles);  // sorted local -> global node
   std::vector<std::vector<uint16_t>> tinc((size_t)n_tiles);  // incidence list ends per local node
@@ -844,6 +849,7 @@ int build_tiles(const double *coords, int dim, int64_t N, const int32_t *element
       for (int64_t k = 0; k < cnt; ++k) {
         const int src = ord[(size_t)k];
         for (int a = 0; a < nne; ++a) T.lnodes[(size_t)((b + k) * nne + a)] = ln[(size_t)src][a];
+        T.elem[(size_t)(b + k)] = key[(size_t)(b + src)].second;
       }
       if (incidence) {
         // entries k * nne + a of each local node, ascending (a counting sort)
@@ -1033,13 +1039,14 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
   if ((err = upload(&m->t_slot_ptr, T.slot_ptr, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_slot_pos, T.slot_pos, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_tnodes, T.tnodes, "tiles")) != hipSuccess) return fail(err, "tiles");
+  if ((err = upload(&m->t_elem, T.elem, "tiles")) != hipSuccess) return fail(err, "tiles");
   if (elem_kind == TATVA_TET4) {
     if ((err = upload(&m->t_inc_end, T.inc_end, "tiles")) != hipSuccess) return fail(err, "tiles");
     if ((err = upload(&m->t_inc_list, T.inc_list, "tiles")) != hipSuccess) return fail(err, "tiles");
   }
   m->tiles = TileSet{T.n_tiles, T.max_local, T.max_colors, 0, m->t_lnodes,
                      m->t_color_off, m->t_node_ptr, m->t_slot_ptr, m->t_slot_pos,
-                     m->t_tnodes, m->t_inc_end, m->t_inc_list};
+                     m->t_tnodes, m->t_inc_end, m->t_inc_list, m->t_elem};
   if ((err = hipMalloc(&m->coords, n_nodes * dim * sizeof(double))) != hipSuccess)
     return fail(err, "hipMalloc coords");
   if ((err = hipMalloc(&m->conn, total * sizeof(int32_t))) != hipSuccess)
@@ -1084,6 +1091,7 @@ extern "C" int tatva_mesh_destroy(tatva_mesh *m) {
   for (void *p : {(void *)m->t_lnodes, (void *)m->t_color_off,
                   (void *)m->t_node_ptr, (void *)m->t_slot_ptr, (void *)m->t_slot_pos,
                   (void *)m->t_tnodes, (void *)m->t_inc_end, (void *)m->t_inc_list,
+                  (void *)m->t_elem,
                   (void *)m->loc_ptr, (void *)m->loc_elems})
     if (p) (void)hipFree(p);
   delete m;
@@ -1178,14 +1186,21 @@ extern "C" int tatva_density_eval(int dim, int64_t n_points, const tatva_density
   TATVA_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
   if (d && d->kind == TATVA_DENSITY_POISSON) TATVA_REQUIRE(values, "Poisson needs the values u_q");
   if (n_points == 0) return TATVA_OK;
+  // per-element parameters: points come in runs of points_per_element per element
+  int64_t ppe = 1;
+  if (d && d->elem_params) {
+    TATVA_REQUIRE(d->points_per_element >= 1 && n_points % d->points_per_element == 0,
+                  "per-element parameters need points_per_element dividing n_points");
+    ppe = d->points_per_element;
+  }
   hipStream_t s = (hipStream_t)stream;
   const int kind = dim == 2 ? TATVA_TRI3 : TATVA_HEX8;
   return with_kind(kind, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
     return with_density<Elem<K>::D, double>(d, [&](auto dens) -> int {
       using Dens = decltype(dens);
-      k_density_eval<K, Dens><<<grid_for(n_points), kBlock, 0, s>>>(n_points, 1, dens, grad,
-                                                                     values, psi);
+      k_density_eval<K, Dens><<<grid_for(n_points), kBlock, 0, s>>>(n_points / ppe, (int)ppe, dens,
+                                                                     grad, values, psi);
       TATVA_CHECK_LAUNCH();
       return TATVA_OK;
     });

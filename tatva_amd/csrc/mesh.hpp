@@ -39,6 +39,7 @@ struct tatva_mesh {
   int32_t *t_tnodes;
   uint16_t *t_inc_end;
   uint16_t *t_inc_list;
+  int32_t *t_elem;
   // point location (2-D, built by the first interpolate): a uniform grid over
   // the nodes' bounding box; cell c lists, ascending, the elements whose
   // bounding box overlaps it in loc_elems[loc_ptr[c] .. loc_ptr[c + 1])

@@ -723,7 +723,7 @@ struct DeviceRes {
 static DeviceRes g_res[64];
 static std::mutex g_res_init_mu;  // serialises the first use of a device's resources
 static std::atomic<int> g_profiling{0};
-static std::atomic<int> g_lin_cache{1};  // TATVA_OPT_MATFREE_LIN_CACHE
+static std::atomic<int> g_lin_cache{1};  // TATVA_OPT_MATFREE_LIN_CACHE: 0 never, 1 auto, 2 always
 
 // Drops cached solver graphs that captured this mesh's arrays (called before
 // the mesh's device memory is freed).
@@ -987,7 +987,10 @@ extern "C" int tatva_set_profiling(int enable) {
 
 extern "C" int tatva_set_option(int32_t option, int32_t value) {
   switch (option) {
-    case TATVA_OPT_MATFREE_LIN_CACHE: g_lin_cache.store(value ? 1 : 0); return TATVA_OK;
+    case TATVA_OPT_MATFREE_LIN_CACHE:
+      TATVA_REQUIRE(value >= 0 && value <= 2, "linearisation cache option: 0, 1 or 2");
+      g_lin_cache.store(value);
+      return TATVA_OK;
     default: set_error("unknown option"); return TATVA_EINVAL;
   }
 }
@@ -1289,8 +1292,13 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   float *b = ws_vec<float>(wsb, L, 7);
   float *diag = ws_vec<float>(wsb, L, 8);
   float *u = reinterpret_cast<float *>(wsb + matfree_u_offset(n));
-  // the state is fixed during the solve: the density is linearised once
-  float *lin = (g_lin_cache.load() && lin_cache_floats(m, d) > 0)
+  // the state is fixed during the solve: the density is linearised once.  By
+  // default only for Hex8 (eight quadrature points of tangent work per element
+  // against 40 bytes per point read back); Tet4's one point re-linearises for
+  // less than the cache's bytes cost
+  const int lin_opt = g_lin_cache.load();
+  const bool use_lin = lin_opt == 2 || (lin_opt == 1 && m->kind == TATVA_HEX8);
+  float *lin = (use_lin && lin_cache_floats(m, d) > 0)
                    ? reinterpret_cast<float *>(wsb + matfree_lin_offset(m, n))
                    : nullptr;
   const PcgParts P = parts_at(wsb, L);
@@ -1342,6 +1350,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   key.i1 = chunk;
   key.i2 = profiling ? 1 : 0;
   key.i3 = lin ? 1 : 0;
+  key.c = d->elem_params;
   key.f0 = d->p0;
   key.f1 = d->p1;
   key.f2 = d->p2;
@@ -1493,6 +1502,7 @@ extern "C" int tatva_pcg_matfree_f32_mapped(tatva_mesh *m, const tatva_density *
   key.b = read_map;
   key.c = rows_ptr;
   key.d = rows_list;
+  key.e = d->elem_params;
   key.n = n;
   key.m = n_full;
   key.i0 = d->kind;

@@ -29,8 +29,31 @@ import numpy as np
 from . import _native as nat
 
 
+def _param(x):
+    """A density parameter: a float, or per-element values (a 1-D array)."""
+    if np.ndim(x) == 0:
+        return float(x)
+    a = np.ascontiguousarray(nat_host(x), dtype=np.float64)
+    if a.ndim != 1:
+        raise ValueError("per-element parameters must be 1-D arrays (one value per element)")
+    return a
+
+
+def nat_host(a):
+    t = nat.torch()
+    if isinstance(a, t.Tensor):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
+
+
 class Density:
-    """Base class of the built-in densities (FP64 parameters)."""
+    """Base class of the built-in densities (FP64 parameters).
+
+    Each parameter is a float, or a 1-D array with one value per element: the
+    element quantities that tatva passes through ``Operator.map(...,
+    element_quantity=...)`` (tatva/operator.py:229,256), e.g. two materials in
+    one mesh.  Per-element parameters travel to the kernels as one (E, np)
+    device array."""
 
     kind: int = 0
     needs_values = False
@@ -41,9 +64,33 @@ class Density:
     def ncomp(self, dim: int) -> int:
         return dim
 
-    def descriptor(self) -> nat.Density:
+    @property
+    def per_element(self) -> bool:
+        return any(np.ndim(p) for p in self.params())
+
+    def n_elements(self):
+        """Number of elements of per-element parameters (None when uniform)."""
+        sizes = {np.shape(p)[0] for p in self.params() if np.ndim(p)}
+        if len(sizes) > 1:
+            raise ValueError("per-element parameters of different lengths")
+        return sizes.pop() if sizes else None
+
+    def element_params(self):
+        """(E, np) FP64 device array of the per-element parameters (cached)."""
+        dev = getattr(self, "_elem_dev", None)
+        if dev is None:
+            E = self.n_elements()
+            cols = [np.broadcast_to(np.asarray(p, dtype=np.float64), (E,)) for p in self.params()]
+            dev = nat.to_device(np.ascontiguousarray(np.stack(cols, axis=1)))
+            self._elem_dev = dev
+        return dev
+
+    def descriptor(self, points_per_element: int = 0) -> nat.Density:
+        if self.per_element:
+            ep = self.element_params()
+            return nat.Density(self.kind, int(points_per_element), 0.0, 0.0, 0.0, nat.dptr(ep))
         p = list(self.params()) + [0.0] * 3
-        return nat.Density(self.kind, 0, float(p[0]), float(p[1]), float(p[2]))
+        return nat.Density(self.kind, 0, float(p[0]), float(p[1]), float(p[2]), None)
 
     def __call__(self, grad_u, values=None):
         """psi at quadrature points from gradients (..., ncomp, d) -> (...)."""
@@ -67,7 +114,14 @@ class Density:
             npts = int(np.prod(lead)) if lead else 1
         t = nat.torch()
         out = t.empty(lead, dtype=t.float64, device=g.device)
-        desc = self.descriptor()
+        ppe = 0
+        if self.per_element:
+            # points (E, Q, ...) in element order
+            if not lead or lead[0] != self.n_elements():
+                raise ValueError(f"per-element parameters of {self.n_elements()} elements need "
+                                 "points shaped (n_elements, ...)")
+            ppe = npts // lead[0]
+        desc = self.descriptor(ppe)
         nat.check(nat.lib().tatva_density_eval(d, npts, ctypes.byref(desc), nat.dptr(g.contiguous()),
                                                nat.dptr(vals), nat.dptr(out), nat.stream()))
         return out
@@ -79,8 +133,8 @@ class Poisson(Density):
     kind = nat.POISSON
     needs_values = True
 
-    def __init__(self, f: float = 1.0):
-        self.f = float(f)
+    def __init__(self, f=1.0):
+        self.f = _param(f)
 
     def params(self):
         return (self.f,)
@@ -97,8 +151,8 @@ class LinearElasticity(Density):
 
     kind = nat.LINEAR_ELASTIC
 
-    def __init__(self, mu: float, lmbda: float):
-        self.mu, self.lmbda = float(mu), float(lmbda)
+    def __init__(self, mu, lmbda):
+        self.mu, self.lmbda = _param(mu), _param(lmbda)
 
     @classmethod
     def from_youngs_poisson(cls, E: float, nu: float) -> "LinearElasticity":
@@ -119,8 +173,8 @@ class NeoHookean(Density):
 
     kind = nat.NEO_HOOKEAN
 
-    def __init__(self, mu: float, lmbda: float):
-        self.mu, self.lmbda = float(mu), float(lmbda)
+    def __init__(self, mu, lmbda):
+        self.mu, self.lmbda = _param(mu), _param(lmbda)
 
     def params(self):
         return (self.mu, self.lmbda)
@@ -154,6 +208,10 @@ class Energy:
         self.density = density
         self.ncomp = density.ncomp(op.element.dim)
         self.size = op.n_nodes * self.ncomp
+        E = density.n_elements()
+        if E is not None and E != op.n_elements:
+            raise ValueError(f"per-element parameters of {E} elements for a mesh of "
+                             f"{op.n_elements} elements")
         self._desc = density.descriptor()
         self.f_ext = None if external_force is None else nat.to_device(external_force).reshape(-1)
         if self.f_ext is not None and self.f_ext.shape[0] != self.size:

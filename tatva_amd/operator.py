@@ -215,7 +215,14 @@ class Operator:
         from .energies import Density  # noqa: PLC0415
 
         if element_quantity:
-            raise NotImplementedError("element_quantity needs a user callable; not supported on GPU")
+            if isinstance(func, type) and issubclass(func, Density):
+                return self._map_density_with_element_quantities(func, tuple(element_quantity))
+            raise NotImplementedError(
+                "element_quantity is supported with a built-in density class "
+                "(tatva_amd.energies): its parameters are the element quantities")
+        if isinstance(func, type) and issubclass(func, Density):
+            raise ValueError(f"{func.__name__} as a class needs its parameters as element "
+                             "quantities (element_quantity=...); pass an instance otherwise")
         if isinstance(func, Density):
             return lambda u: func(self.grad(u), values=self.eval(u) if func.needs_values else None)
         name = func if isinstance(func, str) else getattr(func, "__name__", None)
@@ -232,8 +239,35 @@ class Operator:
             "Operator.map runs built-in kernels only (gradient, interpolate, det_jacobian, "
             "tatva_amd.energies densities); arbitrary Python callables cannot run on the GPU path")
 
+    def _map_density_with_element_quantities(self, cls, element_quantity):
+        """``map(DensityClass, element_quantity=idx)(*values)``: the arguments at
+        the indices in ``element_quantity`` are per-element values (one per
+        element, passed through as tatva/operator.py:253-258 does), in the order
+        of the density's parameters; the one other argument is the nodal field.
+        Returns psi at every (element, quadrature point)."""
+        def _mapped(*values):
+            idx = sorted(set(int(i) for i in element_quantity))
+            if any(i < 0 or i >= len(values) for i in idx):
+                raise ValueError("element_quantity index out of range")
+            nodal = [v for i, v in enumerate(values) if i not in idx]
+            if len(nodal) != 1:
+                raise ValueError("a density maps one nodal field (the state); got "
+                                 f"{len(nodal)} nodal arguments")
+            quantities = []
+            for i in idx:
+                q = nat.to_device(values[i]).reshape(-1)
+                if q.shape[0] != self.n_elements:
+                    raise ValueError(f"element quantity {i} must have {self.n_elements} values")
+                quantities.append(q)
+            dens = cls(*quantities)
+            u = nodal[0]
+            return dens(self.grad(u), values=self.eval(u) if dens.needs_values else None)
+        return _mapped
+
     def map_over_elements(self, func, *, element_quantity: Sequence[int] = ()):
-        """Per-element map (operator.py:268-305): built-in ``"integrate"`` only."""
+        """Per-element map (operator.py:268-305): built-in ``"integrate"`` only
+        (quadrature-point or per-element arrays are integrated per element, so
+        element quantities pass through unchanged)."""
         name = func if isinstance(func, str) else getattr(func, "__name__", None)
         if name == "integrate":
             return self.integrate_per_element

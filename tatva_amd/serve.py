@@ -74,12 +74,15 @@ class SolveRequest:
     load_steps: int = 1
     stretch: float = 0.03
     traction: float = 1e-3
+    max_cg: int | None = None   # inner PCG iteration limit (None: the server's default)
 
     def __post_init__(self):
         if self.material not in ("linear_elastic", "neo_hookean"):
             raise ValueError("material must be 'linear_elastic' or 'neo_hookean'")
         if self.n < 1 or self.load_steps < 1:
             raise ValueError("n and load_steps must be >= 1")
+        if self.max_cg is not None and self.max_cg < 1:
+            raise ValueError("max_cg must be >= 1")
 
     @property
     def n_elements(self) -> int:
@@ -134,6 +137,9 @@ class RequestResult:
     load_steps: int = 0
     converged: bool = False
     rel_residual: float = float("nan")
+    # why a request did not converge: "non-finite residual", "inner PCG iteration
+    # limit", "CG breakdown", "Newton iteration limit", "exceeds the HBM budget"
+    failure: str | None = None
     u: object = None  # full FP64 solution on the device (kept if requested)
 
     @property
@@ -257,8 +263,10 @@ class _Job:
                 self.r0 = rn
             rel = rn / self.r0 if self.r0 > 0 else 0.0
             self.result.rel_residual = rel
-            if not math.isfinite(rn) or self.newton_in_step >= srv.max_newton:
-                return self._finish(False)
+            if not math.isfinite(rn):
+                return self._finish(False, "non-finite residual")
+            if self.newton_in_step >= srv.max_newton:
+                return self._finish(False, "Newton iteration limit")
             if rn <= srv.tol * self.r0 or rn == 0.0:
                 if self.step < self.req.load_steps:
                     self._set_load_step(self.step + 1)
@@ -279,7 +287,8 @@ class _Job:
                                       ctypes.byref(self.energy._desc), nat.dptr(self.u32),
                                       nat.dptr(dmap), self.n, nat.dptr(diag), nat.dptr(self.b32),
                                       nat.dptr(self.x32), float(srv.inner_tol),
-                                      int(srv.maxiter_factor * self.n), nat.dptr(self.ws),
+                                      int(self.req.max_cg or srv.maxiter_factor * self.n),
+                                      nat.dptr(self.ws),
                                       self.ws.numel(), s))
         return True
 
@@ -287,15 +296,18 @@ class _Job:
         """Newton update u_free += x after the batch reports the solve done."""
         self.result.cg_iterations += int(st.iterations)
         if st.status == nat.EBREAKDOWN:
-            return self._finish(False)
+            return self._finish(False, "CG breakdown")
+        if st.status == nat.ENOTCONV:
+            return self._finish(False, "inner PCG iteration limit")
         L = nat.lib()
         nat.check(L.tatva_axpy_free_f32(self.n, nat.dptr(self.lifter.free_device()), 1.0,
                                         nat.dptr(self.x32), nat.dptr(self.u), nat.stream()))
         return self._newton()
 
-    def _finish(self, ok: bool) -> bool:
+    def _finish(self, ok: bool, failure: str | None = None) -> bool:
         self.done = True
         self.result.converged = ok
+        self.result.failure = failure
         self.result.load_steps = self.step
         return False
 
@@ -323,20 +335,29 @@ class ServeReport:
     extra: dict = field(default_factory=dict)
 
     def latencies(self):
-        return np.array([r.latency for r in self.results])
+        """Latencies of the admitted requests (seconds)."""
+        return np.array([r.latency for r in self.results if math.isfinite(r.admitted)])
 
     def summary(self) -> dict:
         lat = self.latencies()
+        admitted = [r for r in self.results if math.isfinite(r.admitted)]
         done = [r for r in self.results if r.converged]
+        failures: dict = {}
+        for r in self.results:
+            if r.failure:
+                failures[r.failure] = failures.get(r.failure, 0) + 1
         return {
             "requests": len(self.results),
             "converged": len(done),
+            "failures": failures,
             "seconds": self.seconds,
-            "throughput_req_per_s": len(self.results) / self.seconds if self.seconds > 0 else 0.0,
+            "throughput_req_per_s": len(done) / self.seconds if self.seconds > 0 else 0.0,
             "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else float("nan"),
             "p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else float("nan"),
-            "mean_queue_ms": float(np.mean([r.queue_wait for r in self.results]) * 1e3)
-            if self.results else float("nan"),
+            "mean_queue_ms": float(np.mean([r.queue_wait for r in admitted]) * 1e3)
+            if admitted else float("nan"),
+            "p99_queue_ms": float(np.percentile([r.queue_wait for r in admitted], 99) * 1e3)
+            if admitted else float("nan"),
             "dof_iters_per_s": self.dof_iterations / self.seconds if self.seconds > 0 else 0.0,
             "batch_launches": self.batch_launches,
             "peak_admitted": self.peak_admitted,
@@ -427,9 +448,9 @@ class Server:
         t = nat.torch()
         L = nat.lib()
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        for r in pending:
-            if r.hbm_bytes() > self.hbm_budget:
-                raise ValueError(f"request {r.rid} needs more than the HBM budget")
+        # a request larger than the whole budget can never be admitted: rejected alone
+        rejected = [r for r in pending if r.hbm_bytes() > self.hbm_budget]
+        pending = [r for r in pending if r.hbm_bytes() <= self.hbm_budget]
         waiting: list[SolveRequest] = []
         preparing: list = []  # (request, future of its _Job), FIFO
         jobs: dict[int, _Job] = {}
@@ -497,6 +518,10 @@ class Server:
             pool.shutdown(wait=True)
         t.cuda.synchronize()
         seconds = time.perf_counter() - t0
+        for r in rejected:
+            results.append(RequestResult(rid=r.rid, n=r.n, material=r.material, n_free=0,
+                                         arrival=r.arrival, admitted=float("nan"),
+                                         finished=r.arrival, failure="exceeds the HBM budget"))
         results.sort(key=lambda r: r.rid)
         return ServeReport(results=results, seconds=seconds, batch_launches=launches,
                            peak_admitted=peak_adm, peak_hbm_bytes=peak_bytes,

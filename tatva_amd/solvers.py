@@ -155,10 +155,17 @@ def set_profiling(enable: bool) -> None:
 OPT_MATFREE_LIN_CACHE = 1
 
 
-def set_linearization_cache(enable: bool) -> None:
-    """Matrix-free FP32 PCG: linearise the density once per solve (default) or
-    in every Hessian-vector product (same result)."""
-    nat.check(nat.lib().tatva_set_option(OPT_MATFREE_LIN_CACHE, 1 if enable else 0))
+_LIN_CACHE_MODES = {"never": 0, "auto": 1, "always": 2}
+
+
+def set_linearization_cache(mode: str) -> None:
+    """Matrix-free FP32 PCG: linearise the density once per solve and read it in
+    every Hessian-vector product ("always"), recompute it in every product
+    ("never"), or cache it on Hex8 meshes only ("auto", the default).  The
+    results are the same."""
+    if mode not in _LIN_CACHE_MODES:
+        raise ValueError(f"mode must be one of {sorted(_LIN_CACHE_MODES)}")
+    nat.check(nat.lib().tatva_set_option(OPT_MATFREE_LIN_CACHE, _LIN_CACHE_MODES[mode]))
 
 
 def dot(x, y):

@@ -140,9 +140,9 @@ def test_pcg_matfree_f32_matches_oracle_solve(kind, jitter):
     CG reports a breakdown).  Tolerance: the true FP64 relative residual <= 1e-4
     for an inner tolerance of 1e-6 (FP32 rounding of the operator times the
     conditioning), and x within 1e-2 of the direct solution (the conditioning
-    times that); a wrong operator leaves O(1) errors.  With the linearisation
-    cached (the default) and recomputed in every product, the solve is the same
-    bit for bit: the cache holds exactly the values the product computes."""
+    times that); a wrong operator leaves O(1) errors.  The solve with the
+    linearisation cached once and the one recomputing it in every product agree
+    to FP32 rounding (iterations within 2, solutions within 1e-3)."""
     import scipy.sparse.linalg as spla
     import torch
 
@@ -162,19 +162,28 @@ def test_pcg_matfree_f32_matches_oracle_solve(kind, jitter):
     red = C["E"].reduced(Lg)
     u32 = torch.as_tensor(u, dtype=torch.float32, device="cuda")
     b32 = torch.as_tensor(b, dtype=torch.float32, device="cuda")
-    x32, info = solvers.pcg_matfree(red, u32, b32, tol=1e-6, chunk=16)
+    solvers.set_linearization_cache("always")
+    try:
+        x32, info = solvers.pcg_matfree(red, u32, b32, tol=1e-6, chunk=16)
+    finally:
+        solvers.set_linearization_cache("auto")
     assert info.converged, info
     x = x32.double().cpu().numpy()
     res = np.linalg.norm(H @ x - b) / np.linalg.norm(b)
     assert res <= 1e-4, res
     assert np.max(np.abs(x - x_ref)) <= 1e-2 * np.max(np.abs(x_ref))
-    solvers.set_linearization_cache(False)
+    solvers.set_linearization_cache("never")
     try:
         x32b, info_b = solvers.pcg_matfree(red, u32, b32, tol=1e-6, chunk=16)
     finally:
-        solvers.set_linearization_cache(True)
-    assert info_b.iterations == info.iterations
-    assert torch.equal(x32b, x32)
+        solvers.set_linearization_cache("auto")
+    # the same solve to FP32 rounding: the two kernels compute the same linearisation,
+    # but the compiler fuses multiply-adds differently in the kernel that stores it
+    # and in the one that uses it right away, so the last bits may differ
+    xb = x32b.double().cpu().numpy()
+    assert abs(info_b.iterations - info.iterations) <= 2, (info_b.iterations, info.iterations)
+    assert np.linalg.norm(H @ xb - b) / np.linalg.norm(b) <= 1e-4
+    assert np.max(np.abs(xb - x)) <= 1e-3 * np.max(np.abs(x))
     # the FP32 HVP is the FP32 restatement of the oracle's: same CG trajectory
     # up to rounding, so the iteration counts are close to the FP64 oracle PCG's
     _, k_or, _ = fem.pcg(H, b, tol=1e-6)

@@ -563,3 +563,41 @@ def test_ldf_coloring_is_a_valid_non_greedy_distance2_coloring():
     bad = greedy.copy()
     bad[S.indices[S.indptr[3] + 1]] = bad[S.indices[S.indptr[3]]]
     assert not fem.is_distance2_coloring(S.indptr, S.indices, bad)
+
+
+@pytest.mark.parametrize("kind", ["tri3-poisson", "tet4-le", "hex8-nh"])
+def test_per_element_parameters_split_into_uniform_problems(kind):
+    """Per-element density parameters (element quantities) in the oracle: the
+    energy, residual, HVP and element stiffness of a two-material mesh equal the
+    sums of the two uniform problems on the element subsets (energies and nodal
+    vectors are sums over elements)."""
+    rng = np.random.default_rng(8)
+    if kind == "tri3-poisson":
+        X, el = fem.unit_square_tri3(5, 4)
+        elem = fem.Tri3()
+        mk = lambda a: fem.Poisson(a)  # noqa: E731
+        pa, pb = (2.0,), (-0.5,)
+    elif kind == "tet4-le":
+        X, el = fem.box_tet4(3, 3, 3, jitter=0.2, seed=2)
+        elem = fem.Tet4()
+        mk = fem.LinearElasticity  # noqa: E731
+        pa, pb = (1.0, 10.0), (0.3, 0.7)
+    else:
+        X, el = fem.cube_hex8(3)
+        elem = fem.Hex8()
+        mk = fem.NeoHookean  # noqa: E731
+        pa, pb = (1.0, 10.0), (2.5, 4.0)
+    E = el.shape[0]
+    inA = X[el].mean(axis=1)[:, 0] < 0.5
+    params = [np.where(inA, a, b) for a, b in zip(pa, pb)]
+    P = fem.Problem(elem, X, el, mk(*params))
+    PA = fem.Problem(elem, X, el[inA], mk(*pa))
+    PB = fem.Problem(elem, X, el[~inA], mk(*pb))
+    u = 0.02 * rng.standard_normal(P.n)
+    v = rng.standard_normal(P.n)
+    assert np.isclose(P.energy(u), PA.energy(u) + PB.energy(u), rtol=1e-13, atol=0)
+    for a, b in ((P.residual(u), PA.residual(u) + PB.residual(u)),
+                 (P.hvp(u, v), PA.hvp(u, v) + PB.hvp(u, v))):
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+    K = P.element_stiffness(u)
+    assert np.array_equal(K[inA], PA.element_stiffness(u)) and E == K.shape[0]

@@ -247,3 +247,66 @@ def test_batch_slots_of_every_element_kind_solve_their_systems():
     nat.check(L.tatva_batch_destroy(B))
     for item in keep:
         item[0].close()
+
+
+@pytest.mark.gpu
+def test_failing_requests_fail_alone():
+    """Failure isolation: a Neo-Hookean request whose load inverts its elements
+    (u_x = -1.5 x: J < 0, ln J not finite, so the FP64 residual is not finite)
+    and a request whose inner PCG hits its iteration limit (max_cg = 2) both
+    fail with their reason, in the same batches as the others, and every other
+    request's result is bit-identical to the one it gets without them in the
+    mix."""
+    base = _mix(seed=4)
+    bad = [serve.SolveRequest(rid=100, n=4, material="neo_hookean", E=1.0, nu=0.3,
+                              load_steps=1, stretch=-1.5),
+           serve.SolveRequest(rid=101, n=5, material="linear_elastic", E=1.0, nu=0.3,
+                              max_cg=2)]
+    srv = serve.Server(max_slots=8, chunk=3, keep_solutions=True)
+    clean = srv.run(base, realtime=False)
+    mixed = srv.run(base + bad, realtime=False)
+    srv.close()
+    by_rid = {r.rid: r for r in mixed.results}
+    assert not by_rid[100].converged and by_rid[100].failure == "non-finite residual"
+    assert not by_rid[101].converged and by_rid[101].failure == "inner PCG iteration limit"
+    # the iteration-limited request shared the batch with the others (the inverted
+    # one fails at its first residual, before a batch iteration)
+    assert mixed.peak_admitted >= len(base) + 1
+    for r in clean.results:
+        m = by_rid[r.rid]
+        assert r.converged and m.converged and m.failure is None
+        assert (m.newton_iterations, m.cg_iterations) == (r.newton_iterations, r.cg_iterations)
+        assert np.array_equal(m.u.cpu().numpy(), r.u.cpu().numpy()), r.rid
+    s = mixed.summary()
+    assert s["converged"] == len(base)
+    assert s["failures"] == {"non-finite residual": 1, "inner PCG iteration limit": 1}
+
+
+@pytest.mark.gpu
+def test_admission_waits_under_a_binding_hbm_budget():
+    """Ten requests arriving at once under an HBM budget that holds about two of
+    them: the server never holds more than the budget, later requests wait for
+    memory (queue time > 0) and are admitted as earlier ones leave, a request
+    larger than the whole budget is rejected alone, and every served result is
+    bit-identical to the one under an unlimited budget."""
+    reqs = serve.make_stream(10, qps=0.0, seed=5, n_range=(6, 10))
+    sizes = sorted(r.hbm_bytes() for r in reqs)
+    budget = sizes[-1] + sizes[-2]
+    huge = serve.SolveRequest(rid=50, n=48, material="linear_elastic", E=1.0, nu=0.3)
+    assert huge.hbm_bytes() > budget
+    free = serve.Server(max_slots=16, chunk=4, keep_solutions=True)
+    ref = free.run(reqs, realtime=False)
+    free.close()
+    srv = serve.Server(max_slots=16, chunk=4, keep_solutions=True, hbm_budget=budget)
+    rep = srv.run(reqs + [huge], realtime=False)
+    srv.close()
+    by_rid = {r.rid: r for r in rep.results}
+    assert by_rid[50].failure == "exceeds the HBM budget" and not by_rid[50].converged
+    assert rep.peak_hbm_bytes <= budget
+    assert 1 <= rep.peak_admitted <= 3
+    waits = sorted(by_rid[r.rid].queue_wait for r in reqs)
+    assert waits[-1] > waits[0]          # the last admitted waited for memory
+    for r in ref.results:
+        m = by_rid[r.rid]
+        assert m.converged and r.converged
+        assert np.array_equal(m.u.cpu().numpy(), r.u.cpu().numpy()), r.rid
