@@ -118,7 +118,8 @@ def solve_load_steps(P, solver, n_steps, stretch, upto=None):
     return u, out
 
 
-def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=False, warmup=True):
+def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=False, warmup=True,
+                  modes=("csr", "mixed")):
     """The same problem solved by FP64 colored-CSR Newton and by mixed Newton
     (FP32 matrix-free PCG inside FP64 Newton/iterative refinement), both to
     ||r|| <= tol ||r_0||.  ``run(solver)`` performs the solve and returns
@@ -134,7 +135,7 @@ def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=Fals
 
     res = {}
     sols = {}
-    for mode in ("csr", "mixed"):
+    for mode in modes:
         gc.collect()
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
@@ -172,12 +173,14 @@ def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=Fals
         res[mode]["torch_peak_reserved_gb"] = torch.cuda.max_memory_reserved() / 1e9
         sols[mode] = u
         del solver
-    a, b = sols["mixed"], sols["csr"]
-    res["mixed_vs_fp64_rel_error"] = float((a - b).abs().max() / b.abs().max())
     if keep_solutions:
-        res["_u_mixed"], res["_u_csr"] = a, b
-    res["dof_iters_speedup"] = res["mixed"]["dof_iters_per_s"] / res["csr"]["dof_iters_per_s"]
-    res["time_to_solution_speedup"] = res["csr"]["seconds"] / res["mixed"]["seconds"]
+        for mode in modes:
+            res[f"_u_{mode}"] = sols[mode]
+    if "csr" in sols and "mixed" in sols:
+        a, b = sols["mixed"], sols["csr"]
+        res["mixed_vs_fp64_rel_error"] = float((a - b).abs().max() / b.abs().max())
+        res["dof_iters_speedup"] = res["mixed"]["dof_iters_per_s"] / res["csr"]["dof_iters_per_s"]
+        res["time_to_solution_speedup"] = res["csr"]["seconds"] / res["mixed"]["seconds"]
     return res
 
 
@@ -281,7 +284,7 @@ def cpu_baseline(K_host, b_host, seconds=12.0, max_iters=400):
     return A.shape[0] * it / dt, it, dt
 
 
-def large_mixed(workload, oracle_check=True, upto=None):
+def large_mixed(workload, oracle_check=True, upto=None, modes=("csr", "mixed")):
     """C3 (Tet4 Neo-Hookean 96^3 x 6, jittered, 10 load steps to stretch 1.3) or
     C5 (Hex8 Neo-Hookean 160^3, 5 load steps to stretch 1.3): mixed Newton vs
     the FP64 colored-CSR Newton on the same problem, both to 1e-8 relative
@@ -304,7 +307,7 @@ def large_mixed(workload, oracle_check=True, upto=None):
     # capture, workspaces) is a negligible share of the timed solve
     done = min(upto or steps, steps)
     res = mixed_vs_fp64(NH["E"], L0, lambda sv: solve_load_steps(NH, sv, steps, stretch, done),
-                        keep_solutions=True, warmup=False)
+                        keep_solutions=True, warmup=False, modes=modes)
     mesh = NH["mesh"]
     res["workload"] = (f"{kind} Neo-Hookean {n}^3{' x 6 tets' if kind == 'tet4' else ''}, "
                        f"mu=1, lambda=10, jitter={jitter}h (seed 0), x=0 clamped, u_x on x=1 "
@@ -328,7 +331,7 @@ def large_mixed(workload, oracle_check=True, upto=None):
         u_load = L_end.lift_from_zeros(np.zeros(free.size)).cpu().numpy()
         r_load = np.linalg.norm(Q.residual(u_load)[free])
         res["oracle_load_residual_norm"] = float(r_load)
-        for mode in ("mixed", "csr"):
+        for mode in modes:
             uh = res.pop(f"_u_{mode}").cpu().numpy()
             r = Q.residual(uh)[free]
             res[mode]["oracle_final_residual_norm"] = float(np.linalg.norm(r))
@@ -756,8 +759,10 @@ def main(argv=None):
     if args.n is None:
         args.n = 64
     if args.workload in ("c3", "c5"):
-        r = large_mixed(args.workload, oracle_check=not args.no_cpu_baseline,
-                        upto=args.large_steps)
+        r = large_mixed(args.workload, oracle_check=not args.no_oracle,
+                        upto=args.large_steps,
+                        modes=("mixed",) if args.no_fp64 else ("csr", "mixed"))
+        r["lin_cache"] = args.lin_cache
         print(json.dumps({"workload": args.workload, **r}), flush=True)
         return 0
     if args.workload == "c4":

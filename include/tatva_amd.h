@@ -208,6 +208,18 @@ int tatva_mass_matrix(tatva_mesh *mesh, int32_t ncomp, const int32_t *dof_map,
                       void *stream);
 int tatva_project_rhs(tatva_mesh *mesh, int32_t nc, const double *quad_values, double *out,
                       void *stream);
+/* The projection's mass matrix under a general lifter (Operator.project with a
+ * Periodic constraint, tatva/utils.py:225-246): the Jacobian of the lifted
+ * bilinear form, rows of the free test dofs, columns through the lift's read map
+ * (a slave's column added to its master's): E_free^T M P.  Reduced row r sums
+ * the full rows row_list[row_ptr[r] .. row_ptr[r+1]) (the projection passes its
+ * own free dof only); flags & 1 as in tatva_colored_hessian_mapped.  Not
+ * symmetric in general: solve it with tatva_bicgstab_csr. */
+int tatva_mass_matrix_mapped(tatva_mesh *mesh, int32_t ncomp, const int32_t *read_map,
+                             const int32_t *row_ptr, const int32_t *row_list, int64_t n_rows,
+                             const int32_t *colors, int32_t n_colors, const int32_t *indptr,
+                             const int32_t *indices, int64_t nnz, int32_t flags, double *data,
+                             void *workspace, int64_t workspace_bytes, void *stream);
 
 /* Operator.interpolate (tatva/operator.py:399-468 with find_containing_polygons,
  * tatva/mesh.py:295-390) on a 2-D Tri3 mesh: out[p, c] = nodal values (N x nc)
@@ -263,6 +275,16 @@ int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *
                   const double *b, double *x, double tol, int32_t maxiter, int32_t chunk,
                   void *workspace, int64_t workspace_bytes, tatva_pcg_stats *stats,
                   void *stream);
+
+/* Jacobi-preconditioned BiCGSTAB on a general CSR matrix in FP64, x in place (x0
+ * on entry), until ||b - A x|| <= tol ||b||; stats as tatva_pcg_csr.  It replaces
+ * the reference's spsolve of a non-symmetric projection matrix
+ * (tatva/utils.py:186-198).  Host-driven: returns after the solve. */
+int64_t tatva_bicgstab_csr_workspace(int64_t n);
+int tatva_bicgstab_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
+                       const double *data, const double *b, double *x, double tol,
+                       int32_t maxiter, void *workspace, int64_t workspace_bytes,
+                       tatva_pcg_stats *stats, void *stream);
 
 /* Jacobi-PCG with the matrix-free FP32 Hessian-vector product of (mesh, density)
  * at the FP32 state u (full vector); b, x, diag are reduced FP32 vectors.  The

@@ -857,8 +857,12 @@ def interpolate(elem, coords, elements, nodal, points, batched=False):
 def project(elem, coords, elements, field_q, lifter=None):
     """L2 projection (operator.py:523-554, utils.py:118-262) with the scalar
     mass matrix and one right-hand side per component: M_ab = sum N_a N_b w,
-    b_a = sum N_a f w; with a lifter, rows/columns of the free dofs, solution
-    lifted from zeros.  Dense assembly; for small meshes."""
+    b_a = sum N_a f w.  With a lifter (utils.py:225-246) the test functions sit
+    at the free dofs and the trial field is lifted, so the matrix is the
+    Jacobian of the lifted form, E_free^T M P with P the linear part of the lift
+    (a Periodic slave's column added to its master's; for Fixed constraints only
+    it is M[free][:, free]); the solution is lifted from zeros.  Dense assembly;
+    for small meshes."""
     w = integration_weights(elem, coords, elements)                 # (E, Q)
     Nq = np.stack([elem.N(xi) for xi in elem.quad_points])         # (Q, nne)
     n = coords.shape[0]
@@ -875,7 +879,8 @@ def project(elem, coords, elements, field_q, lifter=None):
         x = np.linalg.solve(M, b)
     else:
         fr = lifter.free
-        xr = np.linalg.solve(M[np.ix_(fr, fr)], b[fr])
+        P = np.stack([lifter.lift_linear(e) for e in np.eye(fr.size)], axis=1)   # (n, n_free)
+        xr = np.linalg.solve(M[fr] @ P, b[fr])
         x = np.stack([lifter.lift_from_zeros(xr[:, c]) for c in range(xr.shape[1])], axis=1)
     return x.reshape((n,) + vs)
 

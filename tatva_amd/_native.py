@@ -111,6 +111,12 @@ _SIGS = {
     "tatva_cast_f64_f32": (c_int, [c_i64, c_dbl, c_void, c_void, c_void]),
     "tatva_cast_f32_f64": (c_int, [c_i64, c_dbl, c_void, c_void, c_void]),
     "tatva_dot_workspace": (c_i64, [c_i64]),
+    "tatva_mass_matrix_mapped": (c_int, [c_void, c_i32, c_void, c_void, c_void, c_i64, c_void,
+                                         c_i32, c_void, c_void, c_i64, c_i32, c_void, c_void,
+                                         c_i64, c_void]),
+    "tatva_bicgstab_csr_workspace": (c_i64, [c_i64]),
+    "tatva_bicgstab_csr": (c_int, [c_i64, c_i64, c_void, c_void, c_void, c_void, c_void, c_dbl,
+                                   c_i32, c_void, c_i64, ctypes.POINTER(PcgStats), c_void]),
     "tatva_check_distance2_colors": (c_int, [c_i64, c_void, c_void, c_void,
                                              ctypes.POINTER(c_i64)]),
     "tatva_dot": (c_int, [c_i64, c_void, c_void, c_void, c_void, c_i64, c_void]),

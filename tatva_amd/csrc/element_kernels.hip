@@ -1494,6 +1494,46 @@ extern "C" int tatva_mass_matrix(tatva_mesh *m, int32_t ncomp, const int32_t *do
   });
 }
 
+extern "C" int tatva_mass_matrix_mapped(tatva_mesh *m, int32_t ncomp, const int32_t *read_map,
+                                        const int32_t *row_ptr, const int32_t *row_list,
+                                        int64_t n_rows, const int32_t *colors, int32_t n_colors,
+                                        const int32_t *indptr, const int32_t *indices,
+                                        int64_t nnz, int32_t flags, double *data,
+                                        void *workspace, int64_t ws_bytes, void *stream) {
+  MESH_PRELUDE(m);
+  TATVA_REQUIRE(read_map && row_ptr && row_list && colors && indptr && indices && data,
+                "invalid arguments");
+  TATVA_REQUIRE(ncomp >= 1 && ncomp <= 3, "ncomp must be in [1, 3]");
+  TATVA_REQUIRE(n_colors >= 1 && n_colors <= kMaxColors, "n_colors must be in [1, 2048]");
+  TATVA_REQUIRE(workspace && ws_bytes >= tatva_colored_hessian_workspace(m, ncomp),
+                "mass matrix workspace too small");
+  (void)nnz;
+  hipStream_t s = (hipStream_t)stream;
+  double *Ke = (double *)workspace;
+  const int rows_per_block = kBlock / kWave;
+  const size_t lds = (size_t)rows_per_block * n_colors * sizeof(double);
+  const int64_t nblk = (n_rows + rows_per_block - 1) / rows_per_block;
+  TATVA_REQUIRE(nblk < (int64_t(1) << 31), "too many rows");
+  return with_kind(m->kind, [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    constexpr int NNE = Elem<K>::NNE;
+    auto run = [&](auto ncc) -> int {
+      constexpr int NC = decltype(ncc)::value;
+      k_mass_stiffness<K, NC><<<grid_for(m->n_elements), kBlock, 0, s>>>(m->n_elements, m->coords,
+                                                                           m->conn, m->quad, Ke);
+      if (n_rows > 0)
+        k_colored_rows_mapped<NNE, NC><<<(int)nblk, kBlock, lds, s>>>(
+            n_rows, row_ptr, row_list, read_map, colors, n_colors, indptr, indices, m->inv_ptr,
+            m->inv_list, m->conn, Ke, data, flags & 1);
+      TATVA_CHECK_LAUNCH();
+      return TATVA_OK;
+    };
+    if (ncomp == 1) return run(std::integral_constant<int, 1>{});
+    if (ncomp == 2) return run(std::integral_constant<int, 2>{});
+    return run(std::integral_constant<int, 3>{});
+  });
+}
+
 extern "C" int tatva_project_rhs(tatva_mesh *m, int32_t nc, const double *quad_values,
                                  double *out, void *stream) {
   MESH_PRELUDE(m);

@@ -1705,6 +1705,165 @@ extern "C" int tatva_dot(int64_t n, const double *x, const double *y, double *ou
   return TATVA_OK;
 }
 
+// ---------------------------------------------------------------- BiCGSTAB --
+//
+// Jacobi-preconditioned BiCGSTAB (H. A. van der Vorst, SIAM J. Sci. Stat.
+// Comput. 13, 1992, right preconditioning) for the non-symmetric systems of the
+// path: the reduced mass matrix E_free^T M P of an L2 projection under Periodic
+// constraints.  Host-driven: the recurrence scalars are read back once per
+// reduction (these solves are small and set-up only).
+
+__global__ __launch_bounds__(kBlock) void k_lin3(int64_t n, double a, const double *__restrict__ x,
+                                                 double b, const double *__restrict__ y, double c,
+                                                 const double *__restrict__ z,
+                                                 double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = a * x[i] + b * y[i];
+  if (z) v += c * z[i];
+  out[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(int64_t n, const double *__restrict__ d,
+                                                  const double *__restrict__ x,
+                                                  double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = d[i] * x[i];
+}
+
+// dinv[i] = 1 / A_ii (1 where the row has no diagonal entry or it is zero)
+__global__ __launch_bounds__(kBlock) void k_diag_inv_plain(int64_t n, const int32_t *__restrict__ indptr,
+                                                           const int32_t *__restrict__ indices,
+                                                           const double *__restrict__ data,
+                                                           double *__restrict__ dinv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k)
+    if (indices[k] == i) d = data[k];
+  dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+constexpr int kBicgVecs = 9;  // r, rh, p, v, s, t, ph, sh, dinv
+
+extern "C" int64_t tatva_bicgstab_csr_workspace(int64_t n) {
+  if (n <= 0) return -1;
+  return (int64_t)(align256((size_t)kBicgVecs * n * sizeof(double)) +
+                   align256((kMaxReduceBlocks + 8) * sizeof(double)));
+}
+
+extern "C" int tatva_bicgstab_csr(int64_t n, int64_t nnz, const int32_t *indptr,
+                                  const int32_t *indices, const double *data, const double *b,
+                                  double *x, double tol, int32_t maxiter, void *workspace,
+                                  int64_t ws_bytes, tatva_pcg_stats *stats, void *stream) {
+  TATVA_REQUIRE(n > 0 && nnz > 0 && indptr && indices && data && b && x && maxiter >= 0,
+                "invalid arguments");
+  TATVA_REQUIRE(tol >= 0.0, "tol must be non-negative");
+  TATVA_REQUIRE(workspace && ws_bytes >= tatva_bicgstab_csr_workspace(n),
+                "BiCGSTAB workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  double *w = static_cast<double *>(workspace);
+  double *r = w, *rh = w + n, *p = w + 2 * n, *v = w + 3 * n, *sv = w + 4 * n, *t = w + 5 * n;
+  double *ph = w + 6 * n, *sh = w + 7 * n, *dinv = w + 8 * n;
+  double *parts = reinterpret_cast<double *>(static_cast<char *>(workspace) +
+                                             align256((size_t)kBicgVecs * n * sizeof(double)));
+  double *dots = parts + kMaxReduceBlocks;
+  const int g = grid_for(n), gr = reduce_grid(n);
+  const int lpr = choose_lpr(n, nnz);
+  auto spmv = [&](const double *xin, const double *bin, double *yout) -> int {
+    return with_lpr(lpr, [&](auto lc) -> int {
+      constexpr int LL = decltype(lc)::value;
+      k_spmv<LL><<<spmv_grid(n, LL), kBlock, 0, st>>>(n, indptr, indices, data, xin, bin, yout);
+      TATVA_CHECK_LAUNCH();
+      return TATVA_OK;
+    });
+  };
+  // dots[slot] = a . b on the stream
+  auto dot = [&](const double *a, const double *bb, int slot) {
+    k_dot_part<<<gr, kBlock, 0, st>>>(n, a, bb, parts);
+    k_finalize_sum<double><<<1, kBlock, 0, st>>>(parts, gr, dots + slot, 0);
+  };
+  double h[8];
+  auto read = [&](int count) -> int {
+    TATVA_CHECK_LAUNCH();
+    TATVA_TRY_HIP(hipMemcpyAsync(h, dots, count * sizeof(double), hipMemcpyDeviceToHost, st));
+    TATVA_TRY_HIP(hipStreamSynchronize(st));
+    return TATVA_OK;
+  };
+  k_diag_inv_plain<<<g, kBlock, 0, st>>>(n, indptr, indices, data, dinv);
+  int rc = spmv(x, b, r);  // r = b - A x
+  if (rc) return rc;
+  TATVA_TRY_HIP(hipMemcpyAsync(rh, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  dot(b, b, 0);
+  dot(r, r, 1);
+  if ((rc = read(2))) return rc;
+  const double bb = h[0], tol2 = tol * tol * bb;
+  double rr = h[1], rho_old = 1.0, alpha = 1.0, omega = 1.0;
+  int status = TATVA_ENOTCONV, it = 0;
+  if (bb == 0.0 || rr <= tol2) status = TATVA_OK;
+  while (status == TATVA_ENOTCONV && it < maxiter) {
+    dot(rh, r, 0);
+    if ((rc = read(1))) return rc;
+    const double rho = h[0];
+    if (!(std::fabs(rho) > 0.0) || !std::isfinite(rho)) {
+      status = TATVA_EBREAKDOWN;
+      break;
+    }
+    if (it == 0) {
+      TATVA_TRY_HIP(hipMemcpyAsync(p, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      const double beta = (rho / rho_old) * (alpha / omega);
+      k_lin3<<<g, kBlock, 0, st>>>(n, 1.0, r, beta, p, -beta * omega, v, p);
+    }
+    k_scale<<<g, kBlock, 0, st>>>(n, dinv, p, ph);
+    if ((rc = spmv(ph, nullptr, v))) return rc;
+    dot(rh, v, 0);
+    if ((rc = read(1))) return rc;
+    if (!(std::fabs(h[0]) > 0.0) || !std::isfinite(h[0])) {
+      status = TATVA_EBREAKDOWN;
+      break;
+    }
+    alpha = rho / h[0];
+    k_lin3<<<g, kBlock, 0, st>>>(n, 1.0, r, -alpha, v, 0.0, nullptr, sv);
+    dot(sv, sv, 0);
+    if ((rc = read(1))) return rc;
+    ++it;
+    if (h[0] <= tol2) {  // x += alpha ph
+      k_lin3<<<g, kBlock, 0, st>>>(n, 1.0, x, alpha, ph, 0.0, nullptr, x);
+      rr = h[0];
+      status = TATVA_OK;
+      break;
+    }
+    k_scale<<<g, kBlock, 0, st>>>(n, dinv, sv, sh);
+    if ((rc = spmv(sh, nullptr, t))) return rc;
+    dot(t, sv, 0);
+    dot(t, t, 1);
+    if ((rc = read(2))) return rc;
+    if (!(h[1] > 0.0) || !std::isfinite(h[0])) {
+      status = TATVA_EBREAKDOWN;
+      break;
+    }
+    omega = h[0] / h[1];
+    k_lin3<<<g, kBlock, 0, st>>>(n, 1.0, x, alpha, ph, omega, sh, x);
+    k_lin3<<<g, kBlock, 0, st>>>(n, 1.0, sv, -omega, t, 0.0, nullptr, r);
+    dot(r, r, 0);
+    if ((rc = read(1))) return rc;
+    rr = h[0];
+    if (rr <= tol2) status = TATVA_OK;
+    else if (!(std::fabs(omega) > 0.0)) status = TATVA_EBREAKDOWN;
+    rho_old = rho;
+  }
+  TATVA_CHECK_LAUNCH();
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->iterations = it;
+    stats->status = status;
+    stats->b_norm = std::sqrt(bb);
+    stats->rel_residual = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+  }
+  return TATVA_OK;
+}
+
 extern "C" int tatva_rows_cols(int64_t n, const int32_t *indptr, const int32_t *indices,
                                const int32_t *colors, int32_t *rows, int32_t *col_colors,
                                void *stream) {

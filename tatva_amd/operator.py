@@ -317,17 +317,20 @@ class Operator:
         (or with a one-dof-per-node pattern) every component is a separate
         right-hand side of the scalar mass matrix; a pattern with ``dim`` dofs
         per node couples the ``dim`` trailing components (block-diagonal mass).
-        With a ``lifter`` the system is solved on its free dofs and the result
-        lifted (constraint values imposed, as the reference does).  The linear
-        solves are Jacobi-PCG to ``tol`` relative residual (the reference uses a
-        direct solve; the mass matrix is well conditioned).
+        With a ``lifter`` the system is the reference's (utils.py:225-246): test
+        functions at the free dofs, the trial field lifted, so the matrix is the
+        Jacobian E_free^T M P of the lifted form (a Periodic slave's column added
+        to its master's) and the result is lifted (constraint values imposed).
+        The linear solves run to ``tol`` relative residual (the reference uses a
+        direct solve; the mass matrix is well conditioned): Jacobi-PCG when the
+        matrix is symmetric (no lifter, or Fixed constraints only), else
+        Jacobi-BiCGSTAB.
         """
         from . import sparse as sp  # noqa: PLC0415
-        from .solvers import pcg  # noqa: PLC0415
+        from .solvers import bicgstab, pcg  # noqa: PLC0415
 
         t = nat.torch()
-        if lifter is not None and lifter.has_periodic:
-            raise NotImplementedError("project supports Fixed constraints only")
+        mapped = lifter is not None and lifter.has_periodic
         f = nat.to_device(field_q)
         if f.ndim < 2 or f.shape[0] != self.n_elements or f.shape[1] != self.n_quad:
             raise ValueError("field must be shaped (n_elements, n_quad, ...)")
@@ -355,12 +358,24 @@ class Operator:
         ws = t.empty(int(L.tatva_colored_hessian_workspace(self.handle, dim)), dtype=t.uint8,
                      device=f.device)
         data = t.empty(cm.nnz, dtype=t.float64, device=f.device)
-        dmap = lifter.dof_map_device() if lifter is not None else None
-        r2f = lifter.free_device() if lifter is not None else None
-        nat.check(L.tatva_mass_matrix(self.handle, dim, nat.dptr(dmap), nat.dptr(r2f), cm.shape[0],
-                                      nat.dptr(cm.colors), cm.n_colors, nat.dptr(cm.indptr),
-                                      nat.dptr(cm.indices), cm.nnz, nat.dptr(data), nat.dptr(ws),
-                                      ws.numel(), nat.stream()))
+        if mapped:
+            # rows: each free test dof alone; columns: through the lift's read map
+            n_red = lifter.size_reduced
+            row_ptr = t.arange(n_red + 1, dtype=t.int32, device=f.device)
+            row_list = lifter.free_device()
+            flag = _periodic_serial_flag(self.mesh.elements, lifter, dim)
+            nat.check(L.tatva_mass_matrix_mapped(
+                self.handle, dim, nat.dptr(lifter.read_map_device()), nat.dptr(row_ptr),
+                nat.dptr(row_list), cm.shape[0], nat.dptr(cm.colors), cm.n_colors,
+                nat.dptr(cm.indptr), nat.dptr(cm.indices), cm.nnz, flag, nat.dptr(data),
+                nat.dptr(ws), ws.numel(), nat.stream()))
+        else:
+            dmap = lifter.dof_map_device() if lifter is not None else None
+            r2f = lifter.free_device() if lifter is not None else None
+            nat.check(L.tatva_mass_matrix(self.handle, dim, nat.dptr(dmap), nat.dptr(r2f),
+                                          cm.shape[0], nat.dptr(cm.colors), cm.n_colors,
+                                          nat.dptr(cm.indptr), nat.dptr(cm.indices), cm.nnz,
+                                          nat.dptr(data), nat.dptr(ws), ws.numel(), nat.stream()))
         from dataclasses import replace  # noqa: PLC0415
         M = replace(cm, data=data)
         vals = f.reshape(self.n_elements, self.n_quad, nvals).contiguous()
@@ -374,7 +389,10 @@ class Operator:
             bk = rhs[k].contiguous()
             if lifter is not None:
                 bk = lifter.reduce(bk)      # test functions injected at the free dofs
-            xk, info = pcg(M, bk, tol=tol, block_size=dim if dim in (2, 3) else None)
+            if mapped:
+                xk, info = bicgstab(M, bk, tol=tol)
+            else:
+                xk, info = pcg(M, bk, tol=tol, block_size=dim if dim in (2, 3) else None)
             if lifter is not None:
                 xk = lifter.lift_from_zeros(xk)
             out[k] = xk
@@ -386,6 +404,17 @@ class Operator:
         if self._dev is not None:
             self._dev.close()
             object.__setattr__(self, "_dev", None)
+
+
+def _periodic_serial_flag(elements, lifter, nc) -> int:
+    """1 when some element holds two dofs that read the same reduced index (a
+    periodic slave and its master in one element): the mapped decompression
+    then accumulates an element's columns one at a time."""
+    el = np.asarray(elements, dtype=np.int64)
+    dofs = (el[:, :, None] * nc + np.arange(nc)).reshape(el.shape[0], -1)
+    rm = lifter.read_map[dofs]
+    rs = np.sort(np.where(rm >= 0, rm, -1 - np.arange(rm.shape[1])[None, :]), axis=1)
+    return int(np.any(rs[:, 1:] == rs[:, :-1]))
 
 
 __all__ = ["Operator"]

@@ -100,6 +100,28 @@ def pcg(A: ColoredMatrix, b, x0=None, *, tol: float = 1e-10, maxiter: int | None
     return x, info
 
 
+def bicgstab(A: ColoredMatrix, b, x0=None, *, tol: float = 1e-10, maxiter: int | None = None,
+             raise_on_failure: bool = True):
+    """Solve A x = b for a general (non-symmetric) CSR matrix, FP64,
+    Jacobi-preconditioned BiCGSTAB; returns (x, PcgInfo)."""
+    t = nat.torch()
+    n = A.shape[0]
+    bv = nat.to_device(b).reshape(-1).contiguous()
+    x = t.zeros(n, dtype=t.float64, device=bv.device) if x0 is None else \
+        nat.to_device(x0).reshape(-1).clone()
+    L = nat.lib()
+    ws = t.empty(int(L.tatva_bicgstab_csr_workspace(n)), dtype=t.uint8, device=bv.device)
+    st = nat.PcgStats()
+    nat.check(L.tatva_bicgstab_csr(n, A.nnz, nat.dptr(A.indptr), nat.dptr(A.indices),
+                                   nat.dptr(A.data), nat.dptr(bv), nat.dptr(x), float(tol),
+                                   10 * n if maxiter is None else int(maxiter), nat.dptr(ws),
+                                   ws.numel(), ctypes.byref(st), nat.stream()))
+    info = _stats_to_info(st)
+    if raise_on_failure and info.status != nat.OK:
+        nat.check(info.status)
+    return x, info
+
+
 def pcg_matfree(energy: ReducedEnergy, u_full32, b32, x0=None, *, diag32=None, tol: float = 1e-4,
                 maxiter: int | None = None, chunk: int = 16, workspace=None):
     """FP32 matrix-free Jacobi-PCG for H(u) x = b on the free dofs; returns (x32, PcgInfo).
@@ -437,4 +459,4 @@ class Newton:
         return stats
 
 
-__all__ = ["pcg", "pcg_matfree", "PcgInfo", "Newton", "NewtonStats", "norm", "set_profiling"]
+__all__ = ["pcg", "bicgstab", "pcg_matfree", "PcgInfo", "Newton", "NewtonStats", "norm", "set_profiling"]

@@ -607,6 +607,42 @@ def test_project_reproduces_linear_and_fe_fields():
     assert rel_err(got, want) < 1e-9
 
 
+def test_project_under_periodic_constraints():
+    """Operator.project with Fixed and Periodic constraints (utils.py:225-246):
+    the reference's system E_free^T M P (test functions at the free dofs, the
+    trial field lifted: not symmetric, solved by BiCGSTAB) against the oracle's
+    dense solve, 1e-9, for a constrained field (reproduced exactly) and for a
+    random one with non-zero Fixed values (whose constant part the reference's
+    system leaves out); every slave equals its master."""
+    qp, qw = _tri3_quadratic_rule()
+    m = ta.Mesh.unit_square(6, 5)
+    op = ta.Operator(m, ta.Tri3(qp, qw))
+    tri = fem.Tri3(qp, qw)
+    X = m.coords
+    left = np.where(np.isclose(X[:, 0], 0.0))[0]
+    left = left[np.argsort(X[left, 1])]
+    right = np.where(np.isclose(X[:, 0], 1.0))[0]
+    right = right[np.argsort(X[right, 1])]
+    bottom = np.where(np.isclose(X[:, 1], 0.0))[0]
+    slaves = np.setdiff1d(left, bottom)
+    masters = right[np.isin(left, slaves)]
+    g = np.cos(2 * np.pi * X[:, 0]) * X[:, 1] + X[:, 1] ** 2
+    g[slaves] = g[masters]
+    for val in (0.0, 0.3):
+        L = lifter.Lifter(m.n_nodes, lifter.Fixed(bottom, val), lifter.Periodic(slaves, masters))
+        Lo = fem.Lifter(m.n_nodes, fem.Fixed(bottom, val), fem.Periodic(slaves, masters))
+        rng = np.random.default_rng(int(val * 10))
+        nodal = g if val == 0.0 else rng.standard_normal(m.n_nodes)
+        fq = fem.op_eval(tri, X, m.elements, nodal)
+        got = to_np(op.project(fq, lifter=L))
+        want = fem.project(tri, X, m.elements, fq, lifter=Lo)
+        assert rel_err(got, want) < 1e-9, val
+        np.testing.assert_array_equal(got[slaves], got[masters])
+        np.testing.assert_array_equal(got[bottom], val)
+        if val == 0.0:
+            assert rel_err(got, g) < 1e-9
+
+
 def test_interpolate_matches_oracle():
     """operator.py:399-468 on a jittered Tri3 mesh: interpolated values against
     the oracle (same containing element, same Newton step); linear fields exact;

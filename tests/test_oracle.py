@@ -601,3 +601,45 @@ def test_per_element_parameters_split_into_uniform_problems(kind):
         assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
     K = P.element_stiffness(u)
     assert np.array_equal(K[inA], PA.element_stiffness(u)) and E == K.shape[0]
+
+
+def test_projection_under_periodic_constraints_reproduces_constrained_fields():
+    """The oracle's projection with Fixed and Periodic constraints (the reference's
+    E_free^T M P system, utils.py:225-246): a nodal field that satisfies the
+    constraints (slaves equal their masters, zero at the fixed dofs), sampled at
+    the quadrature points, is reproduced exactly; the matrix is not symmetric;
+    with Fixed constraints only it is M[free][:, free].  (The reference's system
+    leaves out the constant part of the lift: with non-zero Fixed values the
+    projected field is not reproduced there, and the oracle keeps that.)"""
+    X, el = fem.unit_square_tri3(5, 4)
+    n = X.shape[0]
+    left = np.where(np.isclose(X[:, 0], 0.0))[0]
+    left = left[np.argsort(X[left, 1])]
+    right = np.where(np.isclose(X[:, 0], 1.0))[0]
+    right = right[np.argsort(X[right, 1])]
+    bottom = np.where(np.isclose(X[:, 1], 0.0))[0]
+    slaves = np.setdiff1d(left, bottom)
+    masters = right[np.isin(left, slaves)]
+    g = np.cos(2 * np.pi * X[:, 0]) * X[:, 1] + X[:, 1] ** 2
+    g[slaves] = g[masters]
+    assert np.all(g[bottom] == 0.0)
+    L = fem.Lifter(n, fem.Fixed(bottom, 0.0), fem.Periodic(slaves, masters))
+    rule = np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]), np.full(3, 1 / 6)
+    tri = fem.Tri3(*rule)
+    gq = fem.op_eval(tri, X, el, g)
+    out = fem.project(tri, X, el, gq, lifter=L)
+    assert np.abs(out - g).max() <= 1e-12 * np.abs(g).max()
+    # the system matrix is not symmetric
+    w = fem.integration_weights(tri, X, el)
+    Nq = np.stack([tri.N(xi) for xi in tri.quad_points])
+    M = np.zeros((n, n))
+    Me = np.einsum("eq,qa,qb->eab", w, Nq, Nq)
+    for e, nodes in enumerate(el):
+        M[np.ix_(nodes, nodes)] += Me[e]
+    P = np.stack([L.lift_linear(e) for e in np.eye(L.size_reduced)], axis=1)
+    A = M[L.free] @ P
+    assert np.abs(A - A.T).max() > 1e-3 * np.abs(A).max()
+    # Fixed only: the restricted mass matrix
+    LF = fem.Lifter(n, fem.Fixed(bottom, 0.0))
+    PF = np.stack([LF.lift_linear(e) for e in np.eye(LF.size_reduced)], axis=1)
+    assert np.array_equal(M[LF.free] @ PF, M[np.ix_(LF.free, LF.free)])
