@@ -36,6 +36,31 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of the part: one wave64 FP32/FP64 VALU instruction per SIMD every
+# four cycles, 1,024 SIMDs (256 CUs x 4) at 2.4 GHz, in G wave-instructions/s
+VALU_PEAK_GINST = 1024 * 2.4 / 4
+
+# Counters of the headline's dominant kernel from its rocprofv3 profile (per launch):
+# VALU wave-instructions (SQ_INSTS_VALU) and HBM bytes (FETCH_SIZE x 2 + WRITE_SIZE x 1,
+# the factors of a plain 512 MiB device copy under the same counters).  The profile
+# holds them as "key: value" lines; the bench divides them by the live launch time.
+HEADLINE_PROFILE = os.path.join(ROOT, "profiles", "k_tile_assemble_hvp_hex8_nh_160cubed.md")
+
+
+def profile_counters(path):
+    """{key: float} of the "- key: value" lines of a committed profile summary."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("- ") and ":" in line:
+            k, v = line[2:].split(":", 1)
+            try:
+                out[k.strip()] = float(v.split()[0].replace(",", ""))
+            except (ValueError, IndexError):
+                pass
+    return out
 
 # HBM bytes per launch of the node-block SpMV on the 64^3 line, from the two
 # rocprofv3 PMC passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
@@ -529,11 +554,15 @@ def headline(args):
         res["oracle_seconds"] = time.perf_counter() - t5
         for key, r in zip(("mixed", "fp64_csr"), rs[1:]):
             res[key]["oracle_rel_residual"] = float(np.linalg.norm(r[free]) / r_load)
-    # the PCG reads the cached linearisation (F^-1 and mu - lambda ln J: 10 floats per
-    # quadrature point) instead of the state
+    # with the cache the PCG reads the linearisation (F^-1 and mu - lambda ln J: 10
+    # floats per quadrature point) instead of the state
     cached = args.lin_cache != "never"     # the headline mesh is Hex8
     tile_bytes, lay = hvp_tile_bytes(op, cached_lin=10 if cached else 0)
-    achieved = tile_bytes / (tile_us * 1e-6) / 1e9 if tile_us else None
+    hbm_rate = tile_bytes / (tile_us * 1e-6) / 1e9 if tile_us else None
+    prof = profile_counters(HEADLINE_PROFILE) if (n == 160 and not cached) else {}
+    valu = prof.get("valu_wave_instructions_per_launch")
+    traffic = prof.get("hbm_bytes_per_launch")
+    achieved = valu / (tile_us * 1e-6) / 1e9 if (valu and tile_us) else None
     out = {
         "metric": "Newton DOF-iters/s + HBM GB/s (mixed vs FP64 @1e-8 rel); serve p50/p99 ms",
         "value": value,
@@ -555,20 +584,29 @@ def headline(args):
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "k_tile_assemble<HEX8, NeoHookean<3,float>, float, kModeHvpLin, affine> "
-                      "(FP32 HVP tile kernel of the matrix-free PCG operator, reading the "
-                      "cached linearisation)",
-            "bound": "hbm",
+            "kernel": (f"k_tile_assemble<HEX8, NeoHookean<3,float>, float, "
+                       f"{'kModeHvpLin' if cached else 'kModeHvp'}, affine> (FP32 HVP tile kernel "
+                       "of the matrix-free PCG operator)"),
+            "bound": "valu",
             "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "peak": VALU_PEAK_GINST,
+            "unit": "G wave64 VALU instructions/s",
+            "frac": (achieved / VALU_PEAK_GINST) if achieved else None,
+            "traffic": traffic,
+            "traffic_source": os.path.relpath(HEADLINE_PROFILE, ROOT) if traffic else None,
+            "valu_wave_instructions_per_launch": valu,
             "algorithmic_bytes": tile_bytes,
+            "hbm_achieved_gbs": hbm_rate,
+            "hbm_frac": hbm_rate / HBM_PEAK_GBS if hbm_rate else None,
             "avg_launch_us": tile_us,
             "operator_us": op_us,
             "samples": op_n,
             "layout": lay,
+            "note": "VALU-issue bound: achieved = the kernel's VALU wave-instructions per launch "
+                    "(rocprofv3 SQ_INSTS_VALU of this workload, in the profile named in "
+                    "traffic_source) / the launch time measured here with HIP events; peak = one "
+                    "wave64 instruction per SIMD per 4 cycles x 1,024 SIMDs x 2.4 GHz.  "
+                    "hbm_achieved_gbs = algorithmic bytes / the same time",
         },
         "cpu_baseline": None,
         "newton_iterations": sum(s.newton_iterations for s in stats),
@@ -693,10 +731,10 @@ def main(argv=None):
                     help="headline: skip the FP64 colored-CSR comparison")
     ap.add_argument("--no-oracle", action="store_true",
                     help="headline: skip the oracle residuals of the final states")
-    ap.add_argument("--lin-cache", choices=("auto", "always", "never"), default="auto",
-                    help="matrix-free PCG: read the density's linearisation from a cache "
-                         "written once per solve (auto: on Hex8 meshes) or re-linearise in "
-                         "every HVP")
+    ap.add_argument("--lin-cache", choices=("auto", "always", "never"), default="never",
+                    help="matrix-free PCG: re-linearise the density in every HVP (never: the "
+                         "default) or read it from a cache written once per solve (auto: on "
+                         "Hex8 meshes; always)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extras", action="store_true",

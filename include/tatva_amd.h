@@ -89,7 +89,7 @@ int tatva_set_profiling(int enable);
  * density once per solve at its fixed state (Neo-Hookean: F^-1 and
  * mu - lambda ln J per quadrature point, FP32, in the workspace) so that every
  * HVP of the solve reads it instead of recomputing it (same arithmetic, same
- * result): 0 never, 1 (default) for Hex8 meshes, 2 always. */
+ * result): 0 (default) never, 1 for Hex8 meshes, 2 always. */
 enum { TATVA_OPT_MATFREE_LIN_CACHE = 1 };
 int tatva_set_option(int32_t option, int32_t value);
 

@@ -295,6 +295,15 @@ __device__ __forceinline__ void hex_dmodes(const S (&M)[8], S x, S y, S z, S (&d
   d[2] = M[4] + M[5] * x + M[6] * y + M[7] * (x * y);
 }
 
+// The same with the point's products given (yz = y * z, xz = x * z, xy = x * y).
+template <class S>
+__device__ __forceinline__ void hex_dmodes(const S (&M)[8], S x, S y, S z, S yz, S xz, S xy,
+                                           S (&d)[3]) {
+  d[0] = M[1] + M[3] * y + M[5] * z + M[7] * yz;
+  d[1] = M[2] + M[3] * x + M[6] * z + M[7] * xz;
+  d[2] = M[4] + M[5] * x + M[6] * y + M[7] * xy;
+}
+
 // Hex8 residual (MODE = kModeResidual: S = P(G), plus the Poisson source) or
 // Hessian-vector product (S = dP(G(u); G(v))): ye = sum_q w_q B_q^T S_q.
 // Geometry in FP64, physics in T.
@@ -307,6 +316,10 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
   constexpr int NC = Dens::NC;
   constexpr bool kUsesU = MODE != kModeHvpLin;
   constexpr bool kUsesV = MODE == kModeHvp || MODE == kModeHvpLin;
+  // kModeHvpLin: the cached linearisation of point q + 1 is loaded while point q
+  // is computed (a point's values are needed only after its tangent gradient)
+  typename Dens::Lin Lnext{};
+  if constexpr (MODE == kModeHvpLin) Lnext = dens.load_lin(lin, kTile);
   double XM[3][8];
   hex_modes<double, 3>(X, XM);
   T UM[NC][8], VM[NC][8];
@@ -317,9 +330,12 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
   for (int c = 0; c < NC; ++c)
 #pragma unroll
     for (int b = 0; b < 8; ++b) A[c][b] = T(0);
-  double inv0[3][3] = {}, det0 = 0.0;
+  // affine: the Jacobian inverse is the same at every point, held in T (the
+  // values each point would cast from the FP64 inverse)
+  T iv0[3][3] = {};
+  double det0 = 0.0;
   if constexpr (AFFINE) {
-    double J[3][3];
+    double J[3][3], inv0[3][3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       J[0][j] = XM[j][1];
@@ -327,15 +343,25 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
       J[2][j] = XM[j][4];
     }
     det0 = det_inv(J, inv0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) iv0[i][j] = T(inv0[i][j]);
   }
   for (int q = 0; q < quad.nq; ++q) {
     const double x = quad.xi[q][0], y = quad.xi[q][1], z = quad.xi[q][2];
+    // the point in T and its products: uniform values (FP32: from the kernel
+    // argument, so they stay scalar operands)
+    T xt, yt, zt, yz, xz, xy, xyz;
+    if constexpr (sizeof(T) == 4) {
+      xt = quad.xf[q][0], yt = quad.xf[q][1], zt = quad.xf[q][2];
+      yz = quad.xf[q][3], xz = quad.xf[q][4], xy = quad.xf[q][5], xyz = quad.xf[q][6];
+    } else {
+      xt = T(x), yt = T(y), zt = T(z);
+      yz = yt * zt, xz = xt * zt, xy = xt * yt, xyz = xt * yt * zt;
+    }
     double inv[3][3], det;
     if constexpr (AFFINE) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) inv[i][j] = inv0[i][j];
       det = det0;
     } else {
       double J[3][3];
@@ -354,15 +380,14 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) iv[i][j] = T(inv[i][j]);
-    const T xt = T(x), yt = T(y), zt = T(z);
+      for (int j = 0; j < 3; ++j) iv[i][j] = AFFINE ? iv0[i][j] : T(inv[i][j]);
     // G[c][j] = sum_k H[c][k] inv[j][k], H = natural derivatives
     T G[NC][3], S[NC][3];
     if constexpr (kUsesU) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         T h[3];
-        hex_dmodes<T>(UM[c], xt, yt, zt, h);
+        hex_dmodes<T>(UM[c], xt, yt, zt, yz, xz, xy, h);
 #pragma unroll
         for (int j = 0; j < 3; ++j) G[c][j] = h[0] * iv[j][0] + h[1] * iv[j][1] + h[2] * iv[j][2];
       }
@@ -380,12 +405,14 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         T h[3];
-        hex_dmodes<T>(VM[c], xt, yt, zt, h);
+        hex_dmodes<T>(VM[c], xt, yt, zt, yz, xz, xy, h);
 #pragma unroll
         for (int j = 0; j < 3; ++j) dG[c][j] = h[0] * iv[j][0] + h[1] * iv[j][1] + h[2] * iv[j][2];
       }
       if constexpr (MODE == kModeHvpLin) {
-        dens.dP(dens.load_lin(lq, kTile), dG, S);
+        const typename Dens::Lin Lq = Lnext;
+        if (q + 1 < quad.nq) Lnext = dens.load_lin(lq + Dens::kLin * kTile, kTile);
+        dens.dP(Lq, dG, S);
       } else {
         const auto L = dens.linearize(G);
         dens.dP(L, dG, S);
@@ -403,7 +430,7 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
       A[c][3] += Q[0] * yt + Q[1] * xt;
       A[c][5] += Q[0] * zt + Q[2] * xt;
       A[c][6] += Q[1] * zt + Q[2] * yt;
-      A[c][7] += Q[0] * (yt * zt) + Q[1] * (xt * zt) + Q[2] * (xt * yt);
+      A[c][7] += Q[0] * yz + Q[1] * xz + Q[2] * xy;
     }
     if constexpr (MODE == kModeResidual && Dens::kSource) {
       // - w f N_a with N_a = (1/8) sum_m phi_m(a) psi_m
@@ -411,11 +438,11 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
       A[0][0] -= s;
       A[0][1] -= s * xt;
       A[0][2] -= s * yt;
-      A[0][3] -= s * (xt * yt);
+      A[0][3] -= s * xy;
       A[0][4] -= s * zt;
-      A[0][5] -= s * (xt * zt);
-      A[0][6] -= s * (yt * zt);
-      A[0][7] -= s * (xt * yt * zt);
+      A[0][5] -= s * xz;
+      A[0][6] -= s * yz;
+      A[0][7] -= s * xyz;
     }
   }
 #pragma unroll
@@ -683,17 +710,50 @@ __global__ __launch_bounds__(kBlock) void k_slot_gather_dot(int64_t n_nodes, con
                                                             const int *__restrict__ skip) {
   if (skip && *skip) return;
   double dot = 0.0;
+  // two nodes per pass (i and i + stride), their slot lists walked side by side
+  // so that the loads of both are in flight together; each node still sums its
+  // slots in ascending tile order
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n_nodes;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    T acc[NC];
-    slot_sum<NC, T>(i, slot_ptr, partial, acc);
+       i += 2 * stride) {
+    const int64_t i1 = i + stride;
+    const bool has1 = i1 < n_nodes;
+    const int32_t b0 = slot_ptr[i], e0 = slot_ptr[i + 1];
+    const int32_t b1 = has1 ? slot_ptr[i1] : 0, e1 = has1 ? slot_ptr[i1 + 1] : 0;
+    T acc0[NC], acc1[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc0[c] = acc1[c] = T(0);
+    const int32_t n0 = e0 - b0, n1 = e1 - b1, nmax = n0 > n1 ? n0 : n1;
+    for (int32_t k = 0; k < nmax; ++k) {
+      if (k < n0) {
+        const T *src = partial + (int64_t)(b0 + k) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc0[c] += src[c];
+      }
+      if (k < n1) {
+        const T *src = partial + (int64_t)(b1 + k) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc1[c] += src[c];
+      }
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int64_t dof = i * NC + c;
       const int64_t r = dof_map ? (int64_t)dof_map[dof] : dof;
       if (r >= 0) {
-        out[r] = acc[c];
-        dot += (double)acc[c] * (double)w[r];
+        out[r] = acc0[c];
+        dot += (double)acc0[c] * (double)w[r];
+      }
+    }
+    if (has1) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int64_t dof = i1 * NC + c;
+        const int64_t r = dof_map ? (int64_t)dof_map[dof] : dof;
+        if (r >= 0) {
+          out[r] = acc1[c];
+          dot += (double)acc1[c] * (double)w[r];
+        }
       }
     }
   }

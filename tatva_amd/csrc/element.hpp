@@ -19,7 +19,19 @@ struct QuadRule {
   int pad;
   double xi[kMaxQuad][3];
   double w[kMaxQuad];
+  // the points in FP32 and their products (x, y, z, yz, xz, xy, xyz), rounded
+  // as a device cast of xi would round them: uniform kernel arguments that an
+  // FP32 kernel uses as scalar operands (set by set_quad_f32)
+  float xf[kMaxQuad][8];
 };
+
+inline void set_quad_f32(QuadRule &q) {
+  for (int i = 0; i < q.nq; ++i) {
+    const float x = (float)q.xi[i][0], y = (float)q.xi[i][1], z = (float)q.xi[i][2];
+    const float v[8] = {x, y, z, y * z, x * z, x * y, x * y * z, 0.0f};
+    for (int k = 0; k < 8; ++k) q.xf[i][k] = v[k];
+  }
+}
 
 // Uniform 2-D grid of point-location cells: n[j] cells of width h[j] from
 // lo[j] along axis j, cell (cx, cy) at index cy * n[0] + cx.
@@ -107,11 +119,20 @@ struct Elem<TATVA_HEX8> {
   }
 };
 
+// 1 / x: correctly rounded in FP64; in FP32 the hardware reciprocal (1 ulp),
+// which the FP32 tangent of the matrix-free product uses.
+template <class T>
+__device__ __forceinline__ T trcp(T x);
+template <>
+__device__ __forceinline__ double trcp<double>(double x) { return 1.0 / x; }
+template <>
+__device__ __forceinline__ float trcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Signed det(J) and inv(J) by cofactors.
 template <class T>
 __device__ __forceinline__ T det_inv(const T J[2][2], T inv[2][2]) {
   const T det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-  const T r = T(1) / det;
+  const T r = trcp<T>(det);
   inv[0][0] = J[1][1] * r;
   inv[0][1] = -J[0][1] * r;
   inv[1][0] = -J[1][0] * r;
@@ -125,7 +146,7 @@ __device__ __forceinline__ T det_inv(const T J[3][3], T inv[3][3]) {
   const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
   const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
   const T det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-  const T r = T(1) / det;
+  const T r = trcp<T>(det);
   inv[0][0] = c00 * r;
   inv[1][0] = c01 * r;
   inv[2][0] = c02 * r;

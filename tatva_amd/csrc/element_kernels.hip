@@ -986,6 +986,7 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
     for (int j = 0; j < 3; ++j) m->quad.xi[q][j] = j < dim ? quad_points[q * dim + j] : 0.0;
     m->quad.w[q] = quad_weights[q];
   }
+  set_quad_f32(m->quad);
   // Uploads go through a private non-blocking stream: mesh creation on a
   // helper thread (a server preparing requests) then does not wait for work
   // queued on the default stream.

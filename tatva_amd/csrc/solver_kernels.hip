@@ -723,7 +723,7 @@ struct DeviceRes {
 static DeviceRes g_res[64];
 static std::mutex g_res_init_mu;  // serialises the first use of a device's resources
 static std::atomic<int> g_profiling{0};
-static std::atomic<int> g_lin_cache{1};  // TATVA_OPT_MATFREE_LIN_CACHE: 0 never, 1 auto, 2 always
+static std::atomic<int> g_lin_cache{0};  // TATVA_OPT_MATFREE_LIN_CACHE: 0 never, 1 Hex8, 2 always
 
 // Drops cached solver graphs that captured this mesh's arrays (called before
 // the mesh's device memory is freed).
@@ -1292,10 +1292,11 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   float *b = ws_vec<float>(wsb, L, 7);
   float *diag = ws_vec<float>(wsb, L, 8);
   float *u = reinterpret_cast<float *>(wsb + matfree_u_offset(n));
-  // the state is fixed during the solve: the density is linearised once.  By
-  // default only for Hex8 (eight quadrature points of tangent work per element
-  // against 40 bytes per point read back); Tet4's one point re-linearises for
-  // less than the cache's bytes cost
+  // the state is fixed during the solve: the density may be linearised once and
+  // read back by every product (option; off by default: on the 160^3 Hex8 cube
+  // the product reading 40 bytes per point ran at 515 us against 433 us for the
+  // one recomputing them, its memory and VALU work not overlapping at 4 waves
+  // per SIMD)
   const int lin_opt = g_lin_cache.load();
   const bool use_lin = lin_opt == 2 || (lin_opt == 1 && m->kind == TATVA_HEX8);
   float *lin = (use_lin && lin_cache_floats(m, d) > 0)
