@@ -63,19 +63,24 @@ def test_pcg_csr_keeps_no_device_memory():
     cm = sparse.ColoredMatrix.from_csr(L.adapt_sparsity(sparse.pattern_from_mesh(m, 1)))
     K = sparse.jacfwd(E.reduced(L).grad, cm)(np.zeros(L.size_reduced))
     b = torch.ones(L.size_reduced, dtype=torch.float64, device="cuda")
+    import gc
+
     solvers.pcg(K, b, tol=1e-10, workspace=solvers._Workspace())  # graphs, kernels loaded
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()
-    free0, _ = torch.cuda.mem_get_info()
+
+    def free_now():
+        gc.collect()      # tensors of earlier tests still waiting for the collector
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        return torch.cuda.mem_get_info()[0]
+
+    free0 = free_now()
     ws = solvers._Workspace()
     x, info = solvers.pcg(K, b, tol=1e-10, workspace=ws)
     assert info.converged
     assert ws.buf.numel() >= 12 * cm.nnz   # the sliced ELL copy is in it
     del ws, x
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()
-    free1, _ = torch.cuda.mem_get_info()
-    assert abs(free1 - free0) <= 2 * 2 ** 20, (free0 - free1)
+    free1 = free_now()
+    assert free1 >= free0 - 2 * 2 ** 20, (free0 - free1)
     # a workspace sized for the block view only is too small for the ELL copy
     Lb = nat.lib()
     n = K.shape[0]
