@@ -379,6 +379,15 @@ int tatva_cast_f32_f64(int64_t n, double alpha, const float *x, double *y, void 
 int64_t tatva_dot_workspace(int64_t n);
 int tatva_dot(int64_t n, const double *x, const double *y, double *out, void *workspace,
               int64_t workspace_bytes, void *stream);
+/* tatva_dot that allocates nothing and takes no workspace.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The partial sums go to a
+ * static device array of the library, 2048 doubles (16 KiB) on each device it
+ * runs on, held until the process ends; the reduction grid is capped at 2048
+ * blocks, so every n >= 1 fits.  Same value as tatva_dot, bit for bit.  All
+ * calls on one device share that array: calls that overlap on one device (two
+ * streams, two threads) race. */
+int tatva_dot_noalloc(int64_t n, const double *x, const double *y, double *out, void *stream);
 
 /* ------------------------------------------------ host-side setup (no GPU) -- */
 

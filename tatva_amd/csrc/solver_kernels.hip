@@ -1706,6 +1706,47 @@ extern "C" int tatva_dot(int64_t n, const double *x, const double *y, double *ou
   return TATVA_OK;
 }
 
+// Scratch of tatva_dot_noalloc: one partial sum per block of the reduction
+// grid, which reduce_grid caps at kMaxReduceBlocks (16 KiB), on every device the
+// library runs on, held until the process ends.  Host code never passes the
+// array's name to a kernel or copy (in host code it is a stand-in, not a device
+// address): each device's address is read once with hipGetSymbolAddress, with
+// that device current, and used from then on.
+static __device__ __align__(256) double scratch[kMaxReduceBlocks];
+static double *g_scratch_dev[64];
+static std::mutex g_scratch_mu;
+
+static int scratch_on(int device, double **out) {
+  TATVA_REQUIRE(device >= 0 && device < 64, "device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    TATVA_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[device];
+  return TATVA_OK;
+}
+
+extern "C" int tatva_dot_noalloc(int64_t n, const double *x, const double *y, double *out,
+                                 void *stream) {
+  TATVA_REQUIRE(n > 0 && x && y && out, "invalid arguments");
+  const int g = reduce_grid(n);
+  TATVA_REQUIRE(g <= kMaxReduceBlocks, "the reduction needs more partial sums than the "
+                                       "no-allocation scratch holds");
+  int device = 0;
+  TATVA_TRY_HIP(hipGetDevice(&device));
+  double *partials = nullptr;
+  const int rc = scratch_on(device, &partials);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // every block writes its partial before the sum reads them: nothing to clear
+  k_dot_part<<<g, kBlock, 0, s>>>(n, x, y, partials);
+  k_finalize_sum<double><<<1, kBlock, 0, s>>>(partials, g, out, 0);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
+}
+
 // ---------------------------------------------------------------- BiCGSTAB --
 //
 // Jacobi-preconditioned BiCGSTAB (H. A. van der Vorst, SIAM J. Sci. Stat.
