@@ -62,11 +62,9 @@ def profile_counters(path):
                 pass
     return out
 
-# HBM bytes per launch of the node-block SpMV on the 64^3 line, from the two
-# rocprofv3 PMC passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
-# plain 512 MiB copy) recorded in profiles/k_pcg_bspmv_hex8_le_64cubed.md.
-SPMV_TRAFFIC_64 = {"fetch_kib": 273681.8, "fetch_factor": 2.000, "write_kib": 6347.5,
-                   "write_factor": 1.000}
+# The C2 line's dominant kernel, the node-block SpMV at 64^3: its profile holds
+# the HBM bytes per launch (FETCH_SIZE x 2 + WRITE_SIZE x 1) as a "- key: value" line.
+C2_PROFILE = os.path.join(ROOT, "profiles", "k_pcg_bspmv_hex8_le_64cubed.md")
 
 
 def build_problem(n):
@@ -184,6 +182,7 @@ def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=Fals
         n_free = solver.red.size
         cg = sum(s.cg_iterations for s in stats)
         op_ms = sum(s.op_ms for s in stats)
+        op_main = sum(s.op_main_ms for s in stats)
         op_n = sum(s.op_samples for s in stats)
         res[mode] = {
             "seconds": dt, "newton_iterations": sum(s.newton_iterations for s in stats),
@@ -192,6 +191,8 @@ def mixed_vs_fp64(E, L0, run, tol=1e-8, colored_matrix=None, keep_solutions=Fals
             "worst_rel_residual": max(s.residual_history[-1] / s.residual_history[0]
                                       for s in stats),
             "operator_us": (op_ms / op_n * 1e3) if op_n else None,
+            # the operator's first kernel alone: the HVP tile kernel / the SpMV
+            "operator_main_kernel_us": (op_main / op_n * 1e3) if op_n else None,
         }
         free1, _total = torch.cuda.mem_get_info()
         res[mode]["device_gb_in_use"] = (free0 - free1) / 1e9
@@ -402,22 +403,28 @@ def device_gb_in_use():
     return (total - free) / 1e9
 
 
-def hvp_tile_bytes(op, dens_ncomp=3, cached_lin=0):
+def mesh_is_affine(op):
+    return bool(op.device_layout().get("hex_affine"))
+
+
+def hvp_tile_bytes(op, dens_ncomp=3, cached_lin=0, geo=False):
     """Algorithmic bytes of one FP32 HVP tile-kernel launch (k_tile_assemble,
     matrix-free PCG operator), from the mesh's device layout: per element its
     nne uint16 local node numbers (+ cached linearisation floats per quadrature
-    point, if any), per (tile, local node) slot its global node (int32), the
-    node's FP64 coordinates, FP32 state and tangent, the tangent's dof-map
-    entries (int32) and the slot's FP32 partial sums written and the int32
-    position it is written to, plus per tile its color offsets and slot range.
-    Coordinates are counted per slot because tiles sharing a node each load it."""
+    point, + its 44-byte geometry cache entry, if any), per (tile, local node)
+    slot its global node (int32), the node's FP64 coordinates (not with the
+    geometry cache), FP32 state and tangent, the tangent's dof-map entries
+    (int32) and the slot's FP32 partial sums written and the int32 position it
+    is written to, plus per tile its color offsets and slot range.  Coordinates
+    are counted per slot because tiles sharing a node each load it."""
     lay = op.device_layout()
     E, S, T = lay["n_elements"], lay["n_slots"], lay["n_tiles"]
     nne = 8 if op.element.kind == 2 else (4 if op.element.kind == 1 else 3)
     d = 3 if nne != 3 else 2
     nc = dens_ncomp
-    b = 2 * nne * E + 4 * cached_lin * lay["n_quad"] * E
-    b += S * (4 + 8 * d + 4 * nc * (0 if cached_lin else 1) + (4 + 4) * nc + 4 * nc + 4)
+    b = 2 * nne * E + 4 * cached_lin * lay["n_quad"] * E + (44 * E if geo else 0)
+    b += S * (4 + (0 if geo else 8 * d) + 4 * nc * (0 if cached_lin else 1) + (4 + 4) * nc
+              + 4 * nc + 4)
     b += T * (2 * (lay["max_colors"] + 1) + 8)
     return int(b), lay
 
@@ -557,9 +564,11 @@ def headline(args):
     # with the cache the PCG reads the linearisation (F^-1 and mu - lambda ln J: 10
     # floats per quadrature point) instead of the state
     cached = args.lin_cache != "never"     # the headline mesh is Hex8
-    tile_bytes, lay = hvp_tile_bytes(op, cached_lin=10 if cached else 0)
+    geo = not args.no_geo_cache and mesh_is_affine(op)
+    tile_bytes, lay = hvp_tile_bytes(op, cached_lin=10 if cached else 0, geo=geo)
     hbm_rate = tile_bytes / (tile_us * 1e-6) / 1e9 if tile_us else None
-    prof = profile_counters(HEADLINE_PROFILE) if (n == 160 and not cached) else {}
+    # the profile's counters are those of the default kernel on this workload
+    prof = profile_counters(HEADLINE_PROFILE) if (n == 160 and not cached and geo) else {}
     valu = prof.get("valu_wave_instructions_per_launch")
     traffic = prof.get("hbm_bytes_per_launch")
     achieved = valu / (tile_us * 1e-6) / 1e9 if (valu and tile_us) else None
@@ -585,8 +594,9 @@ def headline(args):
         },
         "roofline": {
             "kernel": (f"k_tile_assemble<HEX8, NeoHookean<3,float>, float, "
-                       f"{'kModeHvpLin' if cached else 'kModeHvp'}, affine> (FP32 HVP tile kernel "
-                       "of the matrix-free PCG operator)"),
+                       f"{'kModeHvpLin' if cached else 'kModeHvp'}, "
+                       f"{'kGeomCache' if geo else 'kGeomAffine'}> (FP32 HVP tile kernel of the "
+                       "matrix-free PCG operator)"),
             "bound": "valu",
             "achieved": achieved,
             "peak": VALU_PEAK_GINST,
@@ -598,6 +608,7 @@ def headline(args):
             "algorithmic_bytes": tile_bytes,
             "hbm_achieved_gbs": hbm_rate,
             "hbm_frac": hbm_rate / HBM_PEAK_GBS if hbm_rate else None,
+            "geometry_cache": geo,
             "avg_launch_us": tile_us,
             "operator_us": op_us,
             "samples": op_n,
@@ -735,6 +746,10 @@ def main(argv=None):
                     help="matrix-free PCG: re-linearise the density in every HVP (never: the "
                          "default) or read it from a cache written once per solve (auto: on "
                          "Hex8 meshes; always)")
+    ap.add_argument("--no-geo-cache", action="store_true",
+                    help="matrix-free PCG on affine Hex8 meshes: compute each element's J^-1 and "
+                         "det J from the coordinates in every product instead of reading them "
+                         "from the cache written once per solve (the default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extras", action="store_true",
@@ -771,6 +786,7 @@ def main(argv=None):
 
     torch.cuda.set_device(0)
     solvers.set_linearization_cache(args.lin_cache)
+    solvers.set_geometry_cache(not args.no_geo_cache)
     if args.workload == "copy":
         # calibration of the HBM traffic counters: a plain 512 MiB device copy
         src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
@@ -801,6 +817,7 @@ def main(argv=None):
                         upto=args.large_steps,
                         modes=("mixed",) if args.no_fp64 else ("csr", "mixed"))
         r["lin_cache"] = args.lin_cache
+        r["geo_cache"] = not args.no_geo_cache
         print(json.dumps({"workload": args.workload, **r}), flush=True)
         return 0
     if args.workload == "c4":
@@ -871,8 +888,7 @@ def main(argv=None):
     achieved = spmv_bytes / spmv_s / 1e9 if op_samples else None
     spmv_traffic = None
     if blocked and args.n == 64:
-        t = SPMV_TRAFFIC_64
-        spmv_traffic = (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024
+        spmv_traffic = profile_counters(C2_PROFILE).get("hbm_bytes_per_launch")
     converged = all(s.converged for s in stats)
 
     extras = {}

@@ -89,8 +89,15 @@ int tatva_set_profiling(int enable);
  * density once per solve at its fixed state (Neo-Hookean: F^-1 and
  * mu - lambda ln J per quadrature point, FP32, in the workspace) so that every
  * HVP of the solve reads it instead of recomputing it (same arithmetic, same
- * result): 0 (default) never, 1 for Hex8 meshes, 2 always. */
-enum { TATVA_OPT_MATFREE_LIN_CACHE = 1 };
+ * result): 0 (default) never, 1 for Hex8 meshes, 2 always.
+ * TATVA_OPT_MATFREE_GEOM_CACHE: whether tatva_pcg_matfree_f32 on an affine
+ * Hex8 mesh (every element a parallelepiped) writes each element's Jacobian
+ * inverse (FP32) and det J (FP64) once per solve, 44 bytes per element in the
+ * workspace, so that every HVP reads them instead of the node coordinates (the
+ * values the product computes itself, so the same result bit for bit): 1
+ * (default) on affine Hex8 meshes, 0 never.  Both caches are in the workspace
+ * only when selected: set the options before querying its size. */
+enum { TATVA_OPT_MATFREE_LIN_CACHE = 1, TATVA_OPT_MATFREE_GEOM_CACHE = 2 };
 int tatva_set_option(int32_t option, int32_t value);
 
 /* ---------------------------------------------------------------- mesh ---- */
@@ -288,8 +295,10 @@ int tatva_bicgstab_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int3
 
 /* Jacobi-PCG with the matrix-free FP32 Hessian-vector product of (mesh, density)
  * at the FP32 state u (full vector); b, x, diag are reduced FP32 vectors.  The
- * workspace holds the CG vectors, a copy of u and, for a Neo-Hookean density,
- * the cache of its linearisation at u (TATVA_OPT_MATFREE_LIN_CACHE). */
+ * workspace holds the CG vectors, a copy of u and the caches the options
+ * select: for a Neo-Hookean density its linearisation at u
+ * (TATVA_OPT_MATFREE_LIN_CACHE), for an affine Hex8 mesh its element geometry
+ * (TATVA_OPT_MATFREE_GEOM_CACHE). */
 int64_t tatva_pcg_matfree_workspace(tatva_mesh *mesh, const tatva_density *density, int64_t n);
 int tatva_pcg_matfree_f32(tatva_mesh *mesh, const tatva_density *density, const float *u,
                           const int32_t *dof_map, int64_t n, const float *diag, const float *b,

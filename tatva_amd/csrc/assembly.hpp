@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "common.hpp"
 #include "density.hpp"
@@ -59,6 +60,20 @@ enum { kModeResidual = 0, kModeHvp = 1, kModeDiag = 2, kModeLinearize = 3, kMode
 // Cache layout: value k of quadrature point q of the element in tile slot t of
 // tile b at lin[((b * nq + q) * kLin + k) * kTile + t] (a tile's threads read
 // consecutive floats).
+
+// Element geometry variants of the tile kernels.  kGeomGeneral: computed at
+// every quadrature point from the node coordinates.  kGeomAffine: a Hex8
+// parallelepiped, its Jacobian inverted once per element.  kGeomCache: an
+// affine Hex8 mesh whose geometry is read from a cache written once per solve
+// (FP32 HVP / linearisation only).
+enum { kGeomGeneral = 0, kGeomAffine = 1, kGeomCache = 2 };
+
+// Geometry cache layout (affine Hex8): value k of the element in tile slot t
+// of tile b at geo[(b * kGeoFloats + k) * kTile + t] (a tile's threads read
+// consecutive floats): the FP32 cast of the FP64 Jacobian inverse (9 values,
+// row-major) and the FP64 det J as two 32-bit words, the values the kernels
+// computing the geometry themselves use.
+constexpr int kGeoFloats = 11;
 
 // Per-tile sum by the incidence lists instead of element colors.
 template <int KIND>
@@ -304,15 +319,51 @@ __device__ __forceinline__ void hex_dmodes(const S (&M)[8], S x, S y, S z, S yz,
   d[2] = M[4] + M[5] * x + M[6] * y + M[7] * xy;
 }
 
+// Affine Hex8: det J (FP64) and the FP32 cast of J^-1, from the coordinate
+// modes (J rows = modes 1, 2, 4).  Plain multiplies and adds, no fused
+// multiply-add contraction, so the geometry cache kernel and the tile kernels
+// get the same bits.
+template <class T>
+__device__ __forceinline__ double hex8_affine_inverse(const double (&X)[8][3], T (&iv)[3][3]) {
+#pragma clang fp contract(off)
+  double XM[3][8];
+  hex_modes<double, 3>(X, XM);
+  double J[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    J[0][j] = XM[j][1];
+    J[1][j] = XM[j][2];
+    J[2][j] = XM[j][4];
+  }
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double r = 1.0 / det;
+  iv[0][0] = T(c00 * r);
+  iv[1][0] = T(c01 * r);
+  iv[2][0] = T(c02 * r);
+  iv[0][1] = T((J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r);
+  iv[1][1] = T((J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r);
+  iv[2][1] = T((J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r);
+  iv[0][2] = T((J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r);
+  iv[1][2] = T((J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r);
+  iv[2][2] = T((J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r);
+  return det;
+}
+
 // Hex8 residual (MODE = kModeResidual: S = P(G), plus the Poisson source) or
 // Hessian-vector product (S = dP(G(u); G(v))): ye = sum_q w_q B_q^T S_q.
 // Geometry in FP64, physics in T.
 // MODE kModeLinearize / kModeHvpLin: as elem_hvp (cache at lin, stride kTile).
-template <class Dens, class T, int MODE, bool AFFINE>
+// GEO (affine only): J^-1 and det J read from geo (this element's geometry
+// cache, stride kTile) instead of computed from X.
+template <class Dens, class T, int MODE, bool AFFINE, bool GEO = false>
 __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&ue)[8][Dens::NC],
                                              const T (&ve)[8][Dens::NC], const QuadRule &quad,
                                              const Dens &dens, T (&ye)[8][Dens::NC],
-                                             T *lin = nullptr) {
+                                             T *lin = nullptr, const T *geo = nullptr) {
+  static_assert(!GEO || AFFINE, "the Hex8 geometry cache holds one inverse per element");
   constexpr int NC = Dens::NC;
   constexpr bool kUsesU = MODE != kModeHvpLin;
   constexpr bool kUsesV = MODE == kModeHvp || MODE == kModeHvpLin;
@@ -320,8 +371,8 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
   // is computed (a point's values are needed only after its tangent gradient)
   typename Dens::Lin Lnext{};
   if constexpr (MODE == kModeHvpLin) Lnext = dens.load_lin(lin, kTile);
-  double XM[3][8];
-  hex_modes<double, 3>(X, XM);
+  double XM[3][8];  // coordinate modes (general elements)
+  if constexpr (!AFFINE) hex_modes<double, 3>(X, XM);
   T UM[NC][8], VM[NC][8];
   if constexpr (kUsesU) hex_modes<T, NC>(ue, UM);
   if constexpr (kUsesV) hex_modes<T, NC>(ve, VM);
@@ -334,19 +385,16 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
   // values each point would cast from the FP64 inverse)
   T iv0[3][3] = {};
   double det0 = 0.0;
-  if constexpr (AFFINE) {
-    double J[3][3], inv0[3][3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      J[0][j] = XM[j][1];
-      J[1][j] = XM[j][2];
-      J[2][j] = XM[j][4];
-    }
-    det0 = det_inv(J, inv0);
+  if constexpr (GEO) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) iv0[i][j] = T(inv0[i][j]);
+      for (int j = 0; j < 3; ++j) iv0[i][j] = geo[(i * 3 + j) * kTile];
+    const uint32_t lo = __float_as_uint(float(geo[9 * kTile]));
+    const uint32_t hi = __float_as_uint(float(geo[10 * kTile]));
+    det0 = __hiloint2double((int)hi, (int)lo);
+  } else if constexpr (AFFINE) {
+    det0 = hex8_affine_inverse<T>(X, iv0);
   }
   for (int q = 0; q < quad.nq; ++q) {
     const double x = quad.xi[q][0], y = quad.xi[q][1], z = quad.xi[q][2];
@@ -409,21 +457,26 @@ __device__ __forceinline__ void hex8_element(const double (&X)[8][3], const T (&
 #pragma unroll
         for (int j = 0; j < 3; ++j) dG[c][j] = h[0] * iv[j][0] + h[1] * iv[j][1] + h[2] * iv[j][2];
       }
+      // S = w dP(G; dG): the weight folded into the density's moduli
       if constexpr (MODE == kModeHvpLin) {
         const typename Dens::Lin Lq = Lnext;
         if (q + 1 < quad.nq) Lnext = dens.load_lin(lq + Dens::kLin * kTile, kTile);
-        dens.dP(Lq, dG, S);
+        dens.dPw(Lq, dG, w, S);
       } else {
         const auto L = dens.linearize(G);
-        dens.dP(L, dG, S);
+        dens.dPw(L, dG, w, S);
       }
     }
-    // Q[c][k] = w sum_j S[c][j] inv[j][k], accumulated into the derivative modes
+    // Q[c][k] = w sum_j P[c][j] inv[j][k] (residual) / sum_j S[c][j] inv[j][k],
+    // accumulated into the derivative modes
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       T Q[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) Q[k] = w * (S[c][0] * iv[0][k] + S[c][1] * iv[1][k] + S[c][2] * iv[2][k]);
+      for (int k = 0; k < 3; ++k) {
+        const T sk = S[c][0] * iv[0][k] + S[c][1] * iv[1][k] + S[c][2] * iv[2][k];
+        Q[k] = MODE == kModeResidual ? w * sk : sk;
+      }
       A[c][1] += Q[0];
       A[c][2] += Q[1];
       A[c][4] += Q[2];
@@ -490,56 +543,105 @@ constexpr size_t tile_union_bytes(int max_local, int D, int nne, int NC, size_t 
   const size_t elem = (size_t)kTile * nne * NC * tsize;
   return align16(elem > staged ? elem : staged);
 }
+// the color sum's copy of the tile's color offsets, after the staged nodes
+constexpr size_t tile_offs_bytes() { return align16(sizeof(int16_t) * (kMaxTileColors + 1)); }
 constexpr size_t tile_lds_bytes(int kind, int max_local, int D, int nne, int NC, size_t tsize,
                                 int mode) {
-  if (kind != TATVA_TET4) return tile_stage_bytes(max_local, D, NC, tsize, mode);
+  if (kind != TATVA_TET4) return align16(tile_stage_bytes(max_local, D, NC, tsize, mode)) + tile_offs_bytes();
   return tile_union_bytes(max_local, D, nne, NC, tsize, mode) + (size_t)kTile * nne * sizeof(uint16_t) +
          align16((size_t)max_local * sizeof(uint16_t));
 }
 constexpr size_t kTileLdsMax = 65536;  // LDS one workgroup may allocate
 
-// Body of one tile (one workgroup of kTile threads; lds of tile_lds_bytes).  u: full state; v (HVP): tangent read through dof_map
+// Body of one tile (one workgroup of kTile threads; lds of tile_lds_bytes).
+// GEOM (kGeomGeneral / kGeomAffine / kGeomCache): how the elements' geometry
+// is taken; with kGeomCache the FP32 HVP and linearisation read it from geo
+// instead of the node coordinates.  u: full state; v (HVP): tangent read through dof_map
 // (reduced, zero on constrained dofs).  The tile's local nodes are loaded into
 // LDS once (one thread per node), so each element reads its nodes from LDS
 // instead of gathering them from global memory.  Writes the sums of local node
 // j to partial[slot_pos[node_ptr[tile] + j] * NC + c] (node-major order).
-template <int KIND, class Dens, class T, int MODE, bool AFFINE>
+template <int KIND, class Dens, class T, int MODE, int GEOM>
 __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict__ coords,
                                               const QuadRule &quad, const Dens &dens,
                                               const T *__restrict__ u, const T *__restrict__ v,
                                               const int32_t *__restrict__ dof_map,
                                               const TileSet &ts, T *__restrict__ partial,
                                               unsigned char *__restrict__ lds,
-                                              T *__restrict__ lin = nullptr) {
+                                              T *__restrict__ lin = nullptr,
+                                              const T *__restrict__ geo = nullptr) {
   constexpr bool kUsesU = MODE != kModeHvpLin;
   constexpr bool kUsesV = MODE == kModeHvp || MODE == kModeHvpLin;
   using El = Elem<KIND>;
   constexpr int D = El::D, NNE = El::NNE, NC = Dens::NC;
+  // a geometry cache (affine Hex8 HVP / linearisation) replaces the coordinates
+  constexpr bool kAffine = KIND == TATVA_HEX8 && GEOM != kGeomGeneral;
+  constexpr bool kGeo = KIND == TATVA_HEX8 && GEOM == kGeomCache && sizeof(T) == 4 &&
+                        (MODE == kModeHvp || MODE == kModeHvpLin || MODE == kModeLinearize);
+  constexpr bool uses_x = !kGeo;
   const int32_t nb = ts.node_ptr[tile];
   const int nloc = ts.node_ptr[tile + 1] - nb;
   double *Xs = reinterpret_cast<double *>(lds);
   T *acc = reinterpret_cast<T *>(Xs + (size_t)ts.max_local * D);
   T *us = acc + (size_t)ts.max_local * NC;
   T *vs = us + (size_t)ts.max_local * NC;
-  for (int j = threadIdx.x; j < nloc; j += kTile) {
-    const int64_t g = ts.tnodes[nb + j];
+  // Stage the local nodes, two per thread at a time (the second one is the
+  // first again past the end: the same values stored twice).  Every load of
+  // the pair is issued before the first wait: the global node ids, then the
+  // coordinates, state and dof map entries, then the tangent through the map
+  // (a constrained dof, r < 0, reads a valid dummy address and stores 0, so
+  // that no load sits behind a branch).
+  const T *dummy = reinterpret_cast<const T *>(coords);
+  const auto stage = [&](auto mapped) {
+    constexpr bool kMapped = decltype(mapped)::value;
+    for (int j0 = threadIdx.x; j0 < nloc; j0 += 2 * kTile) {
+      const int jj[2] = {j0, j0 + kTile < nloc ? j0 + kTile : j0};
+      int64_t g[2];
 #pragma unroll
-    for (int k = 0; k < D; ++k) Xs[j * D + k] = coords[g * D + k];
+      for (int h = 0; h < 2; ++h) g[h] = ts.tnodes[nb + jj[h]];
+      double xs[2][D];
+      T uv[2][NC], vv[2][NC];
+      int64_t r[2][NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int64_t dof = g * NC + c;
-      acc[j * NC + c] = T(0);
-      if constexpr (kUsesU) us[j * NC + c] = u[dof];
+      for (int h = 0; h < 2; ++h) {
+        if constexpr (uses_x) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) xs[h][k] = coords[g[h] * D + k];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const int64_t dof = g[h] * NC + c;
+          if constexpr (kUsesU) uv[h][c] = u[dof];
+          if constexpr (kUsesV) r[h][c] = kMapped ? (int64_t)dof_map[dof] : dof;
+        }
+      }
       if constexpr (kUsesV) {
-        if (dof_map) {
-          const int32_t r = dof_map[dof];
-          vs[j * NC + c] = r >= 0 ? v[r] : T(0);
-        } else {
-          vs[j * NC + c] = v[dof];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const T val = *(r[h][c] >= 0 ? v + r[h][c] : dummy);
+            vv[h][c] = r[h][c] >= 0 ? val : T(0);
+          }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = jj[h];
+        if constexpr (uses_x) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) Xs[j * D + k] = xs[h][k];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          acc[j * NC + c] = T(0);
+          if constexpr (kUsesU) us[j * NC + c] = uv[h][c];
+          if constexpr (kUsesV) vs[j * NC + c] = vv[h][c];
         }
       }
     }
-  }
+  };
+  if (kUsesV && dof_map) stage(std::true_type{});
+  else stage(std::false_type{});
   // Tet4: the tile's incidence lists are loaded here and stored to LDS after
   // the element work, so their latency overlaps it
   uint64_t inc_pk = 0;
@@ -553,10 +655,27 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
   const int16_t *off = ts.color_off + (int64_t)tile * (ts.max_colors + 1);
   // the tile's elements fill its first off[max_colors] slots (the rest is padding)
   const bool real = (int)threadIdx.x < (int)off[ts.max_colors];
-  const uint16_t *ln = ts.lnodes + slot * NNE;
+  // the color sum reads the offsets from LDS (copied with the nodes)
+  int16_t *offs = reinterpret_cast<int16_t *>(
+      lds + align16(tile_stage_bytes(ts.max_local, D, NC, sizeof(T), MODE)));
+  if constexpr (!tile_gather<KIND>() && MODE != kModeLinearize) {
+    for (int c = threadIdx.x; c <= ts.max_colors; c += kTile) offs[c] = off[c];
+  }
+  // the element's local node numbers (padding slots hold valid entries too):
+  // one 16- / 8-byte load for Hex8 / Tet4
   int l[NNE];
+  if constexpr (NNE == 8 || NNE == 4) {
+    using Vec = std::conditional_t<NNE == 8, uint4, uint2>;
+    const Vec pk = reinterpret_cast<const Vec *>(ts.lnodes)[slot];
+    const uint32_t w[4] = {pk.x, pk.y, NNE == 8 ? ((const uint32_t *)&pk)[2] : 0u,
+                           NNE == 8 ? ((const uint32_t *)&pk)[3] : 0u};
 #pragma unroll
-  for (int a = 0; a < NNE; ++a) l[a] = real ? (int)ln[a] : 0;
+    for (int a = 0; a < NNE; ++a) l[a] = real ? (int)((w[a >> 1] >> (16 * (a & 1))) & 0xffffu) : 0;
+  } else {
+    const uint16_t *ln = ts.lnodes + slot * NNE;
+#pragma unroll
+    for (int a = 0; a < NNE; ++a) l[a] = real ? (int)ln[a] : 0;
+  }
   T ye[NNE][NC];
 #pragma unroll
   for (int a = 0; a < NNE; ++a)
@@ -571,7 +690,7 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
 #pragma unroll
     for (int a = 0; a < NNE; ++a) {
 #pragma unroll
-      for (int k = 0; k < D; ++k) X[a][k] = Xs[l[a] * D + k];
+      for (int k = 0; k < D; ++k) X[a][k] = uses_x ? Xs[l[a] * D + k] : 0.0;
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         ue[a][c] = kUsesU ? us[l[a] * NC + c] : T(0);
@@ -580,12 +699,14 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     }
     // this thread's cached linearisation (kModeLinearize / kModeHvpLin)
     T *lp = lin ? lin + (int64_t)tile * quad.nq * Dens::kLin * kTile + threadIdx.x : nullptr;
+    const T *gp = kGeo ? geo + (int64_t)tile * kGeoFloats * kTile + threadIdx.x : nullptr;
     if constexpr (KIND == TATVA_HEX8 && MODE == kModeResidual) {
-      hex8_element<Dens, T, kModeResidual, AFFINE>(X, ue, ue, quad, de, ye);
+      hex8_element<Dens, T, kModeResidual, kAffine>(X, ue, ue, quad, de, ye);
     } else if constexpr (MODE == kModeResidual) {
       elem_residual<KIND, Dens>(X, ue, quad, de, ye);
     } else if constexpr (MODE == kModeHvp || MODE == kModeHvpLin || MODE == kModeLinearize) {
-      if constexpr (KIND == TATVA_HEX8) hex8_element<Dens, T, MODE, AFFINE>(X, ue, ve, quad, de, ye, lp);
+      if constexpr (KIND == TATVA_HEX8)
+        hex8_element<Dens, T, MODE, kAffine, kGeo>(X, ue, ve, quad, de, ye, lp, gp);
       else elem_hvp<KIND, Dens, T, MODE>(X, ue, ve, quad, de, ye, lp);
     } else {
       elem_diag<KIND, Dens, T>(X, ue, quad, de, ye);
@@ -617,7 +738,7 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     return;
   }
   for (int c = 0; c < ts.max_colors; ++c) {
-    const int lo = off[c], hi = off[c + 1];
+    const int lo = offs[c], hi = offs[c + 1];
     if (lo >= hi) break;  // colors are numbered 0..ncolors-1 without gaps
     if ((int)threadIdx.x >= lo && (int)threadIdx.x < hi) {
 #pragma unroll
@@ -635,7 +756,7 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
   }
 }
 
-template <int KIND, class Dens, class T, int MODE, bool AFFINE>
+template <int KIND, class Dens, class T, int MODE, int GEOM>
 __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restrict__ coords,
                                                          QuadRule quad, Dens dens,
                                                          const T *__restrict__ u,
@@ -643,7 +764,8 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
                                                          const int32_t *__restrict__ dof_map,
                                                          TileSet ts, T *__restrict__ partial,
                                                          const int *__restrict__ skip, int rev,
-                                                         T *__restrict__ lin) {
+                                                         T *__restrict__ lin,
+                                                         const T *__restrict__ geo) {
   if (skip && *skip) return;
   extern __shared__ unsigned char lds_raw[];
   // rev: tiles from the end (a solver alternates the order between iterations
@@ -651,8 +773,8 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
   // previous one; every tile writes its own slots, so the result is the same)
   int tile = (int)xcd_block(blockIdx.x, gridDim.x);
   if (rev) tile = (int)gridDim.x - 1 - tile;
-  tile_assemble<KIND, Dens, T, MODE, AFFINE>(tile, coords, quad, dens, u, v, dof_map, ts, partial,
-                                             lds_raw, lin);
+  tile_assemble<KIND, Dens, T, MODE, GEOM>(tile, coords, quad, dens, u, v, dof_map, ts, partial,
+                                           lds_raw, lin, geo);
 }
 
 // Sum of node i's tile slots (ascending tile): the tiles wrote them to the

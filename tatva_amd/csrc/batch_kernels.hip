@@ -75,8 +75,10 @@ __global__ __launch_bounds__(kTile) void k_batch_assemble(const BatchSlot *__res
   if (S.st->done) return;
   extern __shared__ unsigned char lds_raw[];
   const Dens dens = make_density<Dens>(S.dens);
-  tile_assemble<KIND, Dens, float, kModeHvp, AFFINE>(b - S.a_blk0, S.coords, quad, dens, S.u, S.z,
-                                                     S.dof_map, S.tiles, S.partial, lds_raw);
+  // AFFINE: a Hex8 parallelepiped mesh (no geometry cache here)
+  tile_assemble<KIND, Dens, float, kModeHvp,
+                (AFFINE && KIND == TATVA_HEX8) ? kGeomAffine : kGeomGeneral>(
+      b - S.a_blk0, S.coords, quad, dens, S.u, S.z, S.dof_map, S.tiles, S.partial, lds_raw);
 }
 
 template <int NC>

@@ -48,6 +48,11 @@ struct Poisson {
 #pragma unroll
     for (int j = 0; j < D; ++j) out[0][j] = dG[0][j];
   }
+  // w dP (the quadrature weight folded into the tangent)
+  __device__ __forceinline__ void dPw(const Lin &, const T dG[NC][D], T w, T out[NC][D]) const {
+#pragma unroll
+    for (int j = 0; j < D; ++j) out[0][j] = w * dG[0][j];
+  }
 };
 
 template <int D, class T>
@@ -69,13 +74,17 @@ struct LinearElastic {
   __device__ __forceinline__ void store_lin(const Lin &, T *, int) const {}
   __device__ __forceinline__ Lin load_lin(const T *, int) const { return Lin{}; }
   __device__ __forceinline__ void stress(const T G[NC][D], T s[NC][D]) const {
+    stress_with(mu, lam, G, s);
+  }
+  __device__ __forceinline__ static void stress_with(T m, T l, const T G[NC][D], T s[NC][D]) {
     T tr = T(0);
 #pragma unroll
     for (int i = 0; i < D; ++i) tr += G[i][i];
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
-      for (int j = 0; j < D; ++j) s[i][j] = mu * (G[i][j] + G[j][i]) + (i == j ? lam * tr : T(0));
+      for (int j = 0; j < D; ++j)
+        s[i][j] = i == j ? m * (G[i][j] + G[j][i]) + l * tr : m * (G[i][j] + G[j][i]);
   }
   __device__ __forceinline__ T psi(const T G[NC][D], T) const {
     T s[NC][D];
@@ -91,6 +100,10 @@ struct LinearElastic {
   __device__ __forceinline__ Lin linearize(const T (&)[NC][D]) const { return Lin{}; }
   __device__ __forceinline__ void dP(const Lin &, const T dG[NC][D], T out[NC][D]) const {
     stress(dG, out);
+  }
+  // w dP (the quadrature weight folded into the moduli)
+  __device__ __forceinline__ void dPw(const Lin &, const T dG[NC][D], T w, T out[NC][D]) const {
+    stress_with(w * mu, w * lam, dG, out);
   }
 };
 
@@ -129,7 +142,7 @@ struct NeoHookean {
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
-      for (int j = 0; j < D; ++j) F[i][j] = G[i][j] + (i == j ? T(1) : T(0));
+      for (int j = 0; j < D; ++j) F[i][j] = i == j ? G[i][j] + T(1) : G[i][j];
     Lin L;
     const T J = det_inv(F, L.Fi);
     L.lnJ = tlog<T>(J);
@@ -163,7 +176,7 @@ struct NeoHookean {
     for (int i = 0; i < D; ++i)
 #pragma unroll
       for (int j = 0; j < D; ++j) {
-        F[i][j] = G[i][j] + (i == j ? T(1) : T(0));
+        F[i][j] = i == j ? G[i][j] + T(1) : G[i][j];
         I1 += F[i][j] * F[i][j];
       }
     const T lnJ = tlog<T>(det_only(F));
@@ -181,10 +194,19 @@ struct NeoHookean {
     for (int i = 0; i < D; ++i)
 #pragma unroll
       for (int j = 0; j < D; ++j)
-        out[i][j] = mu * (G[i][j] + (i == j ? T(1) : T(0))) + b * L.Fi[j][i];
+        out[i][j] = mu * (i == j ? G[i][j] + T(1) : G[i][j]) + b * L.Fi[j][i];
   }
   __device__ __forceinline__ void dP(const Lin &L, const T dG[NC][D], T out[NC][D]) const {
-    // W = F^-1 dG, Z = W F^-1; (F^-T dG^T F^-T)_ij = Z_ji; tr(F^-1 dG) = tr W.
+    dP_with(mu, L.c1, lam, L, dG, out);
+  }
+  // w dP (the quadrature weight folded into mu, mu - lambda ln J and lambda)
+  __device__ __forceinline__ void dPw(const Lin &L, const T dG[NC][D], T w, T out[NC][D]) const {
+    dP_with(w * mu, w * L.c1, w * lam, L, dG, out);
+  }
+  // m dG + c Z^T + l tr(W) F^-T with W = F^-1 dG, Z = W F^-1
+  // ((F^-T dG^T F^-T)_ij = Z_ji, tr(F^-1 dG) = tr W)
+  __device__ __forceinline__ static void dP_with(T m, T c, T l, const Lin &L, const T dG[NC][D],
+                                                 T out[NC][D]) {
     T W[D][D];
 #pragma unroll
     for (int j = 0; j < D; ++j)
@@ -198,7 +220,7 @@ struct NeoHookean {
     T tr = T(0);
 #pragma unroll
     for (int j = 0; j < D; ++j) tr += W[j][j];
-    const T lt = lam * tr;
+    const T lt = l * tr;
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -206,7 +228,7 @@ struct NeoHookean {
         T z = T(0);
 #pragma unroll
         for (int k = 0; k < D; ++k) z += W[j][k] * L.Fi[k][i];
-        out[i][j] = mu * dG[i][j] + L.c1 * z + lt * L.Fi[j][i];
+        out[i][j] = m * dG[i][j] + c * z + lt * L.Fi[j][i];
       }
   }
 };

@@ -730,7 +730,7 @@ bool sort_keys(std::vector<MortonKey> &v, int threads) {
 // the thread count).
 int build_tiles(const double *coords, int dim, int64_t N, const int32_t *elements, int64_t E,
                 int nne, bool incidence, HostTiles &T) {
-  constexpr int kMaxColors = 128;
+  constexpr int kMaxColors = kMaxTileColors;  // the LDS copy of a tile's offsets holds this many
   const int64_t n_tiles = (E + kTile - 1) / kTile;
   if (n_tiles >= (int64_t(1) << 31)) {
     set_error("too many tiles");
@@ -1243,7 +1243,8 @@ template <class T, int MODE>
 static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, const T *v,
                            const int32_t *dof_map, const double *f_ext, T *out, hipStream_t s,
                            const int *skip, double *part_dot = nullptr, int gather_blocks = 0,
-                           int rev = 0, hipEvent_t between = nullptr, T *lin = nullptr) {
+                           int rev = 0, hipEvent_t between = nullptr, T *lin = nullptr,
+                           const T *geo = nullptr) {
   T *partial = (T *)m->scratch;
   return with_kind(m->kind, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
@@ -1259,12 +1260,25 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
         set_error("mesh tiles need more LDS than a workgroup holds");
         return TATVA_EINVAL;
       }
-      if (K == TATVA_HEX8 && m->hex_affine)
-        k_tile_assemble<K, Dens, T, MODE, true><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin);
-      else
-        k_tile_assemble<K, Dens, T, MODE, false><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin);
+      // element geometry: from a geometry cache (FP32 HVP / linearisation), once
+      // per element (affine Hex8), or at every point
+      constexpr bool kCacheable = sizeof(T) == 4 && (MODE == kModeHvp || MODE == kModeHvpLin ||
+                                                     MODE == kModeLinearize);
+      if (geo && !(kCacheable && K == TATVA_HEX8 && m->hex_affine)) {
+        set_error("a geometry cache serves the FP32 HVP of affine Hex8 meshes only");
+        return TATVA_EINVAL;
+      }
+      if (geo) {
+        if constexpr (kCacheable)
+          k_tile_assemble<K, Dens, T, MODE, kGeomCache><<<m->tiles.n_tiles, kTile, lds, s>>>(
+              m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, geo);
+      } else if (K == TATVA_HEX8 && m->hex_affine) {
+        k_tile_assemble<K, Dens, T, MODE, kGeomAffine><<<m->tiles.n_tiles, kTile, lds, s>>>(
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, nullptr);
+      } else {
+        k_tile_assemble<K, Dens, T, MODE, kGeomGeneral><<<m->tiles.n_tiles, kTile, lds, s>>>(
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, nullptr);
+      }
       if constexpr (MODE == kModeLinearize) {
         TATVA_CHECK_LAUNCH();
         return TATVA_OK;  // the cache is the output: no gather
@@ -1291,11 +1305,12 @@ int launch_hvp_f64(tatva_mesh *m, const tatva_density *d, const double *u, const
 
 int launch_hvp_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                    const int32_t *dof_map, float *y, hipStream_t s, const int *skip,
-                   const float *lin) {
+                   const float *lin, const float *geo) {
   if (lin)
     return launch_assembly<float, kModeHvpLin>(m, d, u, v, dof_map, nullptr, y, s, skip, nullptr,
-                                               0, 0, nullptr, const_cast<float *>(lin));
-  return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip);
+                                               0, 0, nullptr, const_cast<float *>(lin), geo);
+  return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip, nullptr, 0, 0,
+                                          nullptr, nullptr, geo);
 }
 
 int64_t lin_cache_floats(const tatva_mesh *m, const tatva_density *d) {
@@ -1304,9 +1319,56 @@ int64_t lin_cache_floats(const tatva_mesh *m, const tatva_density *d) {
 }
 
 int launch_linearize_f32(tatva_mesh *m, const tatva_density *d, const float *u, float *lin,
-                         hipStream_t s) {
+                         hipStream_t s, const float *geo) {
   return launch_assembly<float, kModeLinearize>(m, d, u, nullptr, nullptr, nullptr, nullptr, s,
-                                                nullptr, nullptr, 0, 0, nullptr, lin);
+                                                nullptr, nullptr, 0, 0, nullptr, lin, geo);
+}
+
+// Writes the geometry cache of an affine Hex8 mesh (layout at kGeoFloats):
+// one thread per tile slot, its element's nodes gathered from conn.
+__global__ __launch_bounds__(kTile) void k_geometry_cache_hex8(const double *__restrict__ coords,
+                                                               const int32_t *__restrict__ conn,
+                                                               TileSet ts,
+                                                               float *__restrict__ geo) {
+  const int tile = blockIdx.x;
+  const int64_t slot = (int64_t)tile * kTile + threadIdx.x;
+  const int32_t e = ts.elem[slot];
+  float *gp = geo + (int64_t)tile * kGeoFloats * kTile + threadIdx.x;
+  if (e < 0) {  // padding slot: zeros, never read
+#pragma unroll
+    for (int k = 0; k < kGeoFloats; ++k) gp[k * kTile] = 0.0f;
+    return;
+  }
+  double X[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int64_t g = conn[(int64_t)e * 8 + a];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) X[a][k] = coords[g * 3 + k];
+  }
+  float iv[3][3];
+  const double det = hex8_affine_inverse<float>(X, iv);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gp[(i * 3 + j) * kTile] = iv[i][j];
+  gp[9 * kTile] = __uint_as_float((uint32_t)__double2loint(det));
+  gp[10 * kTile] = __uint_as_float((uint32_t)__double2hiint(det));
+}
+
+int64_t geo_cache_floats(const tatva_mesh *m) {
+  if (!m || m->kind != TATVA_HEX8 || !m->hex_affine) return 0;
+  return (int64_t)m->tiles.n_tiles * kTile * kGeoFloats;
+}
+
+int launch_geometry_cache_f32(tatva_mesh *m, float *geo, hipStream_t s) {
+  if (geo_cache_floats(m) == 0) {
+    set_error("a geometry cache needs an affine Hex8 mesh");
+    return TATVA_EINVAL;
+  }
+  k_geometry_cache_hex8<<<m->tiles.n_tiles, kTile, 0, s>>>(m->coords, m->conn, m->tiles, geo);
+  TATVA_CHECK_LAUNCH();
+  return TATVA_OK;
 }
 
 int hvp_gather_blocks(const tatva_mesh *m) {
@@ -1316,13 +1378,14 @@ int hvp_gather_blocks(const tatva_mesh *m) {
 
 int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                        const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
-                       const int *skip, int rev, hipEvent_t between, const float *lin) {
+                       const int *skip, int rev, hipEvent_t between, const float *lin,
+                       const float *geo) {
   if (lin)
     return launch_assembly<float, kModeHvpLin>(m, d, u, v, dof_map, nullptr, y, s, skip, part_dot,
                                                hvp_gather_blocks(m), rev, between,
-                                               const_cast<float *>(lin));
+                                               const_cast<float *>(lin), geo);
   return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip, part_dot,
-                                          hvp_gather_blocks(m), rev, between);
+                                          hvp_gather_blocks(m), rev, between, nullptr, geo);
 }
 
 }  // namespace tatva

@@ -57,14 +57,20 @@ int launch_hvp_f64(tatva_mesh *m, const tatva_density *d, const double *u, const
                    const int32_t *dof_map, double *y, hipStream_t s, const int *skip = nullptr);
 // With lin (a cache written by launch_linearize_f32 at the same u) the tile
 // kernel reads the density's linearisation instead of recomputing it.
+// With geo (an affine Hex8 mesh's geometry cache, written by
+// launch_geometry_cache_f32) J^-1 and det J are read instead of computed.
 int launch_hvp_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                    const int32_t *dof_map, float *y, hipStream_t s, const int *skip = nullptr,
-                   const float *lin = nullptr);
+                   const float *lin = nullptr, const float *geo = nullptr);
 // Floats of the linearisation cache of (mesh, density) at FP32 (0: the density
 // has none to cache), and the kernel writing it from the FP32 state u.
 int64_t lin_cache_floats(const tatva_mesh *m, const tatva_density *d);
 int launch_linearize_f32(tatva_mesh *m, const tatva_density *d, const float *u, float *lin,
-                         hipStream_t s);
+                         hipStream_t s, const float *geo = nullptr);
+// Floats of an affine Hex8 mesh's geometry cache (0 for other meshes) and its
+// writer.
+int64_t geo_cache_floats(const tatva_mesh *m);
+int launch_geometry_cache_f32(tatva_mesh *m, float *geo, hipStream_t s);
 // y = H(u) v (FP32) with the block shares of v.y in part_dot, over
 // hvp_gather_blocks(m) blocks (v, y reduced through dof_map); rev: tiles in
 // reverse order (same result); between: an event recorded after the tile
@@ -72,7 +78,7 @@ int launch_linearize_f32(tatva_mesh *m, const tatva_density *d, const float *u, 
 int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                        const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
                        const int *skip = nullptr, int rev = 0, hipEvent_t between = nullptr,
-                       const float *lin = nullptr);
+                       const float *lin = nullptr, const float *geo = nullptr);
 int hvp_gather_blocks(const tatva_mesh *m);
 int density_ncomp(const tatva_mesh *m, const tatva_density *d);
 // Drops cached solver graphs that captured this mesh's arrays.

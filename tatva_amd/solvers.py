@@ -175,6 +175,7 @@ def set_profiling(enable: bool) -> None:
 
 
 OPT_MATFREE_LIN_CACHE = 1
+OPT_MATFREE_GEOM_CACHE = 2   # include/tatva_amd.h TATVA_OPT_MATFREE_GEOM_CACHE
 
 
 _LIN_CACHE_MODES = {"never": 0, "auto": 1, "always": 2}
@@ -188,6 +189,16 @@ def set_linearization_cache(mode: str) -> None:
     if mode not in _LIN_CACHE_MODES:
         raise ValueError(f"mode must be one of {sorted(_LIN_CACHE_MODES)}")
     nat.check(nat.lib().tatva_set_option(OPT_MATFREE_LIN_CACHE, _LIN_CACHE_MODES[mode]))
+
+
+def set_geometry_cache(on: bool) -> None:
+    """Matrix-free FP32 PCG on affine Hex8 meshes (every element a
+    parallelepiped): write each element's Jacobian inverse (FP32) and det J
+    (FP64) once per solve and read them in every Hessian-vector product instead
+    of the node coordinates (on, the default), or compute them from the
+    coordinates in every product (off).  The same result bit for bit; 44 bytes
+    per element of workspace."""
+    nat.check(nat.lib().tatva_set_option(OPT_MATFREE_GEOM_CACHE, 1 if on else 0))
 
 
 def dot(x, y):

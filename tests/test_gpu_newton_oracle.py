@@ -208,3 +208,46 @@ def test_colored_hessian_of_the_stretch_case_matches_oracle():
     Ko = fem.hessian_csr(prob, Lo, u, So, colors)
     rel = np.max(np.abs(to_np(K.data) - Ko.data)) / np.max(np.abs(Ko.data))
     assert rel <= 1e-12, rel
+
+
+@pytest.mark.parametrize("lin", ["never", "always"])
+def test_geometry_cache_gives_the_same_solve(lin):
+    """With the affine Hex8 geometry cache (each element's J^-1 in FP32 and det J
+    in FP64, written once per solve: tatva_set_option(TATVA_OPT_MATFREE_GEOM_CACHE,
+    1)) the matrix-free FP32 PCG reads the values its products would compute from
+    the coordinates: the solve is the same bit for bit (same iterations, same x),
+    also with the linearisation written through the cache, and so is a mixed
+    Newton load step of the stretch problem.  (The cache is on by default.)"""
+    import torch
+
+    C = stretch_case("hex8", 6, 0.0)
+    E, L, mesh = C["E"], C["L"].at["load"].set(C["values"](0.05)), C["mesh"]
+    nd = mesh.n_nodes * 3
+    rng = np.random.default_rng(11)
+    # a state with a positive definite Hessian: the stretch predictor and a small
+    # perturbation
+    u_np = 0.002 * rng.standard_normal(nd)
+    u_np[0::3] += 0.05 * mesh.coords[:, 0]
+    u = torch.as_tensor(u_np, dtype=torch.float32, device="cuda")
+    red = E.reduced(L)
+    b = torch.as_tensor(rng.standard_normal(red.size), dtype=torch.float32, device="cuda")
+    runs, states = [], []
+    solvers.set_linearization_cache(lin)
+    try:
+        for geo in (False, True):
+            solvers.set_geometry_cache(geo)
+            x, info = solvers.pcg_matfree(red, u, b, tol=1e-6, chunk=16)
+            assert info.converged, (geo, info)
+            runs.append((info.iterations, x.cpu().numpy()))
+            uu = torch.zeros(nd, dtype=torch.float64, device="cuda")
+            uu[0::3] = torch.as_tensor(0.05 * mesh.coords[:, 0], device="cuda")
+            st = solvers.Newton(E, L, mode="mixed", tol=1e-8, inner_tol=1e-3, chunk=16).solve(uu)
+            assert st.converged
+            states.append((st.cg_iterations, uu.cpu().numpy()))
+    finally:
+        solvers.set_geometry_cache(True)   # the default
+        solvers.set_linearization_cache("never")
+    assert runs[0][0] == runs[1][0]
+    np.testing.assert_array_equal(runs[1][1], runs[0][1])
+    assert states[0][0] == states[1][0]
+    np.testing.assert_array_equal(states[1][1], states[0][1])
