@@ -428,10 +428,33 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict
     p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
-// Elements per thread whose loads the CG step issues before its scalar sums.
+// Vectors of the CG step: FP32 in 16-byte vectors of 4 (a clang vector type,
+// which the non-temporal store builtin takes: 115 -> 94 us per step at n =
+// 12.4 M), FP64 one element at a time (in 16-byte vectors of 2 it measured
+// slower, 162 -> 168 us).
+typedef float step_f4 __attribute__((ext_vector_type(4)));
+template <class T>
+struct StepVec;
+template <>
+struct StepVec<float> {
+  using type = step_f4;
+  static constexpr int V = 4;
+};
+template <>
+struct StepVec<double> {
+  using type = double;
+  static constexpr int V = 1;
+};
+
+// Vectors per thread whose loads the CG step issues before its scalar sums.
+template <class T>
+constexpr int step_prefetch_vecs() {
+  return sizeof(T) == 4 ? 2 : 4;
+}
+// The same in elements (the step grid is sized by it).
 template <class T>
 constexpr int step_prefetch() {
-  return sizeof(T) == 4 ? 8 : 4;
+  return step_prefetch_vecs<T>() * StepVec<T>::V;
 }
 
 // KB of the single-reduction form (see the file comment).  u is D^-1 r.  In
@@ -439,7 +462,9 @@ constexpr int step_prefetch() {
 // u_{i+1} / D^-1 is formed for the shares of r.u and r.r (ten vector streams
 // per element instead of twelve; iteration counts within 1 %).  In FP32 that
 // recurrence took 6-8 % more iterations, so r is kept there (r unused and
-// may be NULL in FP64).
+// may be NULL in FP64).  In FP32 each thread moves 16-byte vectors of every
+// stream (the vectors are 16-byte aligned: they sit at 256-byte offsets of a
+// 256-byte aligned workspace); the last n % 4 elements go one per thread.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x, T *__restrict__ r,
                                                     T *__restrict__ u, T *__restrict__ p,
@@ -453,26 +478,36 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
                                                     double *__restrict__ part_rr_new, int q,
                                                     PcgState *__restrict__ st) {
   if (st->done) return;
+  using VT = typename StepVec<T>::type;
+  constexpr int V = StepVec<T>::V;
+  constexpr int kPre = step_prefetch_vecs<T>();
+  constexpr bool kStoreR = sizeof(T) == 4;
+  union Pack {
+    VT v;
+    T e[V];
+  };
   const int iter = st->it[q ^ 1];
-  // the first kStepPrefetch elements of this thread are loaded before the
-  // scalars are summed, so their latency overlaps the partial sums
+  const int64_t nv = n / V;  // whole vectors
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  constexpr int kStepPrefetch = step_prefetch<T>();
-  constexpr bool kStoreR = sizeof(T) == 4;
-  T ru[kStepPrefetch], rp[kStepPrefetch], rw[kStepPrefetch], rs[kStepPrefetch];
-  T rx[kStepPrefetch], rv[kStepPrefetch], rd[kStepPrefetch];
+  const VT *u4 = reinterpret_cast<const VT *>(u), *p4 = reinterpret_cast<const VT *>(p);
+  const VT *w4 = reinterpret_cast<const VT *>(w), *s4 = reinterpret_cast<const VT *>(s);
+  const VT *x4 = reinterpret_cast<const VT *>(x), *r4 = reinterpret_cast<const VT *>(r);
+  const VT *d4 = reinterpret_cast<const VT *>(dinv);
+  // the first kPre vectors of this thread are loaded before the scalars are
+  // summed, so their latency overlaps the partial sums
+  Pack ru[kPre], rp[kPre], rw[kPre], rs[kPre], rx[kPre], rv[kPre], rd[kPre];
 #pragma unroll
-  for (int k = 0; k < kStepPrefetch; ++k) {
-    const int64_t i = i0 + k * stride;
-    if (i < n) {
-      ru[k] = u[i];
-      rp[k] = p[i];
-      rw[k] = w[i];
-      rs[k] = s[i];
-      rx[k] = x[i];
-      if constexpr (kStoreR) rv[k] = r[i];
-      rd[k] = dinv[i];
+  for (int k = 0; k < kPre; ++k) {
+    const int64_t iv = i0 + k * stride;
+    if (iv < nv) {
+      ru[k].v = u4[iv];
+      rp[k].v = p4[iv];
+      rw[k].v = w4[iv];
+      rs[k].v = s4[iv];
+      rx[k].v = x4[iv];
+      if constexpr (kStoreR) rv[k].v = r4[iv];
+      rd[k].v = d4[iv];
     }
   }
   double gamma, rr, delta;
@@ -503,55 +538,68 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
   }
   const T a = T(alpha), bt = T(beta);
   double sg = 0.0, srr = 0.0;
-  // p and s hold no values before iteration 0: selected away, never scaled.
-  // x, r, p, s are stored non-temporally (read again only by the next step;
-  // measured 4 % faster on the 64^3 Hex8 solve), u through the cache.
-#pragma unroll
-  for (int k = 0; k < kStepPrefetch; ++k) {
-    const int64_t i = i0 + k * stride;
-    if (i < n) {
-      const T pi = first ? ru[k] : ru[k] + bt * rp[k];
-      const T si = first ? rw[k] : rw[k] + bt * rs[k];
-      __builtin_nontemporal_store(pi, &p[i]);
-      __builtin_nontemporal_store(si, &s[i]);
-      __builtin_nontemporal_store(rx[k] + a * pi, &x[i]);
-      double ri;
-      T ui;
-      if constexpr (kStoreR) {
-        const T rn = rv[k] - a * si;
-        __builtin_nontemporal_store(rn, &r[i]);
-        ui = rd[k] * rn;
-        ri = (double)rn;
-      } else {
-        ui = ru[k] - a * (rd[k] * si);
-        ri = (double)ui / (double)rd[k];
-      }
-      u[i] = ui;  // cached: the next operator gathers it
-      sg += ri * (double)ui;
-      srr += ri * ri;
-    }
-  }
-  for (int64_t i = i0 + kStepPrefetch * stride; i < n; i += stride) {
-    const T pi = first ? u[i] : u[i] + bt * p[i];
-    const T si = first ? w[i] : w[i] + bt * s[i];
-    __builtin_nontemporal_store(pi, &p[i]);
-    __builtin_nontemporal_store(si, &s[i]);
-    __builtin_nontemporal_store(x[i] + a * pi, &x[i]);
-    const T di = dinv[i];
+  // one element: p and s hold no values before iteration 0 (selected away,
+  // never scaled); returns the new u
+  const auto step1 = [&](T ui, T pi, T wi, T si, T xi, T rvi, T di, T &pn, T &sn, T &xn, T &rn) {
+    pn = first ? ui : ui + bt * pi;
+    sn = first ? wi : wi + bt * si;
+    xn = xi + a * pn;
     double ri;
-    T ui;
+    T un;
     if constexpr (kStoreR) {
-      const T rn = r[i] - a * si;
-      __builtin_nontemporal_store(rn, &r[i]);
-      ui = di * rn;
+      rn = rvi - a * sn;
+      un = di * rn;
       ri = (double)rn;
     } else {
-      ui = u[i] - a * (di * si);
-      ri = (double)ui / (double)di;
+      un = ui - a * (di * sn);
+      ri = (double)un / (double)di;
     }
-    u[i] = ui;
-    sg += ri * (double)ui;
+    sg += ri * (double)un;
     srr += ri * ri;
+    return un;
+  };
+  // x, r, p, s are stored non-temporally (read again only by the next step;
+  // measured 4 % faster on the 64^3 Hex8 solve), u through the cache (the next
+  // operator gathers it)
+  const auto stepv = [&](int64_t iv, const Pack &U, const Pack &P, const Pack &W, const Pack &S,
+                         const Pack &X, const Pack &R, const Pack &D) {
+    Pack pn, sn, xn, rn, un;
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+      un.e[e] = step1(U.e[e], P.e[e], W.e[e], S.e[e], X.e[e], kStoreR ? R.e[e] : T(0), D.e[e],
+                      pn.e[e], sn.e[e], xn.e[e], rn.e[e]);
+    __builtin_nontemporal_store(pn.v, reinterpret_cast<VT *>(p) + iv);
+    __builtin_nontemporal_store(sn.v, reinterpret_cast<VT *>(s) + iv);
+    __builtin_nontemporal_store(xn.v, reinterpret_cast<VT *>(x) + iv);
+    if constexpr (kStoreR) __builtin_nontemporal_store(rn.v, reinterpret_cast<VT *>(r) + iv);
+    reinterpret_cast<VT *>(u)[iv] = un.v;
+  };
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int64_t iv = i0 + k * stride;
+    if (iv < nv) stepv(iv, ru[k], rp[k], rw[k], rs[k], rx[k], rv[k], rd[k]);
+  }
+  for (int64_t iv = i0 + kPre * stride; iv < nv; iv += stride) {
+    Pack U, P, W, S, X, R, D;
+    U.v = u4[iv];
+    P.v = p4[iv];
+    W.v = w4[iv];
+    S.v = s4[iv];
+    X.v = x4[iv];
+    if constexpr (kStoreR) R.v = r4[iv];
+    D.v = d4[iv];
+    stepv(iv, U, P, W, S, X, R, D);
+  }
+  // the last n % V elements, one per thread
+  if (i0 < n - nv * V) {
+    const int64_t i = nv * V + i0;
+    T pn, sn, xn, rn;
+    const T un = step1(u[i], p[i], w[i], s[i], x[i], kStoreR ? r[i] : T(0), dinv[i], pn, sn, xn, rn);
+    p[i] = pn;
+    s[i] = sn;
+    x[i] = xn;
+    if constexpr (kStoreR) r[i] = rn;
+    u[i] = un;
   }
   block_sum2(sg, srr);
   if (threadIdx.x == 0) {
@@ -1106,6 +1154,7 @@ extern "C" int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr,
   TATVA_REQUIRE(tol >= 0.0, "tol must be non-negative");
   const WsLayout L = ws_layout(n, kCsrVecs, sizeof(double), csr_op_parts(n));
   TATVA_REQUIRE(workspace && ws_bytes >= (int64_t)L.total, "PCG workspace too small");
+  TATVA_REQUIRE(((uintptr_t)workspace & 255) == 0, "PCG workspace must be 256-byte aligned");
   chunk += chunk & 1;  // parity ping-pong needs an even chunk
   int device = 0;
   TATVA_TRY_HIP(hipGetDevice(&device));
@@ -1288,6 +1337,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   TATVA_REQUIRE(m && d && u_in && diag_in && b_in && x_io && n > 0, "invalid arguments");
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
   const WsLayout L = ws_layout(n, kMatfreeVecs, sizeof(float));
+  TATVA_REQUIRE(((uintptr_t)workspace & 255) == 0, "PCG workspace must be 256-byte aligned");
   TATVA_REQUIRE(workspace && ws_bytes >= tatva_pcg_matfree_workspace(m, d, n),
                 "PCG workspace too small");
   const int64_t n_full = m->n_nodes * density_ncomp(m, d);
@@ -1457,6 +1507,7 @@ extern "C" int tatva_pcg_matfree_f32_mapped(tatva_mesh *m, const tatva_density *
                 "invalid arguments");
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
   const WsLayout L = ws_layout(n, kMatfreeVecs, sizeof(float));
+  TATVA_REQUIRE(((uintptr_t)workspace & 255) == 0, "PCG workspace must be 256-byte aligned");
   TATVA_REQUIRE(workspace && ws_bytes >= tatva_pcg_matfree_mapped_workspace(m, n),
                 "PCG workspace too small");
   const int64_t n_full = m->n_nodes * density_ncomp(m, d);
