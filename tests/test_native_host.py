@@ -381,3 +381,19 @@ def test_padded_node_layout_matches_brute_force():
     # nothing to pad when whole nodes are constrained
     assert padded_node_layout(m, lifter.Lifter(m.n_nodes * 3, lifter.Fixed(
         (x0[:, None] * 3 + np.arange(3)).ravel(), 0.0)), 3) is None
+
+
+def test_pcg_workspace_must_be_aligned():
+    """The PCG entry points move their vectors in 16-byte loads from 256-byte
+    offsets of the workspace, so they reject a workspace that is not 256-byte
+    aligned before any device work (the pointers below are never dereferenced)."""
+    L = nat.lib()
+    n = 1000
+    need = int(L.tatva_pcg_csr_workspace(n, None))
+    fake = nat.ctypes.c_void_p(1 << 20)
+    st = nat.PcgStats()
+    rc = L.tatva_pcg_csr(n, 5 * n, fake, fake, fake, None, 1, fake, fake, 1e-8, 10, 16,
+                         nat.ctypes.c_void_p((1 << 20) + 16), need, nat.ctypes.byref(st), None)
+    assert rc == nat.EINVAL
+    with pytest.raises(ValueError, match="256-byte aligned"):
+        nat.check(rc)
