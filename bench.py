@@ -239,7 +239,7 @@ def c1_latency(reps=20):
             "cg_iterations": its, "n_free": solver.red.size}
 
 
-def serve_stream(count, qps, seed=0, max_slots=64, chunk=8, hbm_budget=200e9):
+def serve_stream(count, qps, seed=0, max_slots=64, chunk=8, hbm_budget=200e9, prefetch=2):
     """Continuous-batching server on the mixed Hex8 LE / Neo-Hookean request
     stream (n log-uniform in [8, 48]); first an untimed warm-up request of
     each material at a size inside the stream's range, so that one-time
@@ -248,7 +248,7 @@ def serve_stream(count, qps, seed=0, max_slots=64, chunk=8, hbm_budget=200e9):
     in neither path's first request."""
     from tatva_amd import serve
 
-    srv = serve.Server(max_slots=max_slots, chunk=chunk, hbm_budget=hbm_budget)
+    srv = serve.Server(max_slots=max_slots, chunk=chunk, hbm_budget=hbm_budget, prefetch=prefetch)
     srv.run([serve.SolveRequest(rid=-2, n=40, material="linear_elastic", E=1.0, nu=0.3),
              serve.SolveRequest(rid=-1, n=40, material="neo_hookean", E=1.0, nu=0.3,
                                 load_steps=2)], realtime=False)
@@ -258,6 +258,7 @@ def serve_stream(count, qps, seed=0, max_slots=64, chunk=8, hbm_budget=200e9):
     s = rep.summary()
     s.update({
         "qps": qps, "max_slots": max_slots, "chunk": chunk, "hbm_budget_gb": hbm_budget / 1e9,
+        "prefetch_threads": prefetch,
         "workload": f"{count} requests, Poisson arrivals at {qps}/s, hex8 cubes n in [8, 48] "
                     "log-uniform, E ~ LogNormal(0, 0.5), nu ~ U(0.2, 0.45), linear elastic "
                     "(traction 1e-3) / Neo-Hookean (2 load steps of u_x += 0.03) 50/50, "
@@ -275,10 +276,12 @@ def serve_sweep(args):
     """The serving stream at each QPS of --serve-qps (--serve-requests requests
     each, 200 GB budget), then once more at the highest QPS under the binding
     budget --serve-budget-gb (admissions wait for memory)."""
-    out = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk) for q in args.serve_qps]
+    out = [serve_stream(args.serve_requests, q, chunk=args.serve_chunk,
+                        prefetch=args.serve_prefetch) for q in args.serve_qps]
     if args.serve_budget_gb > 0:
         out.append(serve_stream(args.serve_requests, max(args.serve_qps), chunk=args.serve_chunk,
-                                hbm_budget=args.serve_budget_gb * 1e9))
+                                hbm_budget=args.serve_budget_gb * 1e9,
+                                prefetch=args.serve_prefetch))
     return out
 
 
@@ -773,6 +776,8 @@ def main(argv=None):
     ap.add_argument("--serve-budget-gb", type=float, default=0.4,
                     help="HBM budget of the extra serving run at the highest QPS (0: skip)")
     ap.add_argument("--serve-chunk", type=int, default=8, help="batched PCG iterations per poll")
+    ap.add_argument("--serve-prefetch", type=int, default=2,
+                    help="serving: threads preparing arrived requests")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path's "

@@ -247,13 +247,15 @@ class _Job:
         self.red.lifter = self.lifter
         self.lifter.apply_constraints(self.u)
 
-    def begin(self):
+    def begin(self, batch_stream=None):
         self._set_load_step(1)
-        return self._newton()
+        return self._newton(batch_stream)
 
-    def _newton(self) -> bool:
-        """FP64 residual and convergence test; starts the next linear solve in
-        the batch.  Returns True while the job still has work in the batch."""
+    def _newton(self, batch_stream=None) -> bool:
+        """FP64 residual and convergence test on the current stream; starts the
+        next linear solve in the batch on batch_stream (default: the current
+        one), after the current stream's work.  Returns True while the job
+        still has work in the batch."""
         srv = self.server
         L = nat.lib()
         while True:
@@ -283,17 +285,25 @@ class _Job:
                                         dtype=nat.torch().float32)
         self._diag = diag
         self.x32.zero_()
+        t = nat.torch()
+        cur = t.cuda.current_stream()
+        bs = cur if batch_stream is None else batch_stream
+        if bs != cur:
+            # the solve's set-up reads what this stream wrote (and diag, made here)
+            bs.wait_stream(cur)
+            diag.record_stream(bs)
         nat.check(L.tatva_batch_start(srv._batch, self.slot, self.op.handle,
                                       ctypes.byref(self.energy._desc), nat.dptr(self.u32),
                                       nat.dptr(dmap), self.n, nat.dptr(diag), nat.dptr(self.b32),
                                       nat.dptr(self.x32), float(srv.inner_tol),
                                       int(self.req.max_cg or srv.maxiter_factor * self.n),
                                       nat.dptr(self.ws),
-                                      self.ws.numel(), s))
+                                      self.ws.numel(), bs.cuda_stream))
         return True
 
-    def on_linear_done(self, st) -> bool:
-        """Newton update u_free += x after the batch reports the solve done."""
+    def on_linear_done(self, st, batch_stream=None) -> bool:
+        """Newton update u_free += x after the batch reports the solve done (on
+        the current stream; the next solve starts on batch_stream)."""
         self.result.cg_iterations += int(st.iterations)
         if st.status == nat.EBREAKDOWN:
             return self._finish(False, "CG breakdown")
@@ -302,7 +312,7 @@ class _Job:
         L = nat.lib()
         nat.check(L.tatva_axpy_free_f32(self.n, nat.dptr(self.lifter.free_device()), 1.0,
                                         nat.dptr(self.x32), nat.dptr(self.u), nat.stream()))
-        return self._newton()
+        return self._newton(batch_stream)
 
     def _finish(self, ok: bool, failure: str | None = None) -> bool:
         self.done = True
@@ -383,7 +393,10 @@ class Server:
     from the start of the run) and returns a ServeReport.  ``max_slots``
     bounds the requests in flight, ``hbm_budget`` their device memory
     (SolveRequest.hbm_bytes); ``chunk`` PCG iterations run between polls.
-    ``prefetch`` worker threads prepare arrived requests (mesh, tiling,
+    After each poll the solves still iterating are launched again at once,
+    and the FP64 Newton steps of the finished ones (residual, norm, Jacobi
+    diagonal) run meanwhile on a second stream; their next solves join the
+    launch after.  ``prefetch`` worker threads prepare arrived requests (mesh, tiling,
     constraints, device buffers) while the GPU iterates the others, each on
     a torch stream of its own so that preparation does not wait for the
     batch work queued on the serving stream; a prepared request is admitted
@@ -418,10 +431,10 @@ class Server:
         """Worker thread: the request's _Job, built on the thread's own stream,
         and an event the serving stream waits for before using it.  (A new
         thread starts on device 0, so the server's device is entered first.
-        A job's tensors are freed only after the serving stream has been
-        synchronised past its last use (the batch poll or its last residual
-        norm), so their reuse on this stream cannot overtake work queued
-        there.)"""
+        A job's tensors are freed only after the streams of their last uses
+        have been synchronised (the batch poll on the serving stream, the last
+        residual norm on the Newton stream), so their reuse on this stream
+        cannot overtake work queued there.)"""
         t = nat.torch()
         with t.cuda.device(self._device):
             st = getattr(self._local, "stream", None)
@@ -462,6 +475,12 @@ class Server:
         peak_bytes = 0
         dof_iters = 0
         pool = ThreadPoolExecutor(max_workers=self.prefetch, thread_name_prefix="tatva-prepare")
+        serving = t.cuda.current_stream()
+        # Newton steps of finished solves run here while the next batch launch
+        # runs on the serving stream; their next solves start on the serving
+        # stream, after that launch
+        newton = t.cuda.Stream(device=self._device)
+        ahead = False  # a batch launch is queued that no poll has seen yet
         t.cuda.synchronize()
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else None
@@ -484,11 +503,13 @@ class Server:
                     preparing.remove(item)
                     req, fut = item
                     job, ready = fut.result()
-                    t.cuda.current_stream().wait_event(ready)
+                    newton.wait_event(ready)
                     slot = free_slots.pop()
                     job.attach(slot, time.perf_counter() - t0)
                     jobs[slot] = job
-                    if not job.begin():
+                    with t.cuda.stream(newton):
+                        started = job.begin(serving)
+                    if not started:
                         used_bytes -= req.hbm_bytes()
                         self._retire(job, jobs, free_slots, results, t0)
                 peak_adm = max(peak_adm, len(jobs))
@@ -500,20 +521,31 @@ class Server:
                     elif pending and realtime:
                         time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
                     continue
-                nat.check(L.tatva_batch_iterate(self._batch, self.chunk, nat.stream()))
-                launches += 1
+                if not ahead:
+                    nat.check(L.tatva_batch_iterate(self._batch, self.chunk, serving.cuda_stream))
+                    launches += 1
                 running = ctypes.c_int32(0)
                 nat.check(L.tatva_batch_poll(self._batch, self._stats, ctypes.byref(running),
-                                             nat.stream()))
-                for slot in list(jobs):
-                    st = self._stats[slot]
-                    if not st.reserved:  # still iterating
-                        continue
-                    job = jobs[slot]
-                    dof_iters += job.n * int(st.iterations)
-                    if not job.on_linear_done(st):
-                        used_bytes -= job.req.hbm_bytes()
-                        self._retire(job, jobs, free_slots, results, t0)
+                                             serving.cuda_stream))
+                # the solves still iterating go on at once; the finished ones
+                # take their Newton steps meanwhile and join the launch after
+                ahead = running.value > 0
+                if ahead:
+                    nat.check(L.tatva_batch_iterate(self._batch, self.chunk, serving.cuda_stream))
+                    launches += 1
+                finished = [slot for slot in jobs if self._stats[slot].reserved]
+                if finished:
+                    # (the poll synchronised the serving stream: the finished
+                    # solves' x are complete; the launch queued since does not
+                    # touch their slots, which it skips as done)
+                    with t.cuda.stream(newton):
+                        for slot in finished:
+                            st = self._stats[slot]
+                            job = jobs[slot]
+                            dof_iters += job.n * int(st.iterations)
+                            if not job.on_linear_done(st, serving):
+                                used_bytes -= job.req.hbm_bytes()
+                                self._retire(job, jobs, free_slots, results, t0)
         finally:
             pool.shutdown(wait=True)
         t.cuda.synchronize()
