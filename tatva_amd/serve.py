@@ -177,9 +177,10 @@ class CubeProblem:
             self.energy = self.op.energy(NeoHookean(mu, lam))
             x1 = np.where(np.isclose(mesh.coords[:, 0], 1.0))[0]
             self._x1 = x1
-            self._n_clamp = clamp.size
-            self.lifter0 = Lifter(self.size, Fixed(np.concatenate([clamp, x1 * 3]),
-                                                   RuntimeValue("load")))
+            # the prescribed u_x is one scalar per load step, written by the
+            # device kernel without a host-to-device copy of a value array
+            self.lifter0 = Lifter(self.size, Fixed(clamp, 0.0),
+                                  Fixed(x1 * 3, RuntimeValue("load")))
             self.ramp = t.zeros(self.size, dtype=t.float64, device=nat.device())
             self.ramp[0::3] = t.as_tensor(mesh.coords[:, 0], device=nat.device())
 
@@ -187,9 +188,7 @@ class CubeProblem:
         """Constraints of load step ``step`` (1-based)."""
         if self._x1 is None:
             return self.lifter0
-        vals = np.concatenate([np.zeros(self._n_clamp),
-                               np.full(self._x1.size, self.req.stretch * step)])
-        return self.lifter0.at["load"].set(vals)
+        return self.lifter0.at["load"].set(float(self.req.stretch * step))
 
     def predict(self, u) -> None:
         """Start of a load step: homogeneous-stretch increment u_x += stretch * x."""
@@ -231,6 +230,10 @@ class _Job:
         self.r0 = None
         self.newton_in_step = 0
         self.done = False
+        self.solving = False
+        self._r = None
+        self._ev = None
+        self._rr_host = t.empty(1, dtype=t.float64, pin_memory=True)  # r . r
 
     def attach(self, slot: int, admitted: float) -> None:
         self.slot = slot
@@ -247,39 +250,59 @@ class _Job:
         self.red.lifter = self.lifter
         self.lifter.apply_constraints(self.u)
 
-    def begin(self, batch_stream=None):
-        self._set_load_step(1)
-        return self._newton(batch_stream)
+    # A job is either solving (its linear system iterates in the batch) or
+    # waiting for the norm of its FP64 residual: the residual and its dot are
+    # queued on the current stream with a copy of the dot to pinned host
+    # memory and an event, and the server takes the Newton decision
+    # (on_norm) once the event has completed, without blocking on it.
 
-    def _newton(self, batch_stream=None) -> bool:
-        """FP64 residual and convergence test on the current stream; starts the
-        next linear solve in the batch on batch_stream (default: the current
-        one), after the current stream's work.  Returns True while the job
-        still has work in the batch."""
+    def begin(self):
+        """Queue the first residual (load step 1)."""
+        self._set_load_step(1)
+        self._start_residual()
+
+    def _start_residual(self):
+        from .solvers import dot  # noqa: PLC0415
+
+        t = nat.torch()
+        self._r = self.red.residual_full_state(self.u)
+        self._rr_host.copy_(dot(self._r, self._r).reshape(1), non_blocking=True)
+        self._ev = t.cuda.Event()
+        self._ev.record()
+        self.solving = False
+
+    def norm_ready(self) -> bool:
+        return not self.solving and not self.done and self._ev.query()
+
+    def on_norm(self, batch_stream=None) -> bool:
+        """The Newton decision at the residual norm: next load step (another
+        residual queued), end, or the next linear solve (its set-up queued on
+        the current stream, the solve started in the batch on batch_stream,
+        after the current stream's work).  Returns False when the job is done."""
         srv = self.server
         L = nat.lib()
-        while True:
-            r = self.red.residual_full_state(self.u)
-            rn = _norm(r)
-            if self.r0 is None:
-                self.r0 = rn
-            rel = rn / self.r0 if self.r0 > 0 else 0.0
-            self.result.rel_residual = rel
-            if not math.isfinite(rn):
-                return self._finish(False, "non-finite residual")
-            if self.newton_in_step >= srv.max_newton:
-                return self._finish(False, "Newton iteration limit")
-            if rn <= srv.tol * self.r0 or rn == 0.0:
-                if self.step < self.req.load_steps:
-                    self._set_load_step(self.step + 1)
-                    continue
-                return self._finish(True)
-            break
+        rn = math.sqrt(max(float(self._rr_host[0]), 0.0))
+        r = self._r
+        if self.r0 is None:
+            self.r0 = rn
+        rel = rn / self.r0 if self.r0 > 0 else 0.0
+        self.result.rel_residual = rel
+        if not math.isfinite(rn):
+            return self._finish(False, "non-finite residual")
+        if self.newton_in_step >= srv.max_newton:
+            return self._finish(False, "Newton iteration limit")
+        if rn <= srv.tol * self.r0 or rn == 0.0:
+            if self.step < self.req.load_steps:
+                self._set_load_step(self.step + 1)
+                self._start_residual()
+                return True
+            return self._finish(True)
         self.newton_in_step += 1
         self.result.newton_iterations += 1
         s = nat.stream()
         nat.check(L.tatva_cast_f64_f32(self.u.numel(), 1.0, nat.dptr(self.u), nat.dptr(self.u32), s))
         nat.check(L.tatva_cast_f64_f32(self.n, -1.0, nat.dptr(r), nat.dptr(self.b32), s))
+        self._r = None
         dmap = self.lifter.dof_map_device()
         diag = self.energy.hessian_diag(self.u32, dof_map=dmap, n_out=self.n,
                                         dtype=nat.torch().float32)
@@ -299,11 +322,13 @@ class _Job:
                                       int(self.req.max_cg or srv.maxiter_factor * self.n),
                                       nat.dptr(self.ws),
                                       self.ws.numel(), bs.cuda_stream))
+        self.solving = True
         return True
 
-    def on_linear_done(self, st, batch_stream=None) -> bool:
-        """Newton update u_free += x after the batch reports the solve done (on
-        the current stream; the next solve starts on batch_stream)."""
+    def on_linear_done(self, st) -> bool:
+        """Newton update u_free += x after the batch reports the solve done, and
+        the next residual queued (on the current stream).  Returns False when
+        the solve failed (the job is done)."""
         self.result.cg_iterations += int(st.iterations)
         if st.status == nat.EBREAKDOWN:
             return self._finish(False, "CG breakdown")
@@ -312,7 +337,8 @@ class _Job:
         L = nat.lib()
         nat.check(L.tatva_axpy_free_f32(self.n, nat.dptr(self.lifter.free_device()), 1.0,
                                         nat.dptr(self.x32), nat.dptr(self.u), nat.stream()))
-        return self._newton(batch_stream)
+        self._start_residual()
+        return True
 
     def _finish(self, ok: bool, failure: str | None = None) -> bool:
         self.done = True
@@ -325,13 +351,7 @@ class _Job:
         if self.server.keep_solutions:
             self.result.u = self.u
         self.op.close()
-        self.ws = self.u32 = self.b32 = self.x32 = self._diag = None
-
-
-def _norm(x) -> float:
-    from .solvers import norm  # noqa: PLC0415
-
-    return norm(x)
+        self.ws = self.u32 = self.b32 = self.x32 = self._diag = self._r = None
 
 
 @dataclass
@@ -393,10 +413,12 @@ class Server:
     from the start of the run) and returns a ServeReport.  ``max_slots``
     bounds the requests in flight, ``hbm_budget`` their device memory
     (SolveRequest.hbm_bytes); ``chunk`` PCG iterations run between polls.
-    After each poll the solves still iterating are launched again at once,
-    and the FP64 Newton steps of the finished ones (residual, norm, Jacobi
-    diagonal) run meanwhile on a second stream; their next solves join the
-    launch after.  ``prefetch`` worker threads prepare arrived requests (mesh, tiling,
+    After each poll the solves still iterating are launched again at once;
+    the FP64 Newton work of the finished ones (update, residual and its norm,
+    Jacobi diagonal) runs on a second stream, and the loop takes a job's
+    Newton decision only when its residual norm has reached the host (an
+    event it polls, never waits on while the batch has work): its next solve
+    joins a later launch.  ``prefetch`` worker threads prepare arrived requests (mesh, tiling,
     constraints, device buffers) while the GPU iterates the others, each on
     a torch stream of its own so that preparation does not wait for the
     batch work queued on the serving stream; a prepared request is admitted
@@ -508,18 +530,25 @@ class Server:
                     job.attach(slot, time.perf_counter() - t0)
                     jobs[slot] = job
                     with t.cuda.stream(newton):
-                        started = job.begin(serving)
-                    if not started:
-                        used_bytes -= req.hbm_bytes()
-                        self._retire(job, jobs, free_slots, results, t0)
+                        job.begin()
                 peak_adm = max(peak_adm, len(jobs))
                 peak_bytes = max(peak_bytes, used_bytes)
+                # Newton decisions of the jobs whose residual norms have arrived
+                with t.cuda.stream(newton):
+                    for slot, job in list(jobs.items()):
+                        if job.norm_ready() and not job.on_norm(serving):
+                            used_bytes -= job.req.hbm_bytes()
+                            self._retire(job, jobs, free_slots, results, t0)
                 if not jobs:
                     if preparing:
                         futures_wait([p[1] for p in preparing], timeout=0.01,
                                      return_when=FIRST_COMPLETED)
                     elif pending and realtime:
                         time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
+                    continue
+                if not ahead and not any(job.solving for job in jobs.values()):
+                    # nothing in the batch: wait for the oldest residual norm
+                    next(job for job in jobs.values() if not job.solving)._ev.synchronize()
                     continue
                 if not ahead:
                     nat.check(L.tatva_batch_iterate(self._batch, self.chunk, serving.cuda_stream))
@@ -533,7 +562,8 @@ class Server:
                 if ahead:
                     nat.check(L.tatva_batch_iterate(self._batch, self.chunk, serving.cuda_stream))
                     launches += 1
-                finished = [slot for slot in jobs if self._stats[slot].reserved]
+                finished = [slot for slot, job in jobs.items()
+                            if job.solving and self._stats[slot].reserved]
                 if finished:
                     # (the poll synchronised the serving stream: the finished
                     # solves' x are complete; the launch queued since does not
@@ -543,7 +573,7 @@ class Server:
                             st = self._stats[slot]
                             job = jobs[slot]
                             dof_iters += job.n * int(st.iterations)
-                            if not job.on_linear_done(st, serving):
+                            if not job.on_linear_done(st):
                                 used_bytes -= job.req.hbm_bytes()
                                 self._retire(job, jobs, free_slots, results, t0)
         finally:
