@@ -275,7 +275,9 @@ int tatva_spmv_bcsr(int64_t n, const int32_t *indptr, const int32_t *block_cols,
  * synchronisation per call).  The workspace query takes the HOST copy of
  * indptr when the solve will run without a block view (NULL: with one), and
  * then includes the sliced-ELL copy.  The library keeps no buffer of the
- * solve after it returns. */
+ * solve after it returns.  The PCG workspaces (this one and the matrix-free
+ * ones below) must be 256-byte aligned (device allocations are): the CG step
+ * moves its vectors in 16-byte loads from 256-byte offsets of it. */
 int64_t tatva_pcg_csr_workspace(int64_t n, const int32_t *host_indptr);
 int tatva_pcg_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int32_t *indices,
                   const double *data, const int32_t *block_cols, int32_t block_size,
