@@ -171,6 +171,32 @@ __device__ __forceinline__ T det_only(const T J[3][3]) {
          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
 }
 
+// Geometry of a simplex (Tri3 / Tet4: constant over the element), the general
+// formula below with its constant dN/dxi folded in: J[i][j] = X[i+1][j] -
+// X[0][j], B[i][a] = inv[i][a-1] for a >= 1, B[i][0] = -(inv[i][0] + ... +
+// inv[i][D-1]) (the general form multiplies by the constants 1, -1 and 0,
+// which strict FP semantics may not drop).
+template <int D, int NNE>
+__device__ __forceinline__ double simplex_geometry(const double X[NNE][D], double B[D][NNE]) {
+  static_assert(NNE == D + 1, "simplex");
+  double J[D][D], inv[D][D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) J[i][j] = X[i + 1][j] - X[0][j];
+  const double det = det_inv(J, inv);
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double s = -inv[i][0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) s -= inv[i][k];
+    B[i][0] = s;
+#pragma unroll
+    for (int a = 1; a < NNE; ++a) B[i][a] = inv[i][a - 1];
+  }
+  return det;
+}
+
 // Geometry at one quadrature point: B = dNdX (D x NNE) and signed detJ, in FP64.
 template <int KIND>
 __device__ __forceinline__ double qp_geometry(const double X[Elem<KIND>::NNE][Elem<KIND>::D],
@@ -178,6 +204,7 @@ __device__ __forceinline__ double qp_geometry(const double X[Elem<KIND>::NNE][El
                                               double B[Elem<KIND>::D][Elem<KIND>::NNE]) {
   using E = Elem<KIND>;
   constexpr int D = E::D, NNE = E::NNE;
+  if constexpr (KIND != TATVA_HEX8) return simplex_geometry<D, NNE>(X, B);
   double dn[D][NNE];
   E::dN(xi, dn);
   double J[D][D];
