@@ -96,8 +96,17 @@ int tatva_set_profiling(int enable);
  * workspace, so that every HVP reads them instead of the node coordinates (the
  * values the product computes itself, so the same result bit for bit): 1
  * (default) on affine Hex8 meshes, 0 never.  Both caches are in the workspace
- * only when selected: set the options before querying its size. */
-enum { TATVA_OPT_MATFREE_LIN_CACHE = 1, TATVA_OPT_MATFREE_GEOM_CACHE = 2 };
+ * only when selected: set the options before querying its size.
+ * TATVA_OPT_MATFREE_DEFERRED_UPDATE: whether tatva_pcg_matfree_f32 leaves each
+ * CG iteration's p and x updates to the tile kernel of the next HVP (which has
+ * memory bandwidth to spare) instead of its step kernel, with z in two
+ * alternating buffers: 1 (default) on, 0 off.  The iterates are the same bits
+ * either way. */
+enum {
+  TATVA_OPT_MATFREE_LIN_CACHE = 1,
+  TATVA_OPT_MATFREE_GEOM_CACHE = 2,
+  TATVA_OPT_MATFREE_DEFERRED_UPDATE = 3
+};
 int tatva_set_option(int32_t option, int32_t value);
 
 /* ---------------------------------------------------------------- mesh ---- */

@@ -75,6 +75,89 @@ enum { kGeomGeneral = 0, kGeomAffine = 1, kGeomCache = 2 };
 // computing the geometry themselves use.
 constexpr int kGeoFloats = 11;
 
+// The p and x updates of an FP32 conjugate-gradient iteration, which its step
+// kernel leaves to the next operator launch (nothing reads p or x in between):
+// p = z + b p (p = z on the first iteration) and x = x + a p.  The step writes
+// the coefficients; on: 0 no update pending, 1 first iteration, 2 otherwise.
+struct CgDeferCoef {
+  float a, b;
+  int on, pad;
+};
+struct CgDefer {
+  const float *z;  // the z the update belongs to (the previous iteration's)
+  float *p, *x;
+  const CgDeferCoef *coef;
+  int64_t n;  // 0: the launch carries no update
+};
+
+typedef float cg_f4 __attribute__((ext_vector_type(4)));
+
+// A tile kernel carrying a deferred update runs it in workgroups of its own,
+// interleaved with the tiles' so that the streams spread over the whole launch
+// and no tile waits for them: after every kDeferRows rows of kNumXcd tile
+// workgroups comes one row of kNumXcd streaming workgroups (a row holds one
+// workgroup per XCD, so the tiles keep their XCD mapping).
+constexpr int kDeferRows = 8;
+inline int tile_grid(int n_tiles, bool defer) {
+  if (!defer) return n_tiles;
+  const int per = kNumXcd * kDeferRows;
+  return (n_tiles + per - 1) / per * (per + kNumXcd);
+}
+
+// Streaming workgroup sb of S: the contiguous range [sb nv / S, (sb + 1) nv / S)
+// of the nv 16-byte vectors (16-byte aligned), kDeferUnroll vectors per thread
+// in flight; workgroup 0 also takes the last n % 4 entries.  The expressions are
+// those of the step kernel that would otherwise apply the update, so the
+// iterates are the same bits.
+constexpr int kDeferUnroll = 4;
+__device__ __forceinline__ void cg_defer_stream(const CgDefer &dx, int64_t sb, int64_t S) {
+  const CgDeferCoef c = *dx.coef;
+  if (c.on == 0) return;
+  const bool first = c.on == 1;
+  const float a = c.a, b = c.b;
+  const int64_t nv = dx.n / 4;
+  const int64_t v0 = sb * nv / S, v1 = (sb + 1) * nv / S;
+  const cg_f4 *z4 = reinterpret_cast<const cg_f4 *>(dx.z);
+  cg_f4 *p4 = reinterpret_cast<cg_f4 *>(dx.p);
+  cg_f4 *x4 = reinterpret_cast<cg_f4 *>(dx.x);
+  constexpr int U = kDeferUnroll;
+  for (int64_t base = v0 + threadIdx.x; base < v1; base += U * (int64_t)blockDim.x) {
+    cg_f4 zv[U], pv[U], xv[U];
+    // p is loaded on the first iteration too (and not used), so that no load
+    // waits for another
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t iv = base + k * (int64_t)blockDim.x;
+      if (iv < v1) {
+        zv[k] = z4[iv];
+        pv[k] = p4[iv];
+        xv[k] = x4[iv];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t iv = base + k * (int64_t)blockDim.x;
+      if (iv < v1) {
+        cg_f4 pn, xn;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          pn[e] = first ? zv[k][e] : zv[k][e] + b * pv[k][e];
+          xn[e] = xv[k][e] + a * pn[e];
+        }
+        __builtin_nontemporal_store(pn, p4 + iv);
+        __builtin_nontemporal_store(xn, x4 + iv);
+      }
+    }
+  }
+  if (sb == 0 && threadIdx.x < dx.n - nv * 4) {
+    const int64_t i = nv * 4 + threadIdx.x;
+    const float zi = dx.z[i], pi = dx.p[i];
+    const float pn = first ? zi : zi + b * pi;
+    dx.p[i] = pn;
+    dx.x[i] = dx.x[i] + a * pn;
+  }
+}
+
 // Per-tile sum by the incidence lists instead of element colors.
 template <int KIND>
 constexpr bool tile_gather() {
@@ -765,14 +848,30 @@ __global__ __launch_bounds__(kTile) void k_tile_assemble(const double *__restric
                                                          TileSet ts, T *__restrict__ partial,
                                                          const int *__restrict__ skip, int rev,
                                                          T *__restrict__ lin,
-                                                         const T *__restrict__ geo) {
+                                                         const T *__restrict__ geo, CgDefer dx) {
   if (skip && *skip) return;
   extern __shared__ unsigned char lds_raw[];
+  // dx (FP32 HVP): a CG step's p and x streams, run by the launch's streaming
+  // workgroups on the memory bandwidth the tiles leave idle (grid of tile_grid)
+  int b = blockIdx.x;
+  if constexpr (sizeof(T) == 4 && (MODE == kModeHvp || MODE == kModeHvpLin)) {
+    if (dx.n) {
+      constexpr int per = kNumXcd * kDeferRows;
+      const int grp = b / (per + kNumXcd), j = b - grp * (per + kNumXcd);
+      if (j >= per) {
+        cg_defer_stream(dx, (int64_t)grp * kNumXcd + (j - per),
+                        (int64_t)(gridDim.x / (per + kNumXcd)) * kNumXcd);
+        return;
+      }
+      b = grp * per + j;
+      if (b >= ts.n_tiles) return;
+    }
+  }
   // rev: tiles from the end (a solver alternates the order between iterations
   // so that each pass starts on the data the last-level cache kept from the
   // previous one; every tile writes its own slots, so the result is the same)
-  int tile = (int)xcd_block(blockIdx.x, gridDim.x);
-  if (rev) tile = (int)gridDim.x - 1 - tile;
+  int tile = (int)xcd_block(b, ts.n_tiles);
+  if (rev) tile = ts.n_tiles - 1 - tile;
   tile_assemble<KIND, Dens, T, MODE, GEOM>(tile, coords, quad, dens, u, v, dof_map, ts, partial,
                                            lds_raw, lin, geo);
 }

@@ -1244,7 +1244,7 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
                            const int32_t *dof_map, const double *f_ext, T *out, hipStream_t s,
                            const int *skip, double *part_dot = nullptr, int gather_blocks = 0,
                            int rev = 0, hipEvent_t between = nullptr, T *lin = nullptr,
-                           const T *geo = nullptr) {
+                           const T *geo = nullptr, const CgDefer &dx = CgDefer{}) {
   T *partial = (T *)m->scratch;
   return with_kind(m->kind, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
@@ -1268,16 +1268,24 @@ static int launch_assembly(tatva_mesh *m, const tatva_density *d, const T *u, co
         set_error("a geometry cache serves the FP32 HVP of affine Hex8 meshes only");
         return TATVA_EINVAL;
       }
+      constexpr bool kDeferable = sizeof(T) == 4 && (MODE == kModeHvp || MODE == kModeHvpLin);
+      if (dx.n != 0 && !kDeferable) {
+        set_error("a deferred CG update rides on the FP32 HVP only");
+        return TATVA_EINVAL;
+      }
+      const int grid = tile_grid(m->tiles.n_tiles, dx.n != 0);
       if (geo) {
         if constexpr (kCacheable)
-          k_tile_assemble<K, Dens, T, MODE, kGeomCache><<<m->tiles.n_tiles, kTile, lds, s>>>(
-              m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, geo);
+          k_tile_assemble<K, Dens, T, MODE, kGeomCache><<<grid, kTile, lds, s>>>(
+              m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, geo, dx);
       } else if (K == TATVA_HEX8 && m->hex_affine) {
-        k_tile_assemble<K, Dens, T, MODE, kGeomAffine><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, nullptr);
+        k_tile_assemble<K, Dens, T, MODE, kGeomAffine><<<grid, kTile, lds, s>>>(
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, nullptr,
+            dx);
       } else {
-        k_tile_assemble<K, Dens, T, MODE, kGeomGeneral><<<m->tiles.n_tiles, kTile, lds, s>>>(
-            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, nullptr);
+        k_tile_assemble<K, Dens, T, MODE, kGeomGeneral><<<grid, kTile, lds, s>>>(
+            m->coords, m->quad, dens, u, v, dof_map, m->tiles, partial, skip, rev, lin, nullptr,
+            dx);
       }
       if constexpr (MODE == kModeLinearize) {
         TATVA_CHECK_LAUNCH();
@@ -1379,13 +1387,15 @@ int hvp_gather_blocks(const tatva_mesh *m) {
 int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                        const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
                        const int *skip, int rev, hipEvent_t between, const float *lin,
-                       const float *geo) {
+                       const float *geo, const CgDefer *dx) {
+  const CgDefer none{};
+  const CgDefer &defer = dx ? *dx : none;
   if (lin)
     return launch_assembly<float, kModeHvpLin>(m, d, u, v, dof_map, nullptr, y, s, skip, part_dot,
                                                hvp_gather_blocks(m), rev, between,
-                                               const_cast<float *>(lin), geo);
+                                               const_cast<float *>(lin), geo, defer);
   return launch_assembly<float, kModeHvp>(m, d, u, v, dof_map, nullptr, y, s, skip, part_dot,
-                                          hvp_gather_blocks(m), rev, between, nullptr, geo);
+                                          hvp_gather_blocks(m), rev, between, nullptr, geo, defer);
 }
 
 }  // namespace tatva

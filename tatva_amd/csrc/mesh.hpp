@@ -74,11 +74,13 @@ int launch_geometry_cache_f32(tatva_mesh *m, float *geo, hipStream_t s);
 // y = H(u) v (FP32) with the block shares of v.y in part_dot, over
 // hvp_gather_blocks(m) blocks (v, y reduced through dof_map); rev: tiles in
 // reverse order (same result); between: an event recorded after the tile
-// kernel, before the gather (kernel timing).
+// kernel, before the gather (kernel timing); dx: a CG step's deferred p and x
+// update, applied by the tile kernel.
 int launch_hvp_dot_f32(tatva_mesh *m, const tatva_density *d, const float *u, const float *v,
                        const int32_t *dof_map, float *y, double *part_dot, hipStream_t s,
                        const int *skip = nullptr, int rev = 0, hipEvent_t between = nullptr,
-                       const float *lin = nullptr, const float *geo = nullptr);
+                       const float *lin = nullptr, const float *geo = nullptr,
+                       const CgDefer *dx = nullptr);
 int hvp_gather_blocks(const tatva_mesh *m);
 int density_ncomp(const tatva_mesh *m, const tatva_density *d);
 // Drops cached solver graphs that captured this mesh's arrays.

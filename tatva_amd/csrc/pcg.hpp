@@ -26,6 +26,9 @@ struct PcgState {
   double alpha[2];
   double gamma[2];
   int it[2];
+  // FP32 matrix-free CG: the p and x update of iteration i, written by its
+  // step kernel (slot i mod 2) and applied by the next operator launch
+  CgDeferCoef defer[2];
 };
 
 // Vector kernels use at most this many blocks (4 per CU): the partials each

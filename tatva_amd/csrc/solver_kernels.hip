@@ -465,7 +465,12 @@ constexpr int step_prefetch() {
 // may be NULL in FP64).  In FP32 each thread moves 16-byte vectors of every
 // stream (the vectors are 16-byte aligned: they sit at 256-byte offsets of a
 // 256-byte aligned workspace); the last n % 4 elements go one per thread.
-template <class T>
+// kDefer (FP32): p and x are left to the next operator launch (their
+// coefficients go to st->defer[q]; x and p are not touched and may be NULL),
+// and u is written, not read: the new z goes to u while the operator's launch
+// still reads the old one from the other buffer (seven streams instead of
+// twelve on the step's critical path).
+template <class T, bool kDefer = false>
 __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x, T *__restrict__ r,
                                                     T *__restrict__ u, T *__restrict__ p,
                                                     T *__restrict__ s, const T *__restrict__ w,
@@ -477,6 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
                                                     double *__restrict__ part_g_new,
                                                     double *__restrict__ part_rr_new, int q,
                                                     PcgState *__restrict__ st) {
+  static_assert(!kDefer || sizeof(T) == 4, "the deferred update is FP32");
   if (st->done) return;
   using VT = typename StepVec<T>::type;
   constexpr int V = StepVec<T>::V;
@@ -501,11 +507,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
   for (int k = 0; k < kPre; ++k) {
     const int64_t iv = i0 + k * stride;
     if (iv < nv) {
-      ru[k].v = u4[iv];
-      rp[k].v = p4[iv];
+      if constexpr (!kDefer) {
+        ru[k].v = u4[iv];
+        rp[k].v = p4[iv];
+        rx[k].v = x4[iv];
+      }
       rw[k].v = w4[iv];
       rs[k].v = s4[iv];
-      rx[k].v = x4[iv];
       if constexpr (kStoreR) rv[k].v = r4[iv];
       rd[k].v = d4[iv];
     }
@@ -529,21 +537,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
     return;
   }
   const double alpha = gamma / denom;
+  const T a = T(alpha), bt = T(beta);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->alpha[q] = alpha;
     st->gamma[q] = gamma;
     st->it[q] = iter + 1;
     st->iter = iter + 1;
     st->rr = rr;
+    if constexpr (kDefer) st->defer[q] = CgDeferCoef{a, bt, first ? 1 : 2, 0};
   }
-  const T a = T(alpha), bt = T(beta);
   double sg = 0.0, srr = 0.0;
   // one element: p and s hold no values before iteration 0 (selected away,
   // never scaled); returns the new u
   const auto step1 = [&](T ui, T pi, T wi, T si, T xi, T rvi, T di, T &pn, T &sn, T &xn, T &rn) {
-    pn = first ? ui : ui + bt * pi;
+    if constexpr (!kDefer) {
+      pn = first ? ui : ui + bt * pi;
+      xn = xi + a * pn;
+    }
     sn = first ? wi : wi + bt * si;
-    xn = xi + a * pn;
     double ri;
     T un;
     if constexpr (kStoreR) {
@@ -566,11 +577,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
     Pack pn, sn, xn, rn, un;
 #pragma unroll
     for (int e = 0; e < V; ++e)
-      un.e[e] = step1(U.e[e], P.e[e], W.e[e], S.e[e], X.e[e], kStoreR ? R.e[e] : T(0), D.e[e],
-                      pn.e[e], sn.e[e], xn.e[e], rn.e[e]);
-    __builtin_nontemporal_store(pn.v, reinterpret_cast<VT *>(p) + iv);
+      un.e[e] = step1(kDefer ? T(0) : U.e[e], kDefer ? T(0) : P.e[e], W.e[e], S.e[e],
+                      kDefer ? T(0) : X.e[e], kStoreR ? R.e[e] : T(0), D.e[e], pn.e[e], sn.e[e],
+                      xn.e[e], rn.e[e]);
+    if constexpr (!kDefer) {
+      __builtin_nontemporal_store(pn.v, reinterpret_cast<VT *>(p) + iv);
+      __builtin_nontemporal_store(xn.v, reinterpret_cast<VT *>(x) + iv);
+    }
     __builtin_nontemporal_store(sn.v, reinterpret_cast<VT *>(s) + iv);
-    __builtin_nontemporal_store(xn.v, reinterpret_cast<VT *>(x) + iv);
     if constexpr (kStoreR) __builtin_nontemporal_store(rn.v, reinterpret_cast<VT *>(r) + iv);
     reinterpret_cast<VT *>(u)[iv] = un.v;
   };
@@ -581,11 +595,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
   }
   for (int64_t iv = i0 + kPre * stride; iv < nv; iv += stride) {
     Pack U, P, W, S, X, R, D;
-    U.v = u4[iv];
-    P.v = p4[iv];
+    if constexpr (!kDefer) {
+      U.v = u4[iv];
+      P.v = p4[iv];
+      X.v = x4[iv];
+    }
     W.v = w4[iv];
     S.v = s4[iv];
-    X.v = x4[iv];
     if constexpr (kStoreR) R.v = r4[iv];
     D.v = d4[iv];
     stepv(iv, U, P, W, S, X, R, D);
@@ -594,10 +610,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
   if (i0 < n - nv * V) {
     const int64_t i = nv * V + i0;
     T pn, sn, xn, rn;
-    const T un = step1(u[i], p[i], w[i], s[i], x[i], kStoreR ? r[i] : T(0), dinv[i], pn, sn, xn, rn);
-    p[i] = pn;
+    const T un = step1(kDefer ? T(0) : u[i], kDefer ? T(0) : p[i], w[i], s[i],
+                       kDefer ? T(0) : x[i], kStoreR ? r[i] : T(0), dinv[i], pn, sn, xn, rn);
+    if constexpr (!kDefer) {
+      p[i] = pn;
+      x[i] = xn;
+    }
     s[i] = sn;
-    x[i] = xn;
     if constexpr (kStoreR) r[i] = rn;
     u[i] = un;
   }
@@ -653,6 +672,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_state(const double *__restr
     st->iter = 0;
     st->it[0] = 0;
     st->it[1] = 0;
+    st->defer[0] = CgDeferCoef{0.0f, 0.0f, 0, 0};
+    st->defer[1] = CgDeferCoef{0.0f, 0.0f, 0, 0};
     st->maxiter = maxiter;
     int done = 1, status = TATVA_OK;
     if (bad != TATVA_OK) {
@@ -773,6 +794,7 @@ static std::mutex g_res_init_mu;  // serialises the first use of a device's reso
 static std::atomic<int> g_profiling{0};
 static std::atomic<int> g_lin_cache{0};  // TATVA_OPT_MATFREE_LIN_CACHE: 0 never, 1 Hex8, 2 always
 static std::atomic<int> g_geo_cache{1};  // TATVA_OPT_MATFREE_GEOM_CACHE: 0 never, 1 affine Hex8
+static std::atomic<int> g_cg_defer{1};   // TATVA_OPT_MATFREE_DEFERRED_UPDATE: 0 off, 1 on
 
 // Drops cached solver graphs that captured this mesh's arrays (called before
 // the mesh's device memory is freed).
@@ -1044,6 +1066,10 @@ extern "C" int tatva_set_option(int32_t option, int32_t value) {
       TATVA_REQUIRE(value >= 0 && value <= 1, "geometry cache option: 0 or 1");
       g_geo_cache.store(value);
       return TATVA_OK;
+    case TATVA_OPT_MATFREE_DEFERRED_UPDATE:
+      TATVA_REQUIRE(value >= 0 && value <= 1, "deferred update option: 0 or 1");
+      g_cg_defer.store(value);
+      return TATVA_OK;
     default: set_error("unknown option"); return TATVA_EINVAL;
   }
 }
@@ -1303,10 +1329,11 @@ int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
 }  // namespace tatva
 
 // workspace: reduced FP32 vectors z (= D^-1 r), w, p, s, r, D^-1, x, b, D, the
-// partials and the state, then the full FP32 state u, then the caches the
-// options select for this mesh and density: the density's linearisation at u
-// and the affine Hex8 geometry (each 0 bytes when not selected)
-constexpr int kMatfreeVecs = 9;
+// second z buffer, the partials and the state, then the full FP32 state u,
+// then the caches the options select for this mesh and density: the density's
+// linearisation at u and the affine Hex8 geometry (each 0 bytes when not
+// selected)
+constexpr int kMatfreeVecs = 10;
 
 static size_t matfree_u_offset(int64_t n) { return ws_layout(n, kMatfreeVecs, sizeof(float)).total; }
 
@@ -1357,6 +1384,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   float *x = ws_vec<float>(wsb, L, 6);
   float *b = ws_vec<float>(wsb, L, 7);
   float *diag = ws_vec<float>(wsb, L, 8);
+  float *z2 = ws_vec<float>(wsb, L, 9);
   float *u = reinterpret_cast<float *>(wsb + matfree_u_offset(n));
   // the state is fixed during the solve: the density may be linearised once and
   // read back by every product (option; off by default: on the 160^3 Hex8 cube
@@ -1397,23 +1425,34 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
                          gv, tol, maxiter, st, s, lin, geo);
   if (rc) return rc;
   const int g_op = hvp_gather_blocks(m);
+  // deferred p and x (option, on by default): iteration i's z sits in zb[i % 2];
+  // the step of iteration i writes z_{i+1} to the other buffer and leaves
+  // p_i = z_i + beta p_{i-1}, x_{i+1} = x_i + alpha p_i to the tile kernel of
+  // iteration i + 1, which reads z_i from that buffer before the step of
+  // iteration i + 1 overwrites it.  The solve is the same bits either way.
+  const bool defer = g_cg_defer.load() == 1;
+  float *const zb[2] = {z, defer ? z2 : z};
   // KA: w = H z with the shares of z.w; the tiles run in alternating order
   // (iteration parity), as the CSR SpMV streams its matrix
-  const std::function<int(int)> apply_op = [&](int q) -> int {
-    return launch_hvp_dot_f32(m, d, u, z, dof_map, w, P.pap, s, &st->done, q, nullptr, lin, geo);
+  const auto op = [&](int q, hipEvent_t between) -> int {
+    const CgDefer dx{zb[q ^ 1], p, x, &st->defer[q ^ 1], defer ? n : 0};
+    return launch_hvp_dot_f32(m, d, u, zb[q], dof_map, w, P.pap, s, &st->done, q, between, lin,
+                              geo, &dx);
   };
   // the sampled operator also times its tile kernel alone (prof_a to prof_m)
-  const std::function<int()> sampled_op = [&]() -> int {
-    return launch_hvp_dot_f32(m, d, u, z, dof_map, w, P.pap, s, &st->done, 0, res->prof_m, lin,
-                              geo);
-  };
+  const std::function<int()> sampled_op = [&]() -> int { return op(0, res->prof_m); };
   const std::function<int(int, bool)> step = [&](int q, bool with_op) -> int {
     if (with_op) {
-      const int e = apply_op(q);
+      const int e = op(q, nullptr);
       if (e) return e;
     }
-    k_cg_step<float><<<gv, kBlock, 0, s>>>(n, x, r, z, p, sv, w, dinv, P.pap, g_op, P.rz(q ^ 1),
-                                           P.rr(q ^ 1), P.rz(q), P.rr(q), q, st);
+    if (defer)
+      k_cg_step<float, true><<<gv, kBlock, 0, s>>>(n, nullptr, r, zb[q ^ 1], nullptr, sv, w, dinv,
+                                                   P.pap, g_op, P.rz(q ^ 1), P.rr(q ^ 1), P.rz(q),
+                                                   P.rr(q), q, st);
+    else
+      k_cg_step<float><<<gv, kBlock, 0, s>>>(n, x, r, z, p, sv, w, dinv, P.pap, g_op, P.rz(q ^ 1),
+                                             P.rr(q ^ 1), P.rz(q), P.rr(q), q, st);
     return TATVA_OK;
   };
   GraphKey key{};
@@ -1425,7 +1464,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   key.i0 = d->kind;
   key.i1 = chunk;
   key.i2 = profiling ? 1 : 0;
-  key.i3 = (lin ? 1 : 0) | (geo ? 2 : 0);
+  key.i3 = (lin ? 1 : 0) | (geo ? 2 : 0) | (defer ? 4 : 0);
   key.c = d->elem_params;
   key.f0 = d->p0;
   key.f1 = d->p1;

@@ -176,6 +176,7 @@ def set_profiling(enable: bool) -> None:
 
 OPT_MATFREE_LIN_CACHE = 1
 OPT_MATFREE_GEOM_CACHE = 2   # include/tatva_amd.h TATVA_OPT_MATFREE_GEOM_CACHE
+OPT_MATFREE_DEFERRED_UPDATE = 3   # include/tatva_amd.h TATVA_OPT_MATFREE_DEFERRED_UPDATE
 
 
 _LIN_CACHE_MODES = {"never": 0, "auto": 1, "always": 2}
@@ -199,6 +200,14 @@ def set_geometry_cache(on: bool) -> None:
     coordinates in every product (off).  The same result bit for bit; 44 bytes
     per element of workspace."""
     nat.check(nat.lib().tatva_set_option(OPT_MATFREE_GEOM_CACHE, 1 if on else 0))
+
+
+def set_deferred_update(on: bool) -> None:
+    """Matrix-free FP32 PCG: apply each CG iteration's p and x updates in the
+    tile kernel of the next Hessian-vector product, which has memory bandwidth
+    to spare (on, the default), or in the iteration's own step kernel (off).
+    The same iterates bit for bit."""
+    nat.check(nat.lib().tatva_set_option(OPT_MATFREE_DEFERRED_UPDATE, 1 if on else 0))
 
 
 def dot(x, y):

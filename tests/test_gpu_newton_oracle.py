@@ -251,3 +251,54 @@ def test_geometry_cache_gives_the_same_solve(lin):
     np.testing.assert_array_equal(runs[1][1], runs[0][1])
     assert states[0][0] == states[1][0]
     np.testing.assert_array_equal(states[1][1], states[0][1])
+
+
+@pytest.mark.parametrize("kind,jitter,profiling", [("hex8", 0.0, False), ("hex8", 0.1, True),
+                                                   ("tet4", 0.2, False)])
+def test_deferred_update_gives_the_same_solve(kind, jitter, profiling):
+    """With each CG iteration's p and x updates applied by the next product's
+    tile kernel (tatva_set_option(TATVA_OPT_MATFREE_DEFERRED_UPDATE, 1), the
+    default) instead of by the iteration's step kernel, the matrix-free FP32
+    PCG takes the same iterations to the same x bit for bit: converged, stopped
+    by maxiter at an odd and an even count (the last update then reads either z
+    buffer), with the sampled operator launch of the profiling mode, and through
+    a mixed Newton load step.  n_free % 4 != 0 for the 6^3 cubes, so the
+    one-entry tail of the 16-byte vector loops runs too."""
+    import torch
+
+    C = stretch_case(kind, 6, jitter)
+    E, L, mesh = C["E"], C["L"].at["load"].set(C["values"](0.05)), C["mesh"]
+    nd = mesh.n_nodes * 3
+    rng = np.random.default_rng(13)
+    u_np = 0.002 * rng.standard_normal(nd)
+    u_np[0::3] += 0.05 * mesh.coords[:, 0]
+    u = torch.as_tensor(u_np, dtype=torch.float32, device="cuda")
+    red = E.reduced(L)
+    if kind == "hex8":
+        assert red.size % 4 != 0
+    b = torch.as_tensor(rng.standard_normal(red.size), dtype=torch.float32, device="cuda")
+    x0 = torch.as_tensor(0.1 * rng.standard_normal(red.size), dtype=torch.float32, device="cuda")
+    runs = []
+    solvers.set_profiling(profiling)
+    try:
+        for defer in (False, True):
+            solvers.set_deferred_update(defer)
+            out = []
+            for maxiter, tol, x_start in ((None, 1e-6, None), (7, 1e-12, x0), (8, 1e-12, None)):
+                x, info = solvers.pcg_matfree(red, u, b, x_start, tol=tol, maxiter=maxiter,
+                                              chunk=4)
+                assert info.converged == (maxiter is None), (defer, maxiter, info)
+                out.append((info.iterations, x.cpu().numpy()))
+            uu = torch.zeros(nd, dtype=torch.float64, device="cuda")
+            uu[0::3] = torch.as_tensor(0.05 * mesh.coords[:, 0], device="cuda")
+            st = solvers.Newton(E, L, mode="mixed", tol=1e-8, inner_tol=1e-3, chunk=16).solve(uu)
+            assert st.converged
+            out.append((st.cg_iterations, uu.cpu().numpy()))
+            runs.append(out)
+    finally:
+        solvers.set_deferred_update(True)   # the default
+        solvers.set_profiling(False)
+    assert [r[0] for r in runs[0]][1:3] == [7, 8]
+    for (it0, x0_), (it1, x1_) in zip(runs[0], runs[1]):
+        assert it0 == it1
+        np.testing.assert_array_equal(x1_, x0_)
