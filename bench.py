@@ -410,7 +410,7 @@ def mesh_is_affine(op):
     return bool(op.device_layout().get("hex_affine"))
 
 
-def hvp_tile_bytes(op, dens_ncomp=3, cached_lin=0, geo=False, defer_n=0):
+def hvp_tile_bytes(op, dens_ncomp=3, cached_lin=0, geo=False, defer_n=0, mapped=True):
     """Algorithmic bytes of one FP32 HVP tile-kernel launch (k_tile_assemble,
     matrix-free PCG operator), from the mesh's device layout: per element its
     nne uint16 local node numbers (+ cached linearisation floats per quadrature
@@ -420,16 +420,18 @@ def hvp_tile_bytes(op, dens_ncomp=3, cached_lin=0, geo=False, defer_n=0):
     (int32) and the slot's FP32 partial sums written and the int32 position it
     is written to, plus per tile its color offsets and slot range.  Coordinates
     are counted per slot because tiles sharing a node each load it.  With the
-    CG step's deferred update (defer_n free dofs) the launch also streams z, p
-    and x in and p and x out: 20 bytes per free dof."""
+    CG step's deferred update (defer_n entries of the CG vectors) the launch
+    also streams z, p and x in and p and x out: 20 bytes per entry.  Without a
+    dof map (the PCG on full-length vectors: mapped False) the tangent is read
+    by node and no dof-map entries are loaded."""
     lay = op.device_layout()
     E, S, T = lay["n_elements"], lay["n_slots"], lay["n_tiles"]
     nne = 8 if op.element.kind == 2 else (4 if op.element.kind == 1 else 3)
     d = 3 if nne != 3 else 2
     nc = dens_ncomp
     b = 2 * nne * E + 4 * cached_lin * lay["n_quad"] * E + (44 * E if geo else 0)
-    b += S * (4 + (0 if geo else 8 * d) + 4 * nc * (0 if cached_lin else 1) + (4 + 4) * nc
-              + 4 * nc + 4)
+    b += S * (4 + (0 if geo else 8 * d) + 4 * nc * (0 if cached_lin else 1)
+              + (4 if mapped else 0) * nc + 4 * nc + 4 * nc + 4)
     b += T * (2 * (lay["max_colors"] + 1) + 8)
     b += 20 * defer_n
     return int(b), lay
@@ -572,11 +574,13 @@ def headline(args):
     cached = args.lin_cache != "never"     # the headline mesh is Hex8
     geo = not args.no_geo_cache and mesh_is_affine(op)
     defer = not args.no_defer
+    full_vec = not args.no_full_vectors
+    n_vec = mesh.n_nodes * 3 if full_vec else n_free   # length of the CG vectors
     tile_bytes, lay = hvp_tile_bytes(op, cached_lin=10 if cached else 0, geo=geo,
-                                     defer_n=n_free if defer else 0)
+                                     defer_n=n_vec if defer else 0, mapped=not full_vec)
     hbm_rate = tile_bytes / (tile_us * 1e-6) / 1e9 if tile_us else None
     # the profile's counters are those of the default kernel on this workload
-    default_kernel = n == 160 and not cached and geo and defer
+    default_kernel = n == 160 and not cached and geo and defer and full_vec
     prof = profile_counters(HEADLINE_PROFILE) if default_kernel else {}
     valu = prof.get("valu_wave_instructions_per_launch")
     traffic = prof.get("hbm_bytes_per_launch")
@@ -619,6 +623,7 @@ def headline(args):
             "hbm_frac": hbm_rate / HBM_PEAK_GBS if hbm_rate else None,
             "geometry_cache": geo,
             "deferred_cg_update": defer,
+            "full_length_cg_vectors": full_vec,
             "avg_launch_us": tile_us,
             "operator_us": op_us,
             "samples": op_n,
@@ -763,6 +768,9 @@ def main(argv=None):
     ap.add_argument("--no-defer", action="store_true",
                     help="matrix-free PCG: apply each iteration's p and x updates in its CG step "
                          "instead of in the next product's tile kernel (the default)")
+    ap.add_argument("--no-full-vectors", action="store_true",
+                    help="matrix-free PCG: run the CG on the reduced vectors through the dof map "
+                         "instead of on full-length ones (the default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extras", action="store_true",
@@ -803,6 +811,7 @@ def main(argv=None):
     solvers.set_linearization_cache(args.lin_cache)
     solvers.set_geometry_cache(not args.no_geo_cache)
     solvers.set_deferred_update(not args.no_defer)
+    solvers.set_full_vectors(not args.no_full_vectors)
     if args.workload == "copy":
         # calibration of the HBM traffic counters: a plain 512 MiB device copy
         src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
@@ -835,6 +844,7 @@ def main(argv=None):
         r["lin_cache"] = args.lin_cache
         r["geo_cache"] = not args.no_geo_cache
         r["deferred_cg_update"] = not args.no_defer
+        r["full_length_cg_vectors"] = not args.no_full_vectors
         print(json.dumps({"workload": args.workload, **r}), flush=True)
         return 0
     if args.workload == "c4":

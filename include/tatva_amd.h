@@ -101,11 +101,19 @@ int tatva_set_profiling(int enable);
  * CG iteration's p and x updates to the tile kernel of the next HVP (which has
  * memory bandwidth to spare) instead of its step kernel, with z in two
  * alternating buffers: 1 (default) on, 0 off.  The iterates are the same bits
- * either way. */
+ * either way.
+ * TATVA_OPT_MATFREE_FULL_VECTORS: whether tatva_pcg_matfree_f32 with a dof map
+ * runs its CG on full-length vectors (every dof of the mesh, the inverse
+ * diagonal 0 outside the system), so that no kernel of the iteration reads the
+ * map: b, x and the diagonal go in and x comes out through it once per solve.
+ * 1 (default) on, 0 off.  The products are the same bits; the dot products sum
+ * in another order.  The workspace then holds full-length vectors: set the
+ * option before querying its size. */
 enum {
   TATVA_OPT_MATFREE_LIN_CACHE = 1,
   TATVA_OPT_MATFREE_GEOM_CACHE = 2,
-  TATVA_OPT_MATFREE_DEFERRED_UPDATE = 3
+  TATVA_OPT_MATFREE_DEFERRED_UPDATE = 3,
+  TATVA_OPT_MATFREE_FULL_VECTORS = 4
 };
 int tatva_set_option(int32_t option, int32_t value);
 

@@ -566,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
       ri = (double)un / (double)di;
     }
     sg += ri * (double)un;
-    srr += ri * ri;
+    srr += di != T(0) ? ri * ri : 0.0;  // D^-1 = 0: a dof outside the system
     return un;
   };
   // x, r, p, s are stored non-temporally (read again only by the next step;
@@ -639,11 +639,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T *__restr
   double rz = 0.0, rr = 0.0, bb = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const T ri = r[i];
-    const T z = dinv[i] * ri;
+    const T ri = r[i], di = dinv[i];
+    const T z = di * ri;
     p[i] = z;
     rz += (double)ri * (double)z;
-    rr += (double)ri * (double)ri;
+    // D^-1 = 0 marks a dof outside the system (full-length vectors): its r is
+    // not part of the residual
+    rr += di != T(0) ? (double)ri * (double)ri : 0.0;
     bb += (double)b[i] * (double)b[i];
   }
   rz = block_sum(rz);
@@ -734,6 +736,47 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag_f32(int64_t n, const float 
   }
 }
 
+// Full-length vectors of a reduced system (dof map injective, -1 outside it):
+// out[i] = in[map[i]] or 0; the inverse diagonal with 0 outside the system; the
+// reduced vector back.
+__global__ __launch_bounds__(kBlock) void k_expand_f32(int64_t n, const int32_t *__restrict__ map,
+                                                       const float *__restrict__ in,
+                                                       float *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t r = map[i];
+  out[i] = r >= 0 ? in[r] : 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_inv_diag_map_f32(int64_t n, const int32_t *__restrict__ map,
+                                                             const float *__restrict__ diag,
+                                                             float *__restrict__ dinv,
+                                                             PcgState *__restrict__ st) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t r = map[i];
+  if (r < 0) {
+    dinv[i] = 0.0f;
+    return;
+  }
+  const float d = diag[r];
+  if (d != 0.0f && isfinite(d)) {
+    dinv[i] = 1.0f / d;
+  } else {
+    dinv[i] = 1.0f;
+    st->status = TATVA_EINVAL;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_compress_f32(int64_t n, const int32_t *__restrict__ map,
+                                                         const float *__restrict__ in,
+                                                         float *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t r = map[i];
+  if (r >= 0) out[r] = in[i];
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_sub(int64_t n, const T *__restrict__ b,
                                                 const T *__restrict__ Ax, T *__restrict__ r) {
@@ -795,6 +838,7 @@ static std::atomic<int> g_profiling{0};
 static std::atomic<int> g_lin_cache{0};  // TATVA_OPT_MATFREE_LIN_CACHE: 0 never, 1 Hex8, 2 always
 static std::atomic<int> g_geo_cache{1};  // TATVA_OPT_MATFREE_GEOM_CACHE: 0 never, 1 affine Hex8
 static std::atomic<int> g_cg_defer{1};   // TATVA_OPT_MATFREE_DEFERRED_UPDATE: 0 off, 1 on
+static std::atomic<int> g_full_vectors{1};  // TATVA_OPT_MATFREE_FULL_VECTORS: 0 off, 1 on
 
 // Drops cached solver graphs that captured this mesh's arrays (called before
 // the mesh's device memory is freed).
@@ -1070,6 +1114,10 @@ extern "C" int tatva_set_option(int32_t option, int32_t value) {
       TATVA_REQUIRE(value >= 0 && value <= 1, "deferred update option: 0 or 1");
       g_cg_defer.store(value);
       return TATVA_OK;
+    case TATVA_OPT_MATFREE_FULL_VECTORS:
+      TATVA_REQUIRE(value >= 0 && value <= 1, "full-length vectors option: 0 or 1");
+      g_full_vectors.store(value);
+      return TATVA_OK;
     default: set_error("unknown option"); return TATVA_EINVAL;
   }
 }
@@ -1313,9 +1361,12 @@ int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
                       const float *x, float *r, float *p, float *Ap, float *dinv,
                       double *part_rz1, double *part_rr1, double *part_bb, int gv, double tol,
                       int maxiter, PcgState *st, hipStream_t s, const float *lin,
-                      const float *geo) {
+                      const float *geo, const int32_t *diag_map) {
   k_state_reset<<<1, 64, 0, s>>>(st);
-  k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
+  if (diag_map)
+    k_inv_diag_map_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag_map, diag, dinv, st);
+  else
+    k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
   // r = b - H x
   const int rc = launch_hvp_f32(m, d, u, x, dof_map, Ap, s, nullptr, lin, geo);
   if (rc) return rc;
@@ -1328,12 +1379,18 @@ int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
 
 }  // namespace tatva
 
-// workspace: reduced FP32 vectors z (= D^-1 r), w, p, s, r, D^-1, x, b, D, the
-// second z buffer, the partials and the state, then the full FP32 state u,
-// then the caches the options select for this mesh and density: the density's
-// linearisation at u and the affine Hex8 geometry (each 0 bytes when not
-// selected)
+// workspace: FP32 vectors z (= D^-1 r), w, p, s, r, D^-1, x, b, D, the second z
+// buffer (reduced, or full-length under TATVA_OPT_MATFREE_FULL_VECTORS), the
+// partials and the state, then the full FP32 state u, then the caches the
+// options select for this mesh and density: the density's linearisation at u
+// and the affine Hex8 geometry (each 0 bytes when not selected)
 constexpr int kMatfreeVecs = 10;
+
+// Length of the CG vectors the workspace is sized for: with full-length
+// vectors the larger of n and the full dof count.
+static int64_t matfree_vec_len(const tatva_mesh *m, const tatva_density *d, int64_t n) {
+  return g_full_vectors.load() == 1 ? std::max<int64_t>(n, m->n_nodes * density_ncomp(m, d)) : n;
+}
 
 static size_t matfree_u_offset(int64_t n) { return ws_layout(n, kMatfreeVecs, sizeof(float)).total; }
 
@@ -1353,7 +1410,8 @@ static size_t matfree_geo_bytes(const tatva_mesh *m) {
 
 extern "C" int64_t tatva_pcg_matfree_workspace(tatva_mesh *m, const tatva_density *d, int64_t n) {
   if (!m || !d || n <= 0) return -1;
-  return (int64_t)(matfree_lin_offset(m, n) + matfree_lin_bytes(m, d) + matfree_geo_bytes(m));
+  const int64_t nv = matfree_vec_len(m, d, n);
+  return (int64_t)(matfree_lin_offset(m, nv) + matfree_lin_bytes(m, d) + matfree_geo_bytes(m));
 }
 
 extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, const float *u_in,
@@ -1363,11 +1421,18 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
                                      tatva_pcg_stats *stats, void *stream) {
   TATVA_REQUIRE(m && d && u_in && diag_in && b_in && x_io && n > 0, "invalid arguments");
   TATVA_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
-  const WsLayout L = ws_layout(n, kMatfreeVecs, sizeof(float));
   TATVA_REQUIRE(((uintptr_t)workspace & 255) == 0, "PCG workspace must be 256-byte aligned");
   TATVA_REQUIRE(workspace && ws_bytes >= tatva_pcg_matfree_workspace(m, d, n),
                 "PCG workspace too small");
   const int64_t n_full = m->n_nodes * density_ncomp(m, d);
+  // full-length vectors (option, on by default): with a dof map the CG runs on
+  // every dof of the mesh, D^-1 = 0 outside the system, so that the products
+  // index the vectors by node and no kernel of the iteration reads the map;
+  // b, x and the diagonal go in and x comes out through the map once per solve
+  const bool full = dof_map && g_full_vectors.load() == 1;
+  const int64_t nv = full ? n_full : n;
+  const int32_t *op_map = full ? nullptr : dof_map;
+  const WsLayout L = ws_layout(nv, kMatfreeVecs, sizeof(float));
   chunk += chunk & 1;
   TATVA_TRY_HIP(hipSetDevice(m->device));
   DeviceRes *res = nullptr;
@@ -1385,20 +1450,20 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   float *b = ws_vec<float>(wsb, L, 7);
   float *diag = ws_vec<float>(wsb, L, 8);
   float *z2 = ws_vec<float>(wsb, L, 9);
-  float *u = reinterpret_cast<float *>(wsb + matfree_u_offset(n));
+  float *u = reinterpret_cast<float *>(wsb + matfree_u_offset(nv));
   // the state is fixed during the solve: the density may be linearised once and
   // read back by every product (option; off by default: on the 160^3 Hex8 cube
   // the product reading 40 bytes per point ran at 515 us against 433 us for the
   // one recomputing them, its memory and VALU work not overlapping at 4 waves
   // per SIMD)
   const size_t lin_bytes = matfree_lin_bytes(m, d);
-  float *lin = lin_bytes ? reinterpret_cast<float *>(wsb + matfree_lin_offset(m, n)) : nullptr;
+  float *lin = lin_bytes ? reinterpret_cast<float *>(wsb + matfree_lin_offset(m, nv)) : nullptr;
   // affine Hex8: the elements' J^-1 and det J are written once per solve and
   // read by every product instead of the coordinates (option, on by default: the
   // 64^3 linear elasticity product 22.1 -> 19.9 us, the 160^3 Neo-Hookean one
   // unchanged at 334 us)
   float *geo = matfree_geo_bytes(m)
-                   ? reinterpret_cast<float *>(wsb + matfree_lin_offset(m, n) + lin_bytes)
+                   ? reinterpret_cast<float *>(wsb + matfree_lin_offset(m, nv) + lin_bytes)
                    : nullptr;
   const PcgParts P = parts_at(wsb, L);
   PcgState *st = ws_state(wsb, L);
@@ -1406,11 +1471,18 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   hipStream_t s = res->cap;
   TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
   TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
-  TATVA_TRY_HIP(hipMemcpyAsync(x, x_io, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-  TATVA_TRY_HIP(hipMemcpyAsync(b, b_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  if (full) {
+    k_expand_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, x_io, x);
+    k_expand_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, b_in, b);
+    TATVA_CHECK_LAUNCH();
+  } else {
+    TATVA_TRY_HIP(hipMemcpyAsync(x, x_io, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    TATVA_TRY_HIP(hipMemcpyAsync(b, b_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+  // (full-length: the reduced diagonal is copied as is and read through the map)
   TATVA_TRY_HIP(hipMemcpyAsync(diag, diag_in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
   TATVA_TRY_HIP(hipMemcpyAsync(u, u_in, n_full * sizeof(float), hipMemcpyDeviceToDevice, s));
-  const int gv = step_grid(n, step_prefetch<float>());
+  const int gv = step_grid(nv, step_prefetch<float>());
   const bool profiling = g_profiling.load() != 0;
   // r = b - H x, z = D^-1 r, shares of r.z, r.r, b.b (w is scratch here)
   if (geo) {
@@ -1421,8 +1493,8 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
     rc = launch_linearize_f32(m, d, u, lin, s, geo);
     if (rc) return rc;
   }
-  rc = matfree_pcg_setup(m, d, u, dof_map, n, diag, b, x, r, z, w, dinv, P.rz(1), P.rr(1), P.bb,
-                         gv, tol, maxiter, st, s, lin, geo);
+  rc = matfree_pcg_setup(m, d, u, op_map, nv, diag, b, x, r, z, w, dinv, P.rz(1), P.rr(1), P.bb,
+                         gv, tol, maxiter, st, s, lin, geo, full ? dof_map : nullptr);
   if (rc) return rc;
   const int g_op = hvp_gather_blocks(m);
   // deferred p and x (option, on by default): iteration i's z sits in zb[i % 2];
@@ -1435,8 +1507,8 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   // KA: w = H z with the shares of z.w; the tiles run in alternating order
   // (iteration parity), as the CSR SpMV streams its matrix
   const auto op = [&](int q, hipEvent_t between) -> int {
-    const CgDefer dx{zb[q ^ 1], p, x, &st->defer[q ^ 1], defer ? n : 0};
-    return launch_hvp_dot_f32(m, d, u, zb[q], dof_map, w, P.pap, s, &st->done, q, between, lin,
+    const CgDefer dx{zb[q ^ 1], p, x, &st->defer[q ^ 1], defer ? nv : 0};
+    return launch_hvp_dot_f32(m, d, u, zb[q], op_map, w, P.pap, s, &st->done, q, between, lin,
                               geo, &dx);
   };
   // the sampled operator also times its tile kernel alone (prof_a to prof_m)
@@ -1447,19 +1519,19 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
       if (e) return e;
     }
     if (defer)
-      k_cg_step<float, true><<<gv, kBlock, 0, s>>>(n, nullptr, r, zb[q ^ 1], nullptr, sv, w, dinv,
+      k_cg_step<float, true><<<gv, kBlock, 0, s>>>(nv, nullptr, r, zb[q ^ 1], nullptr, sv, w, dinv,
                                                    P.pap, g_op, P.rz(q ^ 1), P.rr(q ^ 1), P.rz(q),
                                                    P.rr(q), q, st);
     else
-      k_cg_step<float><<<gv, kBlock, 0, s>>>(n, x, r, z, p, sv, w, dinv, P.pap, g_op, P.rz(q ^ 1),
+      k_cg_step<float><<<gv, kBlock, 0, s>>>(nv, x, r, z, p, sv, w, dinv, P.pap, g_op, P.rz(q ^ 1),
                                              P.rr(q ^ 1), P.rz(q), P.rr(q), q, st);
     return TATVA_OK;
   };
   GraphKey key{};
   key.ws = workspace;
   key.a = m;
-  key.b = dof_map;
-  key.n = n;
+  key.b = op_map;
+  key.n = nv;
   key.m = n_full;
   key.i0 = d->kind;
   key.i1 = chunk;
@@ -1473,8 +1545,14 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   if (rc == TATVA_OK)
     rc = run_chunks(res, res->matfree.exec, st, maxiter, chunk, profiling, sampled_op, stats,
                     std::function<int()>(), true);
-  if (rc == TATVA_OK)
-    TATVA_TRY_HIP(hipMemcpyAsync(x_io, x, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  if (rc == TATVA_OK) {
+    if (full) {
+      k_compress_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, x, x_io);
+      TATVA_CHECK_LAUNCH();
+    } else {
+      TATVA_TRY_HIP(hipMemcpyAsync(x_io, x, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+  }
   TATVA_TRY_HIP(hipEventRecord(res->ev_out, s));
   TATVA_TRY_HIP(hipStreamWaitEvent(caller, res->ev_out, 0));
   return rc;

@@ -177,6 +177,7 @@ def set_profiling(enable: bool) -> None:
 OPT_MATFREE_LIN_CACHE = 1
 OPT_MATFREE_GEOM_CACHE = 2   # include/tatva_amd.h TATVA_OPT_MATFREE_GEOM_CACHE
 OPT_MATFREE_DEFERRED_UPDATE = 3   # include/tatva_amd.h TATVA_OPT_MATFREE_DEFERRED_UPDATE
+OPT_MATFREE_FULL_VECTORS = 4   # include/tatva_amd.h TATVA_OPT_MATFREE_FULL_VECTORS
 
 
 _LIN_CACHE_MODES = {"never": 0, "auto": 1, "always": 2}
@@ -208,6 +209,15 @@ def set_deferred_update(on: bool) -> None:
     to spare (on, the default), or in the iteration's own step kernel (off).
     The same iterates bit for bit."""
     nat.check(nat.lib().tatva_set_option(OPT_MATFREE_DEFERRED_UPDATE, 1 if on else 0))
+
+
+def set_full_vectors(on: bool) -> None:
+    """Matrix-free FP32 PCG with Fixed constraints: run the CG on full-length
+    vectors (every dof of the mesh, the inverse diagonal 0 on the constrained
+    ones) so that no kernel of the iteration reads the dof map (on, the
+    default), or on the reduced vectors (off).  The products are the same bits;
+    the dot products sum in another order."""
+    nat.check(nat.lib().tatva_set_option(OPT_MATFREE_FULL_VECTORS, 1 if on else 0))
 
 
 def dot(x, y):

@@ -302,3 +302,55 @@ def test_deferred_update_gives_the_same_solve(kind, jitter, profiling):
     for (it0, x0_), (it1, x1_) in zip(runs[0], runs[1]):
         assert it0 == it1
         np.testing.assert_array_equal(x1_, x0_)
+
+
+@pytest.mark.parametrize("kind,jitter", [("tet4", 0.2), ("hex8", 0.0), ("hex8", 0.2)],
+                         ids=IDS)
+def test_full_vectors_solve_matches_oracle(kind, jitter):
+    """The FP32 matrix-free PCG with Fixed constraints runs its CG on full-length
+    vectors (tatva_set_option(TATVA_OPT_MATFREE_FULL_VECTORS, 1), the default:
+    every dof of the mesh, the inverse diagonal 0 on the constrained ones, b, x
+    and the diagonal scattered in and x gathered out through the dof map) or on
+    the reduced vectors (0).  The products are the same; the dot products sum
+    in another order.  Both solve H(u) x = b from a nonzero start to the true
+    FP64 relative residual <= 1e-4 against the oracle's Hessian (inner tolerance
+    1e-6), take iteration counts within 2 of each other and agree to 1e-3; a
+    solve stopped by maxiter returns x of the same iteration either way (to FP32
+    rounding of the dot order)."""
+    import torch
+
+    C = stretch_case(kind, 6, jitter)
+    mesh, prob = C["mesh"], C["prob"]
+    vals = C["values"](0.05)
+    Lg = C["L"].at["load"].set(vals)
+    Lo = fem.Lifter(mesh.n_nodes * 3, fem.Fixed(C["dofs"], vals))
+    rng = np.random.default_rng(5)
+    u = Lo.lift_from_zeros(0.002 * rng.standard_normal(Lo.size_reduced))
+    u[0::3] += 0.05 * mesh.coords[:, 0]
+    u = Lo.lift(Lo.reduce(u), u)
+    So, colors = oracle_pattern(C)
+    H = fem.hessian_csr(prob, Lo, u, So, colors)
+    b = rng.standard_normal(Lo.size_reduced)
+    x0 = 0.01 * rng.standard_normal(Lo.size_reduced)
+    red = C["E"].reduced(Lg)
+    u32 = torch.as_tensor(u, dtype=torch.float32, device="cuda")
+    b32 = torch.as_tensor(b, dtype=torch.float32, device="cuda")
+    x032 = torch.as_tensor(x0, dtype=torch.float32, device="cuda")
+    out, short = [], []
+    try:
+        for full in (True, False):
+            solvers.set_full_vectors(full)
+            x32, info = solvers.pcg_matfree(red, u32, b32, x032, tol=1e-6, chunk=16)
+            assert info.converged, (full, info)
+            x = x32.double().cpu().numpy()
+            res = np.linalg.norm(H @ x - b) / np.linalg.norm(b)
+            assert res <= 1e-4, (full, res)
+            out.append((info.iterations, x))
+            xs, infos = solvers.pcg_matfree(red, u32, b32, x032, tol=1e-12, maxiter=5, chunk=4)
+            assert infos.iterations == 5 and not infos.converged
+            short.append(xs.double().cpu().numpy())
+    finally:
+        solvers.set_full_vectors(True)   # the default
+    assert abs(out[0][0] - out[1][0]) <= 2, (out[0][0], out[1][0])
+    assert np.max(np.abs(out[0][1] - out[1][1])) <= 1e-3 * np.max(np.abs(out[1][1]))
+    assert np.max(np.abs(short[0] - short[1])) <= 1e-4 * np.max(np.abs(short[1]))
