@@ -486,7 +486,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, T *__restrict__ x
   if (st->done) return;
   using VT = typename StepVec<T>::type;
   constexpr int V = StepVec<T>::V;
-  constexpr int kPre = step_prefetch_vecs<T>();
+  // (the deferred form loads four streams instead of seven: twice the vectors
+  // in flight before the sums, 447.9 -> 445.7 us per 160^3 iteration)
+  constexpr int kPre = kDefer ? 2 * step_prefetch_vecs<T>() : step_prefetch_vecs<T>();
   constexpr bool kStoreR = sizeof(T) == 4;
   union Pack {
     VT v;
