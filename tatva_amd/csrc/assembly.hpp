@@ -621,9 +621,13 @@ constexpr size_t tile_stage_bytes(int max_local, int D, int NC, size_t tsize, in
   return (size_t)max_local *
          (D * sizeof(double) + ((mode == kModeHvp || mode == kModeHvpLin) ? 3 : 2) * NC * tsize);
 }
+// Values between two threads' element vectors in the incidence-list sum: nne*NC
+// rounded up to an odd count, so that the threads of a wave writing their
+// vectors hit distinct LDS banks (with 12 the lanes t and t + 8 share one).
+constexpr int tile_elem_stride(int nne, int NC) { return nne * NC + ((nne * NC) % 2 == 0 ? 1 : 0); }
 constexpr size_t tile_union_bytes(int max_local, int D, int nne, int NC, size_t tsize, int mode) {
   const size_t staged = tile_stage_bytes(max_local, D, NC, tsize, mode);
-  const size_t elem = (size_t)kTile * nne * NC * tsize;
+  const size_t elem = (size_t)kTile * tile_elem_stride(nne, NC) * tsize;
   return align16(elem > staged ? elem : staged);
 }
 // the color sum's copy of the tile's color offsets, after the staged nodes
@@ -802,10 +806,11 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     uint16_t *il = reinterpret_cast<uint16_t *>(
         lds + tile_union_bytes(ts.max_local, D, NNE, NC, sizeof(T), MODE));
     uint16_t *ie = il + kTile * NNE;
+    constexpr int kYS = tile_elem_stride(NNE, NC);
 #pragma unroll
     for (int a = 0; a < NNE; ++a)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) ys[((int)threadIdx.x * NNE + a) * NC + c] = ye[a][c];
+      for (int c = 0; c < NC; ++c) ys[(int)threadIdx.x * kYS + a * NC + c] = ye[a][c];
     reinterpret_cast<uint64_t *>(il)[threadIdx.x] = inc_pk;
     if ((int)threadIdx.x < nloc) ie[threadIdx.x] = (uint16_t)inc_e;
     for (int j = threadIdx.x + kTile; j < nloc; j += kTile) ie[j] = ts.inc_end[nb + j];
@@ -815,7 +820,10 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
       const int q0 = j > 0 ? (int)ie[j - 1] : 0;
       const int q1 = ie[j];
       T sum = T(0);
-      for (int q = q0; q < q1; ++q) sum += ys[(int)il[q] * NC + c];
+      for (int q = q0; q < q1; ++q) {
+        const int e = il[q];  // entry slot * NNE + a
+        sum += ys[(e / NNE) * kYS + (e % NNE) * NC + c];
+      }
       partial[(int64_t)ts.slot_pos[nb + j] * NC + c] = sum;
     }
     return;
