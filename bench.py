@@ -794,7 +794,7 @@ def main(argv=None):
     ap.add_argument("--serve-budget-gb", type=float, default=0.4,
                     help="HBM budget of the extra serving run at the highest QPS (0: skip)")
     ap.add_argument("--serve-chunk", type=int, default=8, help="batched PCG iterations per poll")
-    ap.add_argument("--serve-prefetch", type=int, default=3,
+    ap.add_argument("--serve-prefetch", type=int, default=2,
                     help="serving: threads preparing arrived requests")
     args = ap.parse_args(argv)
     if args.gpus != 1:

@@ -236,11 +236,10 @@ class Energy:
                                          nat.dptr(self.f_ext), nat.dptr(out), nat.stream()))
         return out
 
-    def residual(self, u, *, dof_map=None, n_out=None, out=None):
+    def residual(self, u, *, dof_map=None, n_out=None):
         t = nat.torch()
         x = self._vec(u)
-        r = out if out is not None else t.empty((n_out or self.size,), dtype=t.float64,
-                                                 device=x.device)
+        r = t.empty((n_out or self.size,), dtype=t.float64, device=x.device)
         nat.check(nat.lib().tatva_residual(self.op.handle, ctypes.byref(self._desc), nat.dptr(x),
                                            nat.dptr(self.f_ext), nat.dptr(dof_map), nat.dptr(r),
                                            nat.stream()))
@@ -263,12 +262,11 @@ class Energy:
                      nat.dptr(dof_map), nat.dptr(out), nat.stream()))
         return out
 
-    def hessian_diag(self, u, *, dof_map=None, n_out=None, dtype=None, out=None):
+    def hessian_diag(self, u, *, dof_map=None, n_out=None, dtype=None):
         t = nat.torch()
         dt = dtype or t.float64
         x = self._vec(u, dt)
-        if out is None:
-            out = t.empty((n_out or self.size,), dtype=dt, device=x.device)
+        out = t.empty((n_out or self.size,), dtype=dt, device=x.device)
         fn = nat.lib().tatva_hessian_diag_f32 if dt == t.float32 else nat.lib().tatva_hessian_diag
         nat.check(fn(self.op.handle, ctypes.byref(self._desc), nat.dptr(x), nat.dptr(dof_map),
                      nat.dptr(out), nat.stream()))
@@ -310,19 +308,14 @@ class ReducedEnergy:
     def residual(self, u_red):
         return self.residual_full_state(self.lift(u_red))
 
-    def residual_full_state(self, u_full, out=None):
+    def residual_full_state(self, u_full):
         """Reduced residual at an already lifted state: the kernels write the free
-        rows directly (into ``out`` when given); with Periodic constraints the full
-        residual goes through reduce_adjoint (a slave's row is added to its
-        master's)."""
+        rows directly; with Periodic constraints the full residual goes through
+        reduce_adjoint (a slave's row is added to its master's)."""
         lf = self.lifter
         if lf.has_periodic:
-            r = lf.reduce_adjoint(self.energy.residual(u_full))
-            if out is None:
-                return r
-            out.copy_(r)
-            return out
-        return self.energy.residual(u_full, dof_map=lf.dof_map_device(), n_out=self.size, out=out)
+            return lf.reduce_adjoint(self.energy.residual(u_full))
+        return self.energy.residual(u_full, dof_map=lf.dof_map_device(), n_out=self.size)
 
     def hvp(self, u_red, v_red):
         return self.hvp_full_state(self.lift(u_red), v_red)

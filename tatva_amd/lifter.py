@@ -281,25 +281,6 @@ class Lifter:
         """(ptr, list): the full dofs summed into each reduced index by reduce_adjoint."""
         return self._layout.rows_dev()
 
-    def upload(self) -> "Lifter":
-        """Make the device copies of the index tables now (dof map, free dofs,
-        the constraints' dofs) instead of at their first use: a copy from host
-        memory waits for the stream it is queued on, which a caller running a
-        pipeline of GPU work (the serving loop) does ahead of time, on another
-        thread.  Lifters that differ only in runtime values share the tables."""
-        self._require_device_kernels()
-        lay = self._layout
-        lay.map_dev()
-        lay.free_dev()
-        if lay.has_periodic:
-            lay.read_dev()
-        for c in self.constraints:
-            if isinstance(c, Fixed):
-                c._dev_dofs()
-            else:
-                c._dev()
-        return self
-
     def _require_device_kernels(self):
         for c in self.constraints:
             if not isinstance(c, (Fixed, Periodic)):

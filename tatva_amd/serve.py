@@ -209,9 +209,7 @@ class _Job:
         self.server = server
         self.prob = CubeProblem(req)
         self.op, self.energy = self.prob.op, self.prob.energy
-        # the constraint tables go to the device here, on the preparation
-        # thread: a host-to-device copy waits for its stream
-        self.lifter = self.prob.lifter(1).upload()
+        self.lifter = self.prob.lifter(1)
         self.red = self.energy.reduced(self.lifter)
         self.n = self.red.size
         dev = nat.device()
@@ -221,12 +219,6 @@ class _Job:
         self.b32 = t.empty(self.n, dtype=t.float32, device=dev)
         self.x32 = t.empty(self.n, dtype=t.float32, device=dev)
         L = nat.lib()
-        # buffers the Newton steps reuse: residual, its dot and the dot's shares,
-        # the FP32 Jacobi diagonal
-        self._rbuf = t.empty(self.n, dtype=t.float64, device=dev)
-        self._rr_dev = t.empty((), dtype=t.float64, device=dev)
-        self._dot_ws = t.empty(int(L.tatva_dot_workspace(self.n)), dtype=t.uint8, device=dev)
-        self._diag = t.empty(self.n, dtype=t.float32, device=dev)
         nbytes = L.tatva_batch_slot_workspace(self.op.handle, 3, self.n)
         if nbytes < 0:
             nat.check(nat.EINVAL)
@@ -240,7 +232,7 @@ class _Job:
         self.done = False
         self.solving = False
         self._r = None
-        self._ev = t.cuda.Event()  # recorded after each residual norm's copy
+        self._ev = None
         self._rr_host = t.empty(1, dtype=t.float64, pin_memory=True)  # r . r
 
     def attach(self, slot: int, admitted: float) -> None:
@@ -272,9 +264,10 @@ class _Job:
     def _start_residual(self):
         from .solvers import dot  # noqa: PLC0415
 
-        self._r = self.red.residual_full_state(self.u, out=self._rbuf)
-        dot(self._r, self._r, out=self._rr_dev, workspace=self._dot_ws)
-        self._rr_host.copy_(self._rr_dev.reshape(1), non_blocking=True)
+        t = nat.torch()
+        self._r = self.red.residual_full_state(self.u)
+        self._rr_host.copy_(dot(self._r, self._r).reshape(1), non_blocking=True)
+        self._ev = t.cuda.Event()
         self._ev.record()
         self.solving = False
 
@@ -312,15 +305,16 @@ class _Job:
         self._r = None
         dmap = self.lifter.dof_map_device()
         diag = self.energy.hessian_diag(self.u32, dof_map=dmap, n_out=self.n,
-                                        dtype=nat.torch().float32, out=self._diag)
+                                        dtype=nat.torch().float32)
+        self._diag = diag
         self.x32.zero_()
         t = nat.torch()
         cur = t.cuda.current_stream()
         bs = cur if batch_stream is None else batch_stream
         if bs != cur:
-            # the solve's set-up reads what this stream wrote (u32, b32, diag, x32:
-            # the job's own buffers, freed only after the device has drained)
+            # the solve's set-up reads what this stream wrote (and diag, made here)
             bs.wait_stream(cur)
+            diag.record_stream(bs)
         nat.check(L.tatva_batch_start(srv._batch, self.slot, self.op.handle,
                                       ctypes.byref(self.energy._desc), nat.dptr(self.u32),
                                       nat.dptr(dmap), self.n, nat.dptr(diag), nat.dptr(self.b32),
@@ -354,15 +348,10 @@ class _Job:
         return False
 
     def release(self):
-        """Frees the mesh and then the job's buffers.  The mesh's destruction
-        waits for the device, so the batch launches queued before the job was
-        retired (which still list its slot, as done) have run before its
-        buffers can be reused; the server calls this on a thread of its own."""
         if self.server.keep_solutions:
             self.result.u = self.u
         self.op.close()
         self.ws = self.u32 = self.b32 = self.x32 = self._diag = self._r = None
-        self._rbuf = self._rr_dev = self._dot_ws = None
 
 
 @dataclass
@@ -460,11 +449,6 @@ class Server:
         self._local = threading.local()
         _warm_host()
 
-    def _release_on_device(self, job) -> None:
-        t = nat.torch()
-        with t.cuda.device(self._device):
-            job.release()
-
     def _prepare(self, req: SolveRequest):
         """Worker thread: the request's _Job, built on the thread's own stream,
         and an event the serving stream waits for before using it.  (A new
@@ -513,10 +497,6 @@ class Server:
         peak_bytes = 0
         dof_iters = 0
         pool = ThreadPoolExecutor(max_workers=self.prefetch, thread_name_prefix="tatva-prepare")
-        # retired jobs are released here: the mesh's destruction waits for the
-        # device, which the serving loop must not do
-        releaser = ThreadPoolExecutor(max_workers=1, thread_name_prefix="tatva-release")
-        self._releaser = releaser
         serving = t.cuda.current_stream()
         # Newton steps of finished solves run here while the next batch launch
         # runs on the serving stream; their next solves start on the serving
@@ -598,8 +578,6 @@ class Server:
                                 self._retire(job, jobs, free_slots, results, t0)
         finally:
             pool.shutdown(wait=True)
-            releaser.shutdown(wait=True)
-            self._releaser = None
         t.cuda.synchronize()
         seconds = time.perf_counter() - t0
         for r in rejected:
@@ -614,11 +592,7 @@ class Server:
     def _retire(self, job, jobs, free_slots, results, t0):
         nat.check(nat.lib().tatva_batch_release(self._batch, job.slot))
         job.result.finished = time.perf_counter() - t0
-        releaser = getattr(self, "_releaser", None)
-        if releaser is not None:
-            releaser.submit(self._release_on_device, job)
-        else:
-            job.release()
+        job.release()
         results.append(job.result)
         del jobs[job.slot]
         free_slots.append(job.slot)

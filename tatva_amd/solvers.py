@@ -220,19 +220,14 @@ def set_full_vectors(on: bool) -> None:
     nat.check(nat.lib().tatva_set_option(OPT_MATFREE_FULL_VECTORS, 1 if on else 0))
 
 
-def dot(x, y, *, out=None, workspace=None):
+def dot(x, y):
     """x . y (FP64 device scalar) with the deterministic reduction kernel; its
-    partial sums go to ``workspace`` (uint8, at least tatva_dot_workspace(n)
-    bytes) or to one allocated on the current stream.  ``out``: a 0-d FP64
-    tensor to write the result to."""
+    partial sums go to a workspace allocated on the current stream."""
     t = nat.torch()
     L = nat.lib()
     n = x.numel()
-    if out is None:
-        out = t.empty((), dtype=t.float64, device=x.device)
-    ws = workspace
-    if ws is None or ws.numel() < L.tatva_dot_workspace(n):
-        ws = t.empty(int(L.tatva_dot_workspace(n)), dtype=t.uint8, device=x.device)
+    out = t.empty((), dtype=t.float64, device=x.device)
+    ws = t.empty(int(L.tatva_dot_workspace(n)), dtype=t.uint8, device=x.device)
     nat.check(L.tatva_dot(n, nat.dptr(x), nat.dptr(y), nat.dptr(out), nat.dptr(ws), ws.numel(),
                           nat.stream()))
     return out
