@@ -313,11 +313,13 @@ int tatva_bicgstab_csr(int64_t n, int64_t nnz, const int32_t *indptr, const int3
                        tatva_pcg_stats *stats, void *stream);
 
 /* Jacobi-PCG with the matrix-free FP32 Hessian-vector product of (mesh, density)
- * at the FP32 state u (full vector); b, x, diag are reduced FP32 vectors.  The
- * workspace holds the CG vectors, a copy of u and the caches the options
- * select: for a Neo-Hookean density its linearisation at u
- * (TATVA_OPT_MATFREE_LIN_CACHE), for an affine Hex8 mesh its element geometry
- * (TATVA_OPT_MATFREE_GEOM_CACHE). */
+ * at the FP32 state u (full vector); b, x, diag are reduced FP32 vectors (n
+ * entries, dof_map: full dof -> reduced index, -1 on a constrained dof, each
+ * reduced index read by one full dof).  The workspace holds the CG vectors
+ * (full-length under TATVA_OPT_MATFREE_FULL_VECTORS, the default), a copy of u
+ * and the caches the options select: for a Neo-Hookean density its
+ * linearisation at u (TATVA_OPT_MATFREE_LIN_CACHE), for an affine Hex8 mesh its
+ * element geometry (TATVA_OPT_MATFREE_GEOM_CACHE). */
 int64_t tatva_pcg_matfree_workspace(tatva_mesh *mesh, const tatva_density *density, int64_t n);
 int tatva_pcg_matfree_f32(tatva_mesh *mesh, const tatva_density *density, const float *u,
                           const int32_t *dof_map, int64_t n, const float *diag, const float *b,
