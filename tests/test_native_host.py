@@ -397,3 +397,33 @@ def test_pcg_workspace_must_be_aligned():
     assert rc == nat.EINVAL
     with pytest.raises(ValueError, match="256-byte aligned"):
         nat.check(rc)
+
+
+def test_solver_options_take_their_documented_values_only():
+    """tatva_set_option (include/tatva_amd.h) accepts each option's documented
+    values and rejects others, and an unknown option, with TATVA_EINVAL before
+    anything runs; the Python setters mirror it and leave the defaults as they
+    found them."""
+    from tatva_amd import solvers
+
+    L = nat.lib()
+    valid = {solvers.OPT_MATFREE_LIN_CACHE: (0, 1, 2), solvers.OPT_MATFREE_GEOM_CACHE: (0, 1),
+             solvers.OPT_MATFREE_DEFERRED_UPDATE: (0, 1), solvers.OPT_MATFREE_FULL_VECTORS: (0, 1)}
+    defaults = {solvers.OPT_MATFREE_LIN_CACHE: 0, solvers.OPT_MATFREE_GEOM_CACHE: 1,
+                solvers.OPT_MATFREE_DEFERRED_UPDATE: 1, solvers.OPT_MATFREE_FULL_VECTORS: 1}
+    try:
+        for opt, values in valid.items():
+            for v in values:
+                assert L.tatva_set_option(opt, v) == nat.OK, (opt, v)
+            for v in (-1, max(values) + 1):
+                assert L.tatva_set_option(opt, v) == nat.EINVAL, (opt, v)
+        assert L.tatva_set_option(99, 0) == nat.EINVAL
+        with pytest.raises(ValueError, match="unknown option"):
+            nat.check(L.tatva_set_option(99, 0))
+        with pytest.raises(ValueError):
+            solvers.set_linearization_cache("sometimes")
+        solvers.set_deferred_update(False)
+        solvers.set_full_vectors(False)
+    finally:
+        for opt, v in defaults.items():
+            assert L.tatva_set_option(opt, v) == nat.OK
