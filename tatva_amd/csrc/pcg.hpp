@@ -62,13 +62,15 @@ inline int step_grid(int64_t n, int per_thread) {
 // b == 0 or r already small).  lin: the density's cached linearisation at u
 // (launch_linearize_f32), or NULL to linearise in the product; geo: an affine
 // Hex8 mesh's geometry cache (launch_geometry_cache_f32), or NULL.  diag_map:
-// the vectors are full-length and diag is the reduced diagonal read through
-// this dof map (dinv = 0 outside the system), or NULL: diag is n long.
+// the vectors are full-length and diag is the reduced diagonal (n_red entries)
+// read through this dof map (dinv = 0 outside the system), or NULL: diag is n
+// long.
 int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
                       const int32_t *dof_map, int64_t n, const float *diag, const float *b,
                       const float *x, float *r, float *p, float *Ap, float *dinv,
                       double *part_rz1, double *part_rr1, double *part_bb, int gv, double tol,
                       int maxiter, PcgState *st, hipStream_t s, const float *lin = nullptr,
-                      const float *geo = nullptr, const int32_t *diag_map = nullptr);
+                      const float *geo = nullptr, const int32_t *diag_map = nullptr,
+                      int64_t n_red = 0);
 
 }  // namespace tatva

@@ -738,26 +738,29 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag_f32(int64_t n, const float 
   }
 }
 
-// Full-length vectors of a reduced system (dof map injective, -1 outside it):
-// out[i] = in[map[i]] or 0; the inverse diagonal with 0 outside the system; the
-// reduced vector back.
+// Full-length vectors of a reduced system of n_red entries (dof map injective,
+// -1 outside the system; an index outside [0, n_red) is treated as -1, so a
+// malformed map cannot reach past the caller's vectors): out[i] = in[map[i]] or
+// 0; the inverse diagonal with 0 outside the system; the reduced vector back.
 __global__ __launch_bounds__(kBlock) void k_expand_f32(int64_t n, const int32_t *__restrict__ map,
+                                                       int64_t n_red,
                                                        const float *__restrict__ in,
                                                        float *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t r = map[i];
-  out[i] = r >= 0 ? in[r] : 0.0f;
+  out[i] = (r >= 0 && r < n_red) ? in[r] : 0.0f;
 }
 
 __global__ __launch_bounds__(kBlock) void k_inv_diag_map_f32(int64_t n, const int32_t *__restrict__ map,
+                                                             int64_t n_red,
                                                              const float *__restrict__ diag,
                                                              float *__restrict__ dinv,
                                                              PcgState *__restrict__ st) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t r = map[i];
-  if (r < 0) {
+  if (r < 0 || r >= n_red) {
     dinv[i] = 0.0f;
     return;
   }
@@ -771,12 +774,13 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag_map_f32(int64_t n, const in
 }
 
 __global__ __launch_bounds__(kBlock) void k_compress_f32(int64_t n, const int32_t *__restrict__ map,
+                                                         int64_t n_red,
                                                          const float *__restrict__ in,
                                                          float *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t r = map[i];
-  if (r >= 0) out[r] = in[i];
+  if (r >= 0 && r < n_red) out[r] = in[i];
 }
 
 template <class T>
@@ -1363,10 +1367,10 @@ int matfree_pcg_setup(tatva_mesh *m, const tatva_density *d, const float *u,
                       const float *x, float *r, float *p, float *Ap, float *dinv,
                       double *part_rz1, double *part_rr1, double *part_bb, int gv, double tol,
                       int maxiter, PcgState *st, hipStream_t s, const float *lin,
-                      const float *geo, const int32_t *diag_map) {
+                      const float *geo, const int32_t *diag_map, int64_t n_red) {
   k_state_reset<<<1, 64, 0, s>>>(st);
   if (diag_map)
-    k_inv_diag_map_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag_map, diag, dinv, st);
+    k_inv_diag_map_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag_map, n_red, diag, dinv, st);
   else
     k_inv_diag_f32<<<grid_for(n), kBlock, 0, s>>>(n, diag, dinv, st);
   // r = b - H x
@@ -1474,8 +1478,8 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
   TATVA_TRY_HIP(hipEventRecord(res->ev_in, caller));
   TATVA_TRY_HIP(hipStreamWaitEvent(s, res->ev_in, 0));
   if (full) {
-    k_expand_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, x_io, x);
-    k_expand_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, b_in, b);
+    k_expand_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, n, x_io, x);
+    k_expand_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, n, b_in, b);
     TATVA_CHECK_LAUNCH();
   } else {
     TATVA_TRY_HIP(hipMemcpyAsync(x, x_io, n * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -1496,7 +1500,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
     if (rc) return rc;
   }
   rc = matfree_pcg_setup(m, d, u, op_map, nv, diag, b, x, r, z, w, dinv, P.rz(1), P.rr(1), P.bb,
-                         gv, tol, maxiter, st, s, lin, geo, full ? dof_map : nullptr);
+                         gv, tol, maxiter, st, s, lin, geo, full ? dof_map : nullptr, n);
   if (rc) return rc;
   const int g_op = hvp_gather_blocks(m);
   // deferred p and x (option, on by default): iteration i's z sits in zb[i % 2];
@@ -1549,7 +1553,7 @@ extern "C" int tatva_pcg_matfree_f32(tatva_mesh *m, const tatva_density *d, cons
                     std::function<int()>(), true);
   if (rc == TATVA_OK) {
     if (full) {
-      k_compress_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, x, x_io);
+      k_compress_f32<<<grid_for(nv), kBlock, 0, s>>>(nv, dof_map, n, x, x_io);
       TATVA_CHECK_LAUNCH();
     } else {
       TATVA_TRY_HIP(hipMemcpyAsync(x_io, x, n * sizeof(float), hipMemcpyDeviceToDevice, s));
