@@ -815,16 +815,23 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     if ((int)threadIdx.x < nloc) ie[threadIdx.x] = (uint16_t)inc_e;
     for (int j = threadIdx.x + kTile; j < nloc; j += kTile) ie[j] = ts.inc_end[nb + j];
     __syncthreads();
-    for (int i = threadIdx.x; i < nloc * NC; i += kTile) {
-      const int j = i / NC, c = i - j * NC;
+    // one local node per thread, its components together: one read of each
+    // incidence entry serves all NC sums (each still in ascending entry order)
+    for (int j = threadIdx.x; j < nloc; j += kTile) {
       const int q0 = j > 0 ? (int)ie[j - 1] : 0;
       const int q1 = ie[j];
-      T sum = T(0);
+      T sum[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sum[c] = T(0);
       for (int q = q0; q < q1; ++q) {
         const int e = il[q];  // entry slot * NNE + a
-        sum += ys[(e / NNE) * kYS + (e % NNE) * NC + c];
+        const int base = (e / NNE) * kYS + (e % NNE) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sum[c] += ys[base + c];
       }
-      partial[(int64_t)ts.slot_pos[nb + j] * NC + c] = sum;
+      const int64_t pos = (int64_t)ts.slot_pos[nb + j] * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) partial[pos + c] = sum[c];
     }
     return;
   }
