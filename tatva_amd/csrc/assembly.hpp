@@ -158,10 +158,16 @@ __device__ __forceinline__ void cg_defer_stream(const CgDefer &dx, int64_t sb, i
   }
 }
 
-// Per-tile sum by the incidence lists instead of element colors.
+// Per-tile sum by the incidence lists instead of element colors: Tet4 (~30
+// colors per tile) and Hex8 (8-15; 160^3 Neo-Hookean HVP 335.7 -> 301.7 us
+// against the per-color sums, whose barriers and range tests it drops).
+constexpr bool kHex8Incidence = true;
+constexpr bool tile_gather_kind(int kind) {
+  return kind == TATVA_TET4 || (kind == TATVA_HEX8 && kHex8Incidence);
+}
 template <int KIND>
 constexpr bool tile_gather() {
-  return KIND == TATVA_TET4;
+  return tile_gather_kind(KIND);
 }
 
 // ------------------------------------------------------ element kernels ----
@@ -634,7 +640,7 @@ constexpr size_t tile_union_bytes(int max_local, int D, int nne, int NC, size_t 
 constexpr size_t tile_offs_bytes() { return align16(sizeof(int16_t) * (kMaxTileColors + 1)); }
 constexpr size_t tile_lds_bytes(int kind, int max_local, int D, int nne, int NC, size_t tsize,
                                 int mode) {
-  if (kind != TATVA_TET4) return align16(tile_stage_bytes(max_local, D, NC, tsize, mode)) + tile_offs_bytes();
+  if (!tile_gather_kind(kind)) return align16(tile_stage_bytes(max_local, D, NC, tsize, mode)) + tile_offs_bytes();
   return tile_union_bytes(max_local, D, nne, NC, tsize, mode) + (size_t)kTile * nne * sizeof(uint16_t) +
          align16((size_t)max_local * sizeof(uint16_t));
 }
@@ -730,12 +736,14 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
   if (kUsesV && dof_map) stage(std::true_type{});
   else stage(std::false_type{});
   // Tet4: the tile's incidence lists are loaded here and stored to LDS after
-  // the element work, so their latency overlaps it
-  uint64_t inc_pk = 0;
+  // the element work, so their latency overlaps it (Hex8: loaded after the
+  // element work, whose registers are all in use)
+  using IncVec = std::conditional_t<NNE == 8, uint4, uint2>;  // NNE uint16 entries
+  constexpr bool kIncEarly = NNE == 4;
+  IncVec inc_pk{};
   uint32_t inc_e = 0;
-  if constexpr (tile_gather<KIND>()) {
-    static_assert(NNE == 4, "four uint16 incidence entries per thread");
-    inc_pk = reinterpret_cast<const uint64_t *>(ts.inc_list + (int64_t)tile * kTile * NNE)[threadIdx.x];
+  if constexpr (tile_gather<KIND>() && kIncEarly) {
+    inc_pk = reinterpret_cast<const IncVec *>(ts.inc_list + (int64_t)tile * kTile * NNE)[threadIdx.x];
     if ((int)threadIdx.x < nloc) inc_e = ts.inc_end[nb + threadIdx.x];
   }
   const int64_t slot = (int64_t)tile * kTile + threadIdx.x;
@@ -801,6 +809,10 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
   }
   if constexpr (MODE == kModeLinearize) return;  // the cache is the output
   if constexpr (tile_gather<KIND>()) {
+    if constexpr (!kIncEarly) {
+      inc_pk = reinterpret_cast<const IncVec *>(ts.inc_list + (int64_t)tile * kTile * NNE)[threadIdx.x];
+      if ((int)threadIdx.x < nloc) inc_e = ts.inc_end[nb + threadIdx.x];
+    }
     __syncthreads();  // every element has read its nodes: the LDS is free
     T *ys = reinterpret_cast<T *>(lds);
     uint16_t *il = reinterpret_cast<uint16_t *>(
@@ -811,7 +823,7 @@ __device__ __forceinline__ void tile_assemble(int tile, const double *__restrict
     for (int a = 0; a < NNE; ++a)
 #pragma unroll
       for (int c = 0; c < NC; ++c) ys[(int)threadIdx.x * kYS + a * NC + c] = ye[a][c];
-    reinterpret_cast<uint64_t *>(il)[threadIdx.x] = inc_pk;
+    reinterpret_cast<IncVec *>(il)[threadIdx.x] = inc_pk;
     if ((int)threadIdx.x < nloc) ie[threadIdx.x] = (uint16_t)inc_e;
     for (int j = threadIdx.x + kTile; j < nloc; j += kTile) ie[j] = ts.inc_end[nb + j];
     __syncthreads();

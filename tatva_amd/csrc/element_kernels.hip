@@ -1002,7 +1002,8 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
   };
   HostTiles T;
   {
-    const int rc = build_tiles(coords, dim, n_nodes, elements, n_elements, nne, elem_kind == TATVA_TET4, T);
+    const int rc = build_tiles(coords, dim, n_nodes, elements, n_elements, nne,
+                               tile_gather_kind(elem_kind), T);
     if (rc) {
       delete m;
       return rc;
@@ -1041,7 +1042,7 @@ extern "C" int tatva_mesh_create(int device, int elem_kind, const double *coords
   if ((err = upload(&m->t_slot_pos, T.slot_pos, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_tnodes, T.tnodes, "tiles")) != hipSuccess) return fail(err, "tiles");
   if ((err = upload(&m->t_elem, T.elem, "tiles")) != hipSuccess) return fail(err, "tiles");
-  if (elem_kind == TATVA_TET4) {
+  if (tile_gather_kind(elem_kind)) {
     if ((err = upload(&m->t_inc_end, T.inc_end, "tiles")) != hipSuccess) return fail(err, "tiles");
     if ((err = upload(&m->t_inc_list, T.inc_list, "tiles")) != hipSuccess) return fail(err, "tiles");
   }
