@@ -36,9 +36,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# VALU issue peak of the part: one wave64 FP32/FP64 VALU instruction per SIMD every
-# four cycles, 1,024 SIMDs (256 CUs x 4) at 2.4 GHz, in G wave-instructions/s
-VALU_PEAK_GINST = 1024 * 2.4 / 4
+# VALU issue peak of the part, in G wave64 instructions/s: one wave64 instruction per
+# SIMD every two cycles on 1,024 SIMDs (256 CUs x 4) at 2.4 GHz.  Two cycles is what
+# FP32 v_add_f32 / v_mul_f32 / v_fmac_f32 chains measure on gfx950 (`--workload valu`:
+# 2.29-2.32 cycles per instruction including the launch ramp); v_fma_f32 with two
+# scalar operands takes four (4.24), so a kernel's own floor lies between the two.
+VALU_PEAK_GINST = 1024 * 2.4 / 2
 
 # Counters of the headline's dominant kernel from its rocprofv3 profile (per launch):
 # VALU wave-instructions (SQ_INSTS_VALU) and HBM bytes (FETCH_SIZE x 2 + WRITE_SIZE x 1,
@@ -423,7 +426,10 @@ def hvp_tile_bytes(op, dens_ncomp=3, cached_lin=0, geo=False, defer_n=0, mapped=
     CG step's deferred update (defer_n entries of the CG vectors) the launch
     also streams z, p and x in and p and x out: 20 bytes per entry.  Without a
     dof map (the PCG on full-length vectors: mapped False) the tangent is read
-    by node and no dof-map entries are loaded."""
+    by node and no dof-map entries are loaded.  Hex8 and Tet4 tiles sum by
+    per-tile incidence lists (a uint16 entry per element node, a uint16 list
+    end per slot) and read only the tile's element count of its color
+    offsets; Tri3 tiles read every color offset."""
     lay = op.device_layout()
     E, S, T = lay["n_elements"], lay["n_slots"], lay["n_tiles"]
     nne = 8 if op.element.kind == 2 else (4 if op.element.kind == 1 else 3)
@@ -432,7 +438,11 @@ def hvp_tile_bytes(op, dens_ncomp=3, cached_lin=0, geo=False, defer_n=0, mapped=
     b = 2 * nne * E + 4 * cached_lin * lay["n_quad"] * E + (44 * E if geo else 0)
     b += S * (4 + (0 if geo else 8 * d) + 4 * nc * (0 if cached_lin else 1)
               + (4 if mapped else 0) * nc + 4 * nc + 4 * nc + 4)
-    b += T * (2 * (lay["max_colors"] + 1) + 8)
+    incidence = nne in (4, 8)
+    if incidence:
+        b += 2 * nne * E + 2 * S + T * (2 + 8)
+    else:
+        b += T * (2 * (lay["max_colors"] + 1) + 8)
     b += 20 * defer_n
     return int(b), lay
 
@@ -631,7 +641,9 @@ def headline(args):
             "note": "VALU-issue bound: achieved = the kernel's VALU wave-instructions per launch "
                     "(rocprofv3 SQ_INSTS_VALU of this workload, in the profile named in "
                     "traffic_source) / the launch time measured here with HIP events; peak = one "
-                    "wave64 instruction per SIMD per 4 cycles x 1,024 SIMDs x 2.4 GHz.  "
+                    "wave64 instruction per SIMD per 2 cycles x 1,024 SIMDs x 2.4 GHz (the rate "
+                    "FP32 VOP2 add/mul/fmac chains measure; v_fma_f32 with scalar operands issues "
+                    "at half of it).  "
                     "hbm_achieved_gbs = algorithmic bytes / the same time",
         },
         "cpu_baseline": None,
@@ -779,7 +791,7 @@ def main(argv=None):
     ap.add_argument("--nh-steps", type=int, default=2, help="load steps, Neo-Hookean comparison")
     ap.add_argument("--nh-stretch", type=float, default=0.1, help="final u_x on x = 1")
     ap.add_argument("--workload", choices=("headline", "c2", "c2mixed", "c3", "c4", "c5", "nh48",
-                                           "copy"),
+                                           "copy", "valu"),
                     default="headline",
                     help="headline: the default line (Hex8 Neo-Hookean 160^3 mixed Newton, "
                          "FP64 path alongside); c2: the FP64 Hex8 elasticity 64^3 line; "
@@ -821,6 +833,32 @@ def main(argv=None):
         torch.cuda.synchronize()
         print(json.dumps({"workload": "copy", "bytes_read": src.numel() * 8,
                           "bytes_written": dst.numel() * 8, "launches": 10}), flush=True)
+        return 0
+    if args.workload == "valu":
+        # calibration of the VALU issue rate: FP32 chains of one instruction form
+        # (0 v_fma_f32, 1 v_fmac_f32, 2 v_add_f32, 3 v_mul_f32), 8 waves per SIMD
+        from tatva_amd import _native
+        blocks, iters, launches = 2048, 4096, 10
+        out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+        st = torch.cuda.current_stream()
+        for op, form in enumerate(("v_fma_f32", "v_fmac_f32", "v_add_f32", "v_mul_f32")):
+            run = lambda: _native.check(_native.lib().tatva_valu_calibration(
+                op, blocks, iters, out.data_ptr(), st.cuda_stream))
+            run()
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record(st)
+            for _ in range(launches):
+                run()
+            t1.record(st)
+            torch.cuda.synchronize()
+            if not bool(torch.isfinite(out).all()):
+                raise RuntimeError(f"calibration {form} produced non-finite values")
+            us = t0.elapsed_time(t1) * 1e3 / launches
+            n = blocks * 4 * 32 * iters
+            print(json.dumps({"workload": "valu", "form": form, "op": op, "blocks": blocks,
+                              "iters": iters, "launches": launches + 1,
+                              "loop_wave_instructions": n, "avg_launch_us": us,
+                              "g_wave_instr_per_s": n / us / 1e3}), flush=True)
         return 0
     if args.workload == "headline":
         if args.n is None:

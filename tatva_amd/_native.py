@@ -112,6 +112,7 @@ _SIGS = {
     "tatva_cast_f32_f64": (c_int, [c_i64, c_dbl, c_void, c_void, c_void]),
     "tatva_dot_workspace": (c_i64, [c_i64]),
     "tatva_dot_noalloc": (c_int, [c_i64, c_void, c_void, c_void, c_void]),
+    "tatva_valu_calibration": (c_int, [c_i32, c_i32, c_i32, c_void, c_void]),
     "tatva_mass_matrix_mapped": (c_int, [c_void, c_i32, c_void, c_void, c_void, c_i64, c_void,
                                          c_i32, c_void, c_void, c_i64, c_i32, c_void, c_void,
                                          c_i64, c_void]),
