@@ -836,12 +836,15 @@ def main(argv=None):
         return 0
     if args.workload == "valu":
         # calibration of the VALU issue rate: FP32 chains of one instruction form
-        # (0 v_fma_f32, 1 v_fmac_f32, 2 v_add_f32, 3 v_mul_f32), 8 waves per SIMD
+        # (0 v_fma_f32 s, s; 1 v_fmac_f32; 2 v_add_f32; 3 v_mul_f32; 4 v_fma_f32 v, v;
+        # 5 v_fmac_f32 s), 8 waves per SIMD
         from tatva_amd import _native
         blocks, iters, launches = 2048, 4096, 10
         out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
         st = torch.cuda.current_stream()
-        for op, form in enumerate(("v_fma_f32", "v_fmac_f32", "v_add_f32", "v_mul_f32")):
+        for op, form in enumerate(("v_fma_f32 (scalar a, b)", "v_fmac_f32", "v_add_f32",
+                                   "v_mul_f32", "v_fma_f32 (vector a, b)",
+                                   "v_fmac_f32 (scalar a)")):
             run = lambda: _native.check(_native.lib().tatva_valu_calibration(
                 op, blocks, iters, out.data_ptr(), st.cuda_stream))
             run()

@@ -465,8 +465,9 @@ int tatva_check_distance2_colors(int64_t n, const int32_t *indptr, const int32_t
 
 /* VALU issue-rate calibration: `blocks` workgroups of 256 threads, each thread
  * running eight independent FP32 chains for 32 x iters instructions of one
- * form (op 0: v_fma_f32 with scalar operands, 1: v_fmac_f32 with vector
- * operands, 2: v_add_f32, 3: v_mul_f32), one float per thread written to out
+ * form (op 0: v_fma_f32 with two scalar operands, 1: v_fmac_f32 with vector
+ * operands, 2: v_add_f32, 3: v_mul_f32, 4: v_fma_f32 with vector operands,
+ * 5: v_fmac_f32 with one scalar operand), one float per thread written to out
  * (blocks x 256).  No counterpart in the reference: it
  * gives the roofline of the VALU-bound FP32 tile kernels a measured ceiling
  * (SQ_INSTS_VALU of this kernel over its duration), as a plain copy does for
