@@ -638,12 +638,12 @@ def headline(args):
             "operator_us": op_us,
             "samples": op_n,
             "layout": lay,
-            "note": "VALU-issue bound: achieved = the kernel's VALU wave-instructions per launch "
-                    "(rocprofv3 SQ_INSTS_VALU of this workload, in the profile named in "
-                    "traffic_source) / the launch time measured here with HIP events; peak = one "
-                    "wave64 instruction per SIMD per 2 cycles x 1,024 SIMDs x 2.4 GHz (the rate "
-                    "FP32 VOP2 add/mul/fmac chains measure; v_fma_f32 with scalar operands issues "
-                    "at half of it).  "
+            "note": "VALU roofline (neither HBM nor MFMA bounds this kernel): achieved = the "
+                    "kernel's VALU wave-instructions per launch (rocprofv3 SQ_INSTS_VALU of this "
+                    "workload, in the profile named in traffic_source) / the launch time measured "
+                    "here with HIP events; peak = one wave64 instruction per SIMD per 2 cycles x "
+                    "1,024 SIMDs x 2.4 GHz (the rate FP32 chains with vector operands measure; an "
+                    "instruction with a scalar operand issues at half of it).  "
                     "hbm_achieved_gbs = algorithmic bytes / the same time",
         },
         "cpu_baseline": None,
