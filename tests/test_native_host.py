@@ -427,3 +427,37 @@ def test_solver_options_take_their_documented_values_only():
     finally:
         for opt, v in defaults.items():
             assert L.tatva_set_option(opt, v) == nat.OK
+
+
+def test_hvp_tile_bytes_counts_every_array_once():
+    """bench.hvp_tile_bytes (the roofline's algorithmic bytes) against a count
+    written per array: Hex8 and Tet4 tiles read incidence lists (a uint16 entry
+    per element node, a uint16 end per slot) and one color offset per tile;
+    Tri3 tiles read every color offset."""
+    import bench
+
+    class Op:
+        def __init__(self, kind, lay):
+            self.element = type("El", (), {"kind": kind})()
+            self._lay = lay
+
+        def device_layout(self):
+            return dict(self._lay)
+
+    E, S, T, C, n = 4096, 7300, 16, 12, 12_000
+    lay = {"n_elements": E, "n_slots": S, "n_tiles": T, "max_colors": C, "n_quad": 8}
+    # Hex8, geometry cache, full-length vectors, deferred update
+    hex8 = (2 * 8 * E          # local node numbers
+            + 44 * E           # geometry cache entry
+            + S * (4 + 12 + 12 + 12 + 4)  # node id, state, tangent, partial sums, position
+            + 2 * 8 * E + 2 * S  # incidence entries and list ends
+            + T * (2 + 8)      # element count, slot range
+            + 20 * n)          # deferred p, x update
+    assert bench.hvp_tile_bytes(Op(2, lay), geo=True, defer_n=n, mapped=False)[0] == hex8
+    # Tet4, coordinates read (no cache), reduced vectors through the dof map
+    tet4 = (2 * 4 * E + S * (4 + 8 * 3 + 12 + 12 + 12 + 12 + 4) + 2 * 4 * E + 2 * S
+            + T * (2 + 8))
+    assert bench.hvp_tile_bytes(Op(1, lay), geo=False, mapped=True)[0] == tet4
+    # Tri3, two components: every color offset, 2-D coordinates
+    tri3 = (2 * 3 * E + S * (4 + 8 * 2 + 8 + 8 + 8 + 8 + 4) + T * (2 * (C + 1) + 8))
+    assert bench.hvp_tile_bytes(Op(0, lay), dens_ncomp=2, mapped=True)[0] == tri3
